@@ -1,0 +1,117 @@
+/*
+ * kifi.h -- C ABI of the MI355X fusion library (libkifi_hip.so).
+ *
+ * The reference (KinectFission/kifi) exposes a C++ class API with no FFI; this
+ * ABI carries exactly what those entry points take and return, as plain
+ * pointers, sizes and PODs, so that the C++ drop-in classes in
+ * merge_fusion_amd/cpp/kifi/ (and ctypes/cgo-style bindings) sit on top of it.
+ *
+ * Conventions shared with the reference:
+ *   - matrices are float[16] in column storage, element (r, c) at m[r + 4c]
+ *     (util/math.h:59-80, 259-265);
+ *   - depth frames are row-major, pixel (x, y) at y*width + x (util/vector2d.h:88-99);
+ *     they are passed as uint16 millimetres and converted on the device exactly as
+ *     DepthStream.cpp:89-90 does ((float)mm * 0.001f);
+ *   - 32-bit voxel keys are z<<20 | y<<10 | x (util/math.h:442-457); 64-bit keys
+ *     use 21 bits per axis (z<<42 | y<<21 | x) for resolutions above 1024.
+ *
+ * Errors: every call returns a kifi_status; kifi_last_error() gives the text.
+ * No exception crosses the ABI.  One context per host thread; calls on one
+ * context are ordered on its HIP stream, and a call that returns host-visible
+ * data is synchronous.  The library owns device memory; callers own host buffers.
+ */
+#ifndef KIFI_H
+#define KIFI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    KIFI_OK = 0,
+    KIFI_ERR_ARG = 1,      /* invalid argument (the reference asserts)           */
+    KIFI_ERR_HIP = 2,      /* HIP runtime error                                  */
+    KIFI_ERR_NOMEM = 3,    /* device allocation failed                           */
+    KIFI_ERR_CAPACITY = 4, /* caller buffer too small; *count gives the size     */
+    KIFI_ERR_STATE = 5     /* invalid state (e.g. device-side overflow detected) */
+} kifi_status;
+
+typedef struct kifi_ctx kifi_ctx;
+typedef struct kifi_volume kifi_volume;
+
+/* DepthSensorParams (DepthSensorParams.h:23-45): resolution, focal length and
+ * principal point in pixels, sensible range in metres. */
+typedef struct {
+    int   width, height;
+    float fx, fy;
+    float cx, cy;
+    float znear, zfar;
+} kifi_intrinsics;
+
+enum { KIFI_STORAGE_F32 = 0, KIFI_STORAGE_F16U8 = 1 };
+
+/* Volume(resolution, sideLength, truncationMargin) (Volume.h:15-31) plus the
+ * MI355X layout choices: key width and voxel storage. */
+typedef struct {
+    int   res;        /* power of two; <= 1024 for 32-bit keys, <= 2^20 for 64-bit */
+    float side;       /* metres */
+    float trunc;      /* metres */
+    int   key_bits;   /* 32 or 64 */
+    int   storage;    /* KIFI_STORAGE_F32: {float sum, float weight} (Voxel.h:18-20);
+                         KIFI_STORAGE_F16U8: uint32 = binary16 distance | weight << 16 */
+} kifi_volume_desc;
+
+/* ---- context ---- */
+kifi_status kifi_ctx_create(int device, kifi_ctx **out);
+void        kifi_ctx_destroy(kifi_ctx *ctx);
+const char *kifi_last_error(const kifi_ctx *ctx);
+kifi_status kifi_ctx_synchronize(kifi_ctx *ctx);
+/* The HIP stream the context orders its work on (hipStream_t). */
+void       *kifi_ctx_stream(kifi_ctx *ctx);
+
+/* ---- volume: kifi::Volume (Volume.h:15-31, Volume.cpp:12-37) ---- */
+kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *desc, kifi_volume **out);
+void        kifi_volume_destroy(kifi_volume *vol);
+kifi_status kifi_volume_clear(kifi_volume *vol);                  /* flat_map::clear (flat_map.h:158-162) */
+kifi_status kifi_volume_size(kifi_volume *vol, size_t *n);        /* flat_map::size (flat_map.h:190-194) */
+/* Volume::Data() read-out (keys_cbegin / values_cbegin, flat_map.h:86-121):
+ * keys as uint32 or uint64 per key_bits, values as 8 or 4 bytes per storage.
+ * Either pointer may be NULL.  Returns KIFI_ERR_CAPACITY if cap < size. */
+kifi_status kifi_volume_download(kifi_volume *vol, void *keys, void *values, size_t cap, size_t *n);
+/* Replaces the content with n sorted unique keys and their values. */
+kifi_status kifi_volume_upload(kifi_volume *vol, const void *keys, const void *values, size_t n);
+
+/* ---- integration: kifi::Integrator::Integrate (Integrator.h:19-26, Integrator.cpp:26-71) ----
+ * depth_mm: host pointer (kifi_integrate) or device pointer already in HBM
+ * (kifi_integrate_device).  worldToEye: column storage.  Asynchronous. */
+kifi_status kifi_integrate(kifi_volume *vol, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const float worldToEye[16]);
+kifi_status kifi_integrate_device(kifi_volume *vol, const uint16_t *depth_mm_dev,
+                                  const kifi_intrinsics *cam, const float worldToEye[16]);
+/* Counts of the last integration: [0] K keygen keys, [1] 8-neighbourhood candidates,
+ * [2] E keys handed to the merge, [3] volume size after the merge.  Synchronous. */
+kifi_status kifi_integrate_stats(kifi_volume *vol, size_t stats[4]);
+
+/* ---- meshing: kifi::Mesher (Mesher.h:17-18, Mesher.cpp:14-45) ----
+ * kifi_mesh: Mesh(volume, verts) -- float3 vertices in generation order.
+ * kifi_mesh_indexed: Mesh(volume, verts, idx) -- vertices ordered by vertex ID,
+ * indices (3 per triangle) into them.  Synchronous; KIFI_ERR_CAPACITY with the
+ * required counts if a buffer is too small.  *stalled (may be NULL) reports the
+ * reference's join stall (an index ID without a vertex ID). */
+kifi_status kifi_mesh(kifi_volume *vol, float *verts, size_t vcap, size_t *nverts);
+kifi_status kifi_mesh_indexed(kifi_volume *vol, float *verts, size_t vcap, uint32_t *idx,
+                              size_t icap, size_t *nverts, size_t *nidx, int *stalled);
+
+/* ---- model prediction: kifi::Renderer::Bin (Renderer.h:22-29, Renderer.cpp:39-80) ----
+ * z-buffered splat of n float3 points (host) into a width x height float3 map
+ * (host, 0 = empty) with the given worldToClip. Synchronous. */
+kifi_status kifi_bin(kifi_ctx *ctx, const float *points, size_t n, const float worldToClip[16],
+                     int width, int height, float *out_points);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

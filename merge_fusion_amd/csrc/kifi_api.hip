@@ -1,0 +1,387 @@
+// kifi_api.hip -- the extern "C" boundary of libkifi_hip.so (declared in include/kifi.h).
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "kifi_internal.hpp"
+
+namespace kifi {
+
+kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+                               const float w2e[16]);
+kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
+                          size_t *nverts, size_t *nidx, int *stalled);
+kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,
+                     float *out);
+
+kifi_status set_error(kifi_ctx *ctx, kifi_status st, const std::string &msg)
+{
+    if (ctx)
+        ctx->err = msg;
+    return st;
+}
+
+kifi_status hip_error(kifi_ctx *ctx, hipError_t e, const char *what)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s failed: %s", what, hipGetErrorString(e));
+    return set_error(ctx, e == hipErrorOutOfMemory ? KIFI_ERR_NOMEM : KIFI_ERR_HIP, buf);
+}
+
+static kifi_status ensure_buf(kifi_ctx *ctx, void **buf, size_t *have, size_t bytes)
+{
+    if (bytes <= *have)
+        return KIFI_OK;
+    size_t want = bytes + bytes / 4 + (1 << 20);
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (*buf)
+        KIFI_HIP(ctx, hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+    KIFI_HIP(ctx, hipMalloc(buf, want));
+    *have = want;
+    return KIFI_OK;
+}
+
+kifi_status ensure_scratch(kifi_ctx *ctx, size_t bytes)
+{
+    return ensure_buf(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes);
+}
+
+kifi_status ensure_out(kifi_ctx *ctx, size_t bytes)
+{
+    return ensure_buf(ctx, &ctx->out_scratch, &ctx->out_bytes, bytes);
+}
+
+// Grow both buffer pairs of a volume to hold `needed` voxels, keeping content.
+kifi_status grow_volume(kifi_volume *vol, size_t needed)
+{
+    kifi_ctx *ctx = vol->ctx;
+    size_t cap = vol->cap ? vol->cap : (size_t)1 << 20;
+    while (cap < needed)
+        cap = cap + cap / 2;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t n = 0;
+    KIFI_HIP(ctx, hipMemcpy(&n, vol->d_size, 8, hipMemcpyDeviceToHost));
+    void *nk[2] = { nullptr, nullptr }, *nv[2] = { nullptr, nullptr };
+    for (int b = 0; b < 2; ++b) {
+        hipError_t e1 = hipMalloc(&nk[b], cap * vol->key_bytes);
+        hipError_t e2 = e1 == hipSuccess ? hipMalloc(&nv[b], cap * vol->val_bytes) : e1;
+        if (e2 != hipSuccess) {
+            for (int q = 0; q <= b; ++q) {
+                if (nk[q]) (void)hipFree(nk[q]);
+                if (nv[q]) (void)hipFree(nv[q]);
+            }
+            return hip_error(ctx, e2, "hipMalloc(volume)");
+        }
+    }
+    if (n) {
+        KIFI_HIP(ctx, hipMemcpy(nk[0], vol->keys[vol->cur], n * vol->key_bytes, hipMemcpyDeviceToDevice));
+        KIFI_HIP(ctx, hipMemcpy(nv[0], vol->vals[vol->cur], n * vol->val_bytes, hipMemcpyDeviceToDevice));
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (vol->keys[b]) KIFI_HIP(ctx, hipFree(vol->keys[b]));
+        if (vol->vals[b]) KIFI_HIP(ctx, hipFree(vol->vals[b]));
+        vol->keys[b] = nk[b];
+        vol->vals[b] = nv[b];
+    }
+    vol->cur = 0;
+    vol->cap = cap;
+    vol->size_bound = n;
+    return KIFI_OK;
+}
+
+} // namespace kifi
+
+using namespace kifi;
+
+extern "C" {
+
+kifi_status kifi_ctx_create(int device, kifi_ctx **out)
+{
+    if (!out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    kifi_ctx *ctx = new (std::nothrow) kifi_ctx;
+    if (!ctx)
+        return KIFI_ERR_NOMEM;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return e == hipErrorOutOfMemory ? KIFI_ERR_NOMEM : KIFI_ERR_HIP;
+    }
+    *out = ctx;
+    return KIFI_OK;
+}
+
+void kifi_ctx_destroy(kifi_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->out_scratch) (void)hipFree(ctx->out_scratch);
+    if (ctx->d_depth) (void)hipFree(ctx->d_depth);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *kifi_last_error(const kifi_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : "no context";
+}
+
+kifi_status kifi_ctx_synchronize(kifi_ctx *ctx)
+{
+    if (!ctx)
+        return KIFI_ERR_ARG;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KIFI_OK;
+}
+
+void *kifi_ctx_stream(kifi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+static bool pow2(int x) { return x > 0 && !(x & (x - 1)); }
+
+kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *d, kifi_volume **out)
+{
+    if (!ctx || !d || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    // Volume.cpp:22-26 asserts, extended to 64-bit keys
+    const int maxres = d->key_bits == 32 ? 1024 : (1 << 20);
+    if ((d->key_bits != 32 && d->key_bits != 64) || !pow2(d->res) || d->res > maxres || d->res < 2 ||
+        !(d->side > 0.0f) || !(d->trunc > 0.0f) ||
+        (d->storage != KIFI_STORAGE_F32 && d->storage != KIFI_STORAGE_F16U8))
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_volume_create: invalid volume description");
+    KIFI_HIP(ctx, hipSetDevice(ctx->device));
+    kifi_volume *v = new (std::nothrow) kifi_volume;
+    if (!v)
+        return KIFI_ERR_NOMEM;
+    v->ctx = ctx;
+    v->desc = *d;
+    v->key_bytes = d->key_bits / 8;
+    v->val_bytes = d->storage == KIFI_STORAGE_F32 ? 8 : 4;
+    // Volume.cpp:28-36
+    const float minx = -0.5f * d->side;
+    v->consts.res = d->res;
+    v->consts.vl = d->side / (float)d->res;
+    v->consts.vlHalfPlusMin = v->consts.vl * 0.5f + minx;
+    v->consts.vlInv = 1.0f / v->consts.vl;
+    v->consts.minOverNegVl = v->consts.vlInv * -minx;
+    v->consts.trunc = d->trunc;
+    hipError_t e = hipMalloc(&v->d_counters, sizeof(FrameCounters));
+    if (e == hipSuccess) e = hipHostMalloc(&v->h_counters, sizeof(FrameCounters), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&v->d_size, 8);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&v->size_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemcpy(v->d_size, &v->size_bound, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(v->d_counters, 0, sizeof(FrameCounters));
+    if (e != hipSuccess) {
+        kifi_volume_destroy(v);
+        return hip_error(ctx, e, "kifi_volume_create");
+    }
+    std::memset(v->h_counters, 0, sizeof(FrameCounters));
+    kifi_status st = grow_volume(v, (size_t)1 << 20);
+    if (st != KIFI_OK) {
+        kifi_volume_destroy(v);
+        return st;
+    }
+    *out = v;
+    return KIFI_OK;
+}
+
+void kifi_volume_destroy(kifi_volume *v)
+{
+    if (!v)
+        return;
+    (void)hipSetDevice(v->ctx->device);
+    (void)hipStreamSynchronize(v->ctx->stream);
+    for (int b = 0; b < 2; ++b) {
+        if (v->keys[b]) (void)hipFree(v->keys[b]);
+        if (v->vals[b]) (void)hipFree(v->vals[b]);
+    }
+    if (v->d_counters) (void)hipFree(v->d_counters);
+    if (v->h_counters) (void)hipHostFree(v->h_counters);
+    if (v->d_size) (void)hipFree(v->d_size);
+    if (v->size_event) (void)hipEventDestroy(v->size_event);
+    delete v;
+}
+
+static kifi_status sync_size(kifi_volume *v, uint64_t *n)
+{
+    kifi_ctx *ctx = v->ctx;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (v->size_event_pending) {
+        v->size_event_pending = false;
+        if (v->h_counters->overflow)
+            return set_error(ctx, KIFI_ERR_STATE, "device-side capacity overflow in the last integration");
+    }
+    KIFI_HIP(ctx, hipMemcpy(n, v->d_size, 8, hipMemcpyDeviceToHost));
+    v->size_bound = *n;
+    return KIFI_OK;
+}
+
+kifi_status kifi_volume_clear(kifi_volume *v)
+{
+    if (!v)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t zero = 0;
+    KIFI_HIP(ctx, hipMemcpy(v->d_size, &zero, 8, hipMemcpyHostToDevice));
+    v->size_bound = 0;
+    v->size_event_pending = false;
+    return KIFI_OK;
+}
+
+kifi_status kifi_volume_size(kifi_volume *v, size_t *n)
+{
+    if (!v || !n)
+        return KIFI_ERR_ARG;
+    uint64_t s = 0;
+    kifi_status st = sync_size(v, &s);
+    *n = (size_t)s;
+    return st;
+}
+
+kifi_status kifi_volume_download(kifi_volume *v, void *keys, void *values, size_t cap, size_t *n)
+{
+    if (!v || !n)
+        return KIFI_ERR_ARG;
+    uint64_t s = 0;
+    kifi_status st = sync_size(v, &s);
+    if (st != KIFI_OK)
+        return st;
+    *n = (size_t)s;
+    if (cap < s)
+        return set_error(v->ctx, KIFI_ERR_CAPACITY, "kifi_volume_download: buffer too small");
+    if (s && keys)
+        KIFI_HIP(v->ctx, hipMemcpy(keys, v->keys[v->cur], s * v->key_bytes, hipMemcpyDeviceToHost));
+    if (s && values)
+        KIFI_HIP(v->ctx, hipMemcpy(values, v->vals[v->cur], s * v->val_bytes, hipMemcpyDeviceToHost));
+    return KIFI_OK;
+}
+
+kifi_status kifi_volume_upload(kifi_volume *v, const void *keys, const void *values, size_t n)
+{
+    if (!v || (n && (!keys || !values)))
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (n > v->cap) {
+        uint64_t zero = 0;
+        KIFI_HIP(ctx, hipMemcpy(v->d_size, &zero, 8, hipMemcpyHostToDevice));
+        kifi_status st = grow_volume(v, n);
+        if (st != KIFI_OK)
+            return st;
+    }
+    if (n) {
+        KIFI_HIP(ctx, hipMemcpy(v->keys[v->cur], keys, n * v->key_bytes, hipMemcpyHostToDevice));
+        KIFI_HIP(ctx, hipMemcpy(v->vals[v->cur], values, n * v->val_bytes, hipMemcpyHostToDevice));
+    }
+    uint64_t s = n;
+    KIFI_HIP(ctx, hipMemcpy(v->d_size, &s, 8, hipMemcpyHostToDevice));
+    v->size_bound = n;
+    v->size_event_pending = false;
+    return KIFI_OK;
+}
+
+static kifi_status check_cam(kifi_ctx *ctx, const kifi_intrinsics *cam)
+{
+    // Integrator.cpp:35-36, 89-90 asserts
+    if (!cam || cam->width <= 0 || cam->height <= 0 || cam->width % 4 != 0 || !(cam->fx > 0.0f) ||
+        !(cam->fy > 0.0f))
+        return set_error(ctx, KIFI_ERR_ARG, "invalid intrinsics (width must be a positive multiple of 4)");
+    return KIFI_OK;
+}
+
+kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, const kifi_intrinsics *cam,
+                                  const float worldToEye[16])
+{
+    if (!v || !depth_dev || !worldToEye)
+        return KIFI_ERR_ARG;
+    kifi_status st = check_cam(v->ctx, cam);
+    if (st != KIFI_OK)
+        return st;
+    return integrate_dispatch(v, depth_dev, cam, worldToEye);
+}
+
+kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const float worldToEye[16])
+{
+    if (!v || !depth_mm || !worldToEye)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    kifi_status st = check_cam(ctx, cam);
+    if (st != KIFI_OK)
+        return st;
+    const size_t P = (size_t)cam->width * cam->height;
+    if (P > ctx->depth_pixels) {
+        KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->d_depth)
+            KIFI_HIP(ctx, hipFree(ctx->d_depth));
+        ctx->d_depth = nullptr;
+        ctx->depth_pixels = 0;
+        KIFI_HIP(ctx, hipMalloc(&ctx->d_depth, P * 2));
+        ctx->depth_pixels = P;
+    }
+    KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream));
+    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye);
+}
+
+kifi_status kifi_integrate_stats(kifi_volume *v, size_t stats[4])
+{
+    if (!v || !stats)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FrameCounters c;
+    KIFI_HIP(ctx, hipMemcpy(&c, v->d_counters, sizeof c, hipMemcpyDeviceToHost));
+    if (c.overflow)
+        return set_error(ctx, KIFI_ERR_STATE, "device-side capacity overflow in the last integration");
+    stats[0] = c.n_keys;
+    stats[1] = c.n_cand;
+    stats[2] = c.n_new;
+    stats[3] = (size_t)c.v_new;
+    return KIFI_OK;
+}
+
+kifi_status kifi_mesh(kifi_volume *v, float *verts, size_t vcap, size_t *nverts)
+{
+    if (!v || !nverts || (vcap && !verts))
+        return KIFI_ERR_ARG;
+    kifi_status st = mesh_dispatch(v, false, verts, vcap, nullptr, 0, nverts, nullptr, nullptr);
+    if (st == KIFI_ERR_CAPACITY)
+        set_error(v->ctx, st, "kifi_mesh: vertex buffer too small");
+    return st;
+}
+
+kifi_status kifi_mesh_indexed(kifi_volume *v, float *verts, size_t vcap, uint32_t *idx, size_t icap,
+                              size_t *nverts, size_t *nidx, int *stalled)
+{
+    if (!v || !nverts || !nidx || (vcap && !verts) || (icap && !idx))
+        return KIFI_ERR_ARG;
+    kifi_status st = mesh_dispatch(v, true, verts, vcap, idx, icap, nverts, nidx, stalled);
+    if (st == KIFI_ERR_CAPACITY)
+        set_error(v->ctx, st, "kifi_mesh_indexed: buffer too small");
+    return st;
+}
+
+kifi_status kifi_bin(kifi_ctx *ctx, const float *points, size_t n, const float worldToClip[16], int width,
+                     int height, float *out_points)
+{
+    if (!ctx || (n && !points) || !worldToClip || width <= 0 || height <= 0 || !out_points)
+        return KIFI_ERR_ARG;
+    if (n >= 0xffffffffull)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_bin: too many points");
+    return bin_host(ctx, points, n, worldToClip, width, height, out_points);
+}
+
+} // extern "C"

@@ -1,0 +1,770 @@
+// kifi_integrate.hip -- per-frame fusion on the GPU:
+//   keygen (+2x2x2 neighbourhood) -> LSD radix sort -> unique (drop max)
+//   -> merge-path union with the volume fused with the weighted TSDF update.
+//
+// Reference path: Integrator::Integrate (KinectFission/kifi/Integrator.cpp:26-71).
+// The reference sorts the K pixel keys, uniques them and dilates them with
+// three passes (:186-250) whose x pass never emits the largest key's +1
+// neighbour.  The dilated set equals the sorted unique set of all eight
+// 2x2x2-neighbourhood keys of every pixel key minus its single maximum, so one
+// sort of the 8K candidates replaces sort + unique + three unions.
+#include <cfloat>
+
+#include "kifi_internal.hpp"
+#include "kifi_prims.hpp"
+
+namespace kifi {
+
+constexpr int kThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kThreads * kRsItems;   // 4096 keys per radix tile
+constexpr int kMgTile = 4096;                  // merged positions per merge tile
+
+// ===================================================================== reset
+__global__ void k_reset(FrameCounters *c, uint32_t *totals, int n_totals)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
+        c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
+        totals[i] = 0;
+}
+
+// ==================================================================== keygen
+struct KeygenParams {
+    int   W, H;
+    float flInvX, flInvY, ppX, ppY, maxIndex;
+    VolumeConsts vc;
+    Mat4  e2w;
+};
+
+// Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
+// the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
+// 218, 242).  Output order is irrelevant: the sort that follows fixes it.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict__ depth,
+                                                     KeygenParams p, K *__restrict__ out,
+                                                     FrameCounters *__restrict__ cnt)
+{
+    const int P = p.W * p.H;
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    bool valid = false;
+    K key = 0;
+    if (pix < P) {
+        const int v = pix / p.W, u = pix - v * p.W;
+        const float d = (float)depth[pix] * 0.001f;           // DepthStream.cpp:89-90
+        if (d > 0.0f) {
+            float q[4] = { (float)u + 0.0f, 0.0f + -(float)v, 0.0f + -1.0f, 0.0f + 0.0f };
+            q[0] = q[0] * p.flInvX + p.ppX;                    // :126-129
+            q[1] = q[1] * p.flInvY + p.ppY;
+            q[2] = q[2] * 1.0f + 0.0f;
+            q[3] = q[3] * 1.0f + 0.0f;
+            q[0] = q[0] * d + 0.0f;                            // :136-139
+            q[1] = q[1] * d + 0.0f;
+            q[2] = q[2] * d + 0.0f;
+            q[3] = q[3] * d + 1.0f;
+            mat_vec(p.e2w, q, q);                              // :141-144
+            const float ix = q[0] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;   // :146-149
+            const float iy = q[1] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
+            const float iz = q[2] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
+            const float iw = q[3] * 1.0f + 0.0f - 0.5f;
+            valid = ix >= 0.0f && ix < p.maxIndex && iy >= 0.0f && iy < p.maxIndex &&
+                    iz >= 0.0f && iz < p.maxIndex && iw >= 0.0f && iw < p.maxIndex; // :151-154
+            if (valid)
+                key = pack_key<K>((K)(uint32_t)ix, (K)(uint32_t)iy, (K)(uint32_t)iz);
+        }
+    }
+    // wave-aggregated append: one atomic per wavefront
+    const uint64_t m = __ballot(valid);
+    if (m == 0)
+        return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t total = popc64(m);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) {
+        base = atomicAdd(&cnt->n_cand, total * 8u);
+        atomicAdd(&cnt->n_keys, total);
+    }
+    base = __shfl(base, leader, kWave);
+    if (valid) {
+        constexpr K dx = 1, dy = K(1) << KeyTraits<K>::kb, dz = K(1) << (2 * KeyTraits<K>::kb);
+        K *o = out + base + popc64(m & lanemask_lt()) * 8u;
+        o[0] = key;           o[1] = key + dx;
+        o[2] = key + dy;      o[3] = key + dy + dx;
+        o[4] = key + dz;      o[5] = key + dz + dx;
+        o[6] = key + dz + dy; o[7] = key + dz + dy + dx;
+    }
+}
+
+// ================================================================ radix sort
+// LSD radix sort with 8-bit digits (the reference's util::radix_sort,
+// algorithm.h:63-91, is the same digit order and stability contract).
+// Per pass: upsweep (tile digit counts, wave-aggregated LDS histogram),
+// scan (one workgroup per digit), downsweep (stable wave-match ranking,
+// LDS staging, coalesced scatter).
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ keys,
+                                                         const uint32_t *__restrict__ d_n, int shift,
+                                                         uint32_t *__restrict__ counts, int Tb,
+                                                         uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t hist[256];
+    const uint32_t n = *d_n;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    if (base < n) {
+#pragma unroll 4
+        for (int j = 0; j < kRsItems; ++j) {
+            const size_t i = base + (size_t)j * kThreads + threadIdx.x;
+            const bool ok = i < n;
+            const uint32_t d = ok ? (uint32_t)(keys[i] >> shift) & 0xffu : 0u;
+            const uint64_t act = __ballot(ok);
+            const uint64_t m = match_digit(d, 8, act);
+            if (ok && (m & lanemask_lt()) == 0)
+                atomicAdd(&hist[d], popc64(m));
+        }
+    }
+    __syncthreads();
+    const uint32_t c = hist[threadIdx.x];
+    counts[(size_t)threadIdx.x * Tb + blockIdx.x] = c;
+    if (c)
+        atomicAdd(&totals[threadIdx.x], c);
+}
+
+// One workgroup per digit: exclusive scan of that digit's tile counts, offset
+// by the number of keys with smaller digits.
+__global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ counts, int Tb,
+                                                      const uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const int d = blockIdx.x;
+    const uint32_t t = threadIdx.x < (unsigned)d ? totals[threadIdx.x] : 0u;
+    uint32_t carry = block_reduce_sum<kThreads>(t, tmp);
+    uint32_t *row = counts + (size_t)d * Tb;
+    constexpr int IT = 4;
+    for (int start = 0; start < Tb; start += kThreads * IT) {
+        uint32_t loc[IT], sum = 0;
+        const int mb = start + threadIdx.x * IT;
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            loc[q] = mb + q < Tb ? row[mb + q] : 0u;
+            sum += loc[q];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_exclusive_scan<kThreads>(sum, tmp, tot);
+#pragma unroll
+        for (int q = 0; q < IT; ++q) {
+            if (mb + q < Tb)
+                row[mb + q] = run;
+            run += loc[q];
+        }
+        carry += tot;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__ in,
+                                                           K *__restrict__ out,
+                                                           const uint32_t *__restrict__ d_n, int shift,
+                                                           const uint32_t *__restrict__ offsets, int Tb)
+{
+    __shared__ uint32_t wcnt[kThreads / kWave][256];
+    __shared__ uint32_t dstart[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    __shared__ K stage[kRsTile];
+    const uint32_t n = *d_n;
+    const size_t tbase = (size_t)blockIdx.x * kRsTile;
+    if (tbase >= n)
+        return;
+    const int w = threadIdx.x / kWave, lane = __lane_id();
+    for (int q = lane; q < 256; q += kWave)
+        wcnt[w][q] = 0;
+    // Each wave ranks its own contiguous 1024-key sub-tile, item by item, so the
+    // per-wave counters need no barrier (a wave's LDS operations are in order).
+    K keys[kRsItems];
+    uint32_t rank[kRsItems];
+    const size_t wbase = tbase + (size_t)w * (kWave * kRsItems);
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j) {
+        const size_t i = wbase + (size_t)j * kWave + lane;
+        const bool ok = i < n;
+        const K k = ok ? in[i] : K(0);
+        keys[j] = k;
+        const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+        const uint64_t act = __ballot(ok);
+        const uint64_t m = match_digit(d, 8, act);
+        const uint32_t before = popc64(m & lanemask_lt());
+        uint32_t old = 0;
+        if (ok)
+            old = wcnt[w][d];
+        rank[j] = old + before;
+        if (ok && before == 0)
+            wcnt[w][d] = old + popc64(m);
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t c[kThreads / kWave], acc = 0;
+#pragma unroll
+        for (int q = 0; q < kThreads / kWave; ++q) {
+            c[q] = wcnt[q][d];
+            wcnt[q][d] = acc;
+            acc += c[q];
+        }
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kThreads>(acc, tmp, tot);
+        dstart[d] = ex;
+        gbase[d] = offsets[(size_t)d * Tb + blockIdx.x] - ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j) {
+        const size_t i = wbase + (size_t)j * kWave + lane;
+        if (i < n) {
+            const uint32_t d = (uint32_t)(keys[j] >> shift) & 0xffu;
+            stage[dstart[d] + wcnt[w][d] + rank[j]] = keys[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t tn = (uint32_t)min((size_t)kRsTile, (size_t)n - tbase);
+    for (uint32_t i = threadIdx.x; i < tn; i += kThreads) {
+        const K k = stage[i];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+        out[gbase[d] + i] = k;
+    }
+}
+
+// ============================================================ unique (E keys)
+// Heads of runs of equal keys; the count of the last unique key is dropped
+// (the reference's x pass never emits the maximum, Integrator.cpp:231-245).
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_unique_count(const K *__restrict__ keys,
+                                                           const uint32_t *__restrict__ d_n,
+                                                           uint32_t *__restrict__ tile_counts)
+{
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const uint32_t n = *d_n;
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    uint32_t c = 0;
+    if (base < n) {
+        const size_t mine = base + (size_t)threadIdx.x * kRsItems;
+        K prev = mine > 0 && mine - 1 < n ? keys[mine - 1] : K(0);
+        for (int q = 0; q < kRsItems; ++q) {
+            const size_t i = mine + q;
+            if (i >= n)
+                break;
+            const K k = keys[i];
+            c += (i == 0 || k != prev);
+            prev = k;
+        }
+    }
+    c = block_reduce_sum<kThreads>(c, tmp);
+    if (threadIdx.x == 0)
+        tile_counts[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_unique_scan(uint32_t *__restrict__ tile_counts, int Tb,
+                                                      FrameCounters *__restrict__ cnt)
+{
+    __shared__ uint32_t total;
+    block_scan_array<1024>(tile_counts, (size_t)Tb, &total);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cnt->n_unique = total;
+        cnt->n_new = total ? total - 1 : 0;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_unique_scatter(const K *__restrict__ keys,
+                                                             const uint32_t *__restrict__ d_n,
+                                                             const uint32_t *__restrict__ tile_off,
+                                                             const FrameCounters *__restrict__ cnt,
+                                                             K *__restrict__ out)
+{
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const uint32_t n = *d_n;
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    if (base >= n)
+        return;
+    const uint32_t E = cnt->n_new;
+    const size_t mine = base + (size_t)threadIdx.x * kRsItems;
+    K loc[kRsItems];
+    uint32_t flags = 0, c = 0;
+    K prev = mine > 0 && mine - 1 < n ? keys[mine - 1] : K(0);
+#pragma unroll
+    for (int q = 0; q < kRsItems; ++q) {
+        const size_t i = mine + q;
+        K k = i < n ? keys[i] : K(0);
+        loc[q] = k;
+        const bool h = i < n && (i == 0 || k != prev);
+        flags |= (uint32_t)h << q;
+        c += h;
+        prev = k;
+    }
+    uint32_t tot;
+    uint32_t pos = tile_off[blockIdx.x] + block_exclusive_scan<kThreads>(c, tmp, tot);
+#pragma unroll
+    for (int q = 0; q < kRsItems; ++q) {
+        if (flags >> q & 1u) {
+            if (pos < E)
+                out[pos] = loc[q];
+            ++pos;
+        }
+    }
+}
+
+// ======================================================== merge + update
+// util::set_union with values (algorithm.h:138-198) as flat_map::insert uses it
+// (flat_map.h:164-197): old volume A first on ties, new keys B get Voxel().
+// Merge path over output diagonals of kMgTile; B elements equal to an A element
+// (dups) are dropped, so each tile's output base is its diagonal minus the
+// dups of all earlier tiles.
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_merge_partition(const K *__restrict__ A,
+                                                              const uint64_t *__restrict__ d_nA,
+                                                              const K *__restrict__ B,
+                                                              FrameCounters *__restrict__ cnt,
+                                                              uint64_t *__restrict__ part_a, int Tbound)
+{
+    const uint64_t nA = *d_nA, nB = cnt->n_new, tot = nA + nB;
+    const uint64_t T = (tot + kMgTile - 1) / kMgTile;
+    const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t == 0) {
+        cnt->v_old = nA;
+        cnt->n_tiles = (uint32_t)T;
+        if (T > (uint64_t)Tbound)
+            cnt->overflow = 1;
+    }
+    if (t > T || t > (uint64_t)Tbound)
+        return;
+    const uint64_t diag = min(t * kMgTile, tot);
+    uint64_t lo = diag > nB ? diag - nB : 0, hi = min(diag, nA);
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[diag - 1 - mid])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    part_a[t] = lo;
+}
+
+// Shared between the dup count and the merge: B[b0+j] is a dup when it equals
+// an A element of the same tile or the A element just before the tile.
+template <typename K>
+struct MergeTile {
+    uint64_t a0, a1, b0, b1;
+    uint32_t na, nb;
+};
+
+template <typename K>
+__device__ __forceinline__ MergeTile<K> merge_tile(const uint64_t *part_a, uint64_t t, uint64_t tot)
+{
+    MergeTile<K> mt;
+    const uint64_t d0 = t * kMgTile, d1 = min(d0 + kMgTile, tot);
+    mt.a0 = part_a[t];
+    mt.a1 = part_a[t + 1];
+    mt.b0 = d0 - mt.a0;
+    mt.b1 = d1 - mt.a1;
+    mt.na = (uint32_t)(mt.a1 - mt.a0);
+    mt.nb = (uint32_t)(mt.b1 - mt.b0);
+    return mt;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_merge_dups(const K *__restrict__ A,
+                                                         const K *__restrict__ B,
+                                                         const uint64_t *__restrict__ part_a,
+                                                         const FrameCounters *__restrict__ cnt,
+                                                         uint32_t *__restrict__ tile_dups)
+{
+    __shared__ K sA[kMgTile];
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const uint64_t T = cnt->n_tiles;
+    if (blockIdx.x >= T || cnt->overflow)
+        return;
+    const uint64_t tot = cnt->v_old + cnt->n_new;
+    const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
+    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads)
+        sA[i] = A[mt.a0 + i];
+    __syncthreads();
+    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads) {
+        const K b = B[mt.b0 + j];
+        const uint32_t lb = lds_lower_bound(sA, mt.na, b);
+        c += (lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b);
+    }
+    c = block_reduce_sum<kThreads>(c, tmp);
+    if (threadIdx.x == 0)
+        tile_dups[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_merge_scan(uint32_t *__restrict__ tile_dups,
+                                                     FrameCounters *__restrict__ cnt,
+                                                     uint64_t *__restrict__ d_size, uint64_t cap)
+{
+    __shared__ uint64_t total;
+    const uint32_t T = cnt->n_tiles;
+    if (cnt->overflow) {
+        return;
+    }
+    block_scan_array<1024>(tile_dups, (size_t)T, &total);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t v1 = cnt->v_old + cnt->n_new - total;
+        cnt->v_new = v1;
+        if (v1 > cap)
+            cnt->overflow = 1;
+        else
+            *d_size = v1;
+    }
+}
+
+struct UpdateParams {
+    Mat4  w2c;
+    float vl, c0, trunc, ndcU, ndcV, maxU, maxV;
+    int   W, H;
+    const uint16_t *depth;
+};
+
+// Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356).  Returns true and
+// the clamped signed distance when the voxel receives an update.
+template <typename K>
+__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, float &sd_out)
+{
+    uint32_t x, y, z;
+    unpack_key<K>(key, x, y, z);
+    float k[4] = { (float)x * p.vl + p.c0, (float)y * p.vl + p.c0, (float)z * p.vl + p.c0,
+                   1.0f };                                         // Volume.cpp:75-78
+    mat_vec(p.w2c, k, k);                                          // :290-293
+    const float kw = k[3];
+    float u = k[0] / kw, v = k[1] / kw, zz = k[2] / kw, ww = k[3] / kw;  // :305-308
+    u = u * p.ndcU + p.ndcU;                                       // :310-313
+    v = v * p.ndcV + p.ndcV;
+    zz = zz * 0.0f + 0.0f;
+    ww = ww * 0.0f + 0.0f;
+    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zz >= -1.0f && zz < 1.0f &&
+        ww >= -FLT_MAX && ww < FLT_MAX) {                          // :315
+        const float d = (float)p.depth[(size_t)(p.H - (unsigned)v - 1) * p.W + (unsigned)u] * 0.001f;
+        const float sd = d - kw;                                   // :319-320
+        if (d > 0.0f && sd >= -p.trunc) {                          // :322
+            sd_out = p.trunc < sd ? p.trunc : sd;                  // std::min, :323
+            return true;
+        }
+    }
+    return false;
+}
+
+struct VoxF32 { float sum, weight; };
+
+__device__ __forceinline__ void voxel_apply(VoxF32 &v, float s)
+{
+    v.sum += s;                                                    // Voxel.h:53-57
+    v.weight++;
+}
+
+// binary16 distance + uint8 weight; binary32 update d' = (d*w + s)/(w + 1)
+__device__ __forceinline__ void voxel_apply(uint32_t &v, float s)
+{
+    const uint32_t w = (v >> 16) & 0xffu;
+    const float d = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+    const float wf = (float)w;
+    const float dn = (d * wf + s) / (wf + 1.0f);
+    const uint32_t wn = w < 255u ? w + 1u : 255u;
+    v = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)dn) | (wn << 16);
+}
+
+template <typename K, typename V>
+__global__ __launch_bounds__(kThreads) void k_merge_update(
+    const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B,
+    const uint64_t *__restrict__ part_a, const uint32_t *__restrict__ tile_base_dups,
+    const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, UpdateParams up)
+{
+    __shared__ K sA[kMgTile];
+    __shared__ K sB[kMgTile];
+    __shared__ uint32_t sDup[kMgTile + 1];
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const uint64_t T = cnt->n_tiles;
+    if (blockIdx.x >= T || cnt->overflow)
+        return;
+    const uint64_t tot = cnt->v_old + cnt->n_new;
+    const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
+    const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
+    const uint64_t obase = (uint64_t)blockIdx.x * kMgTile - tile_base_dups[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads)
+        sA[i] = A[mt.a0 + i];
+    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads)
+        sB[j] = B[mt.b0 + j];
+    __syncthreads();
+    // dup flags of B, then their exclusive prefix (16 consecutive per thread)
+    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
+    constexpr int PER = kMgTile / kThreads;
+    uint32_t fl = 0, c = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t j = threadIdx.x * PER + q;
+        bool dup = false;
+        if (j < mt.nb) {
+            const K b = sB[j];
+            const uint32_t lb = lds_lower_bound(sA, mt.na, b);
+            dup = (lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b);
+        }
+        fl |= (uint32_t)dup << q;
+        c += dup;
+    }
+    uint32_t dtot;
+    uint32_t run = block_exclusive_scan<kThreads>(c, tmp, dtot);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        sDup[threadIdx.x * PER + q] = run;
+        run += fl >> q & 1u;
+    }
+    if (threadIdx.x == 0)
+        sDup[kMgTile] = dtot;
+    __syncthreads();
+    // old voxels: keep their value (first range wins), then update
+    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads) {
+        const K k = sA[i];
+        const uint32_t lb = lds_lower_bound(sB, mt.nb, k);
+        const uint64_t o = obase + i + lb - sDup[lb];
+        V val = Av[mt.a0 + i];
+        float s;
+        if (o < upd_end && voxel_sample<K>(k, up, s))
+            voxel_apply(val, s);
+        O[o] = k;
+        Ov[o] = val;
+    }
+    // new voxels: Voxel() default, then update
+    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads) {
+        const bool dup = (j + 1 <= mt.nb) && (sDup[j + 1] != sDup[j]);
+        if (dup)
+            continue;
+        const K k = sB[j];
+        const uint32_t lb = lds_lower_bound(sA, mt.na, k);
+        const uint64_t o = obase + (j - sDup[j]) + lb;
+        V val{};
+        float s;
+        if (o < upd_end && voxel_sample<K>(k, up, s))
+            voxel_apply(val, s);
+        O[o] = k;
+        Ov[o] = val;
+    }
+}
+
+// ====================================================================== host
+static void host_mat_vec(const float m[16], const float v[4], float out[4])
+{
+    float r[4];
+    for (int i = 0; i < 4; ++i)
+        r[i] = (m[i] * v[0] + m[i + 4] * v[1]) + (m[i + 8] * v[2] + m[i + 12] * v[3]);
+    for (int i = 0; i < 4; ++i)
+        out[i] = r[i];
+}
+
+void host_mat_mul(const float m[16], const float n[16], float out[16])
+{
+    float r[16];
+    for (int c = 0; c < 4; ++c)
+        host_mat_vec(m, n + 4 * c, r + 4 * c);
+    for (int i = 0; i < 16; ++i)
+        out[i] = r[i];
+}
+
+void host_invert_transform(float m[16])          // util/math.h:615-625
+{
+    float t[4] = { -m[12], -m[13], -m[14], 1.0f };
+    m[12] = 0.0f; m[13] = 0.0f; m[14] = 0.0f; m[15] = 1.0f;
+    float s;
+    s = m[1]; m[1] = m[4]; m[4] = s;
+    s = m[2]; m[2] = m[8]; m[8] = s;
+    s = m[6]; m[6] = m[9]; m[9] = s;
+    float c3[4];
+    host_mat_vec(m, t, c3);
+    for (int i = 0; i < 4; ++i)
+        m[12 + i] = c3[i];
+}
+
+void host_eye_to_clip_rh(const kifi_intrinsics *c, float out[16])  // DepthSensorParams.cpp:47-63
+{
+    for (int i = 0; i < 16; ++i)
+        out[i] = 0.0f;
+    const float W = (float)c->width, H = (float)c->height;
+    out[0] = 2.0f * c->fx / W;
+    out[8] = W - 2.0f * c->cx;
+    out[5] = 2.0f * c->fy / H;
+    out[9] = H - 2.0f * c->cy;
+    out[10] = -(c->zfar + c->znear) / (c->zfar - c->znear);
+    out[14] = -2.0f * c->zfar * c->znear / (c->zfar - c->znear);
+    out[11] = -1.0f;
+}
+
+kifi_status grow_volume(kifi_volume *vol, size_t needed);
+
+// scratch layout for one integration (all offsets 256-byte aligned)
+struct IntegrateScratch {
+    void     *candA, *candB;
+    uint32_t *rs_counts;     // [256][Tb]
+    uint32_t *rs_totals;     // [passes][256]
+    uint32_t *uq_tiles;      // [Tb]
+    size_t    bytes;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int passes)
+{
+    IntegrateScratch s{};
+    const size_t ncand = 8 * P;
+    const size_t Tb = (ncand + kRsTile - 1) / kRsTile;
+    size_t off = 0;
+    s.candA = base + off; off += align256(ncand * kbytes);
+    s.candB = base + off; off += align256(ncand * kbytes);
+    s.rs_counts = (uint32_t *)(base + off); off += align256(256 * Tb * 4);
+    s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * 256 * 4);
+    s.uq_tiles = (uint32_t *)(base + off); off += align256((Tb + 1) * 4);
+    s.bytes = off;
+    return s;
+}
+
+template <typename K, typename V>
+static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+                                  const float w2e[16])
+{
+    kifi_ctx *ctx = vol->ctx;
+    const int W = cam->width, H = cam->height;
+    const size_t P = (size_t)W * H;
+    const int passes = KeyTraits<K>::sort_bits / 8;
+    const size_t Tb = (8 * P + kRsTile - 1) / kRsTile;
+
+    // host-side binary32 constants in the reference's order
+    float clip[16], w2c[16], e2w[16];
+    host_eye_to_clip_rh(cam, clip);
+    host_mat_mul(clip, w2e, w2c);                                  // Integrator.cpp:41
+    for (int i = 0; i < 16; ++i)
+        e2w[i] = w2e[i];
+    host_invert_transform(e2w);                                    // :42
+
+    // capacity: the volume can grow by at most 8P voxels per frame
+    if (vol->size_event_pending && hipEventQuery(vol->size_event) == hipSuccess) {
+        vol->size_bound = vol->h_counters->overflow ? vol->size_bound : (size_t)vol->h_counters->v_new;
+        vol->size_event_pending = false;
+    }
+    if (vol->size_bound + 8 * P > vol->cap) {
+        KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        uint64_t exact = 0;
+        KIFI_HIP(ctx, hipMemcpy(&exact, vol->d_size, 8, hipMemcpyDeviceToHost));
+        vol->size_bound = exact;
+        vol->size_event_pending = false;
+        if (exact + 8 * P > vol->cap) {
+            kifi_status st = grow_volume(vol, exact + 8 * P);
+            if (st != KIFI_OK)
+                return st;
+        }
+    }
+    const size_t Tm = (vol->size_bound + 8 * P + kMgTile - 1) / kMgTile + 1;
+
+    // scratch: integration buffers + merge partitions/dups
+    const IntegrateScratch probe = layout_scratch(nullptr, P, sizeof(K), passes);
+    const size_t merge_bytes = align256((Tm + 2) * 8) + align256((Tm + 2) * 4);
+    kifi_status st = ensure_scratch(ctx, probe.bytes + merge_bytes);
+    if (st != KIFI_OK)
+        return st;
+    char *base = (char *)ctx->scratch;
+    IntegrateScratch s = layout_scratch(base, P, sizeof(K), passes);
+    uint64_t *part_a = (uint64_t *)(base + s.bytes);
+    uint32_t *tile_dups = (uint32_t *)(base + s.bytes + align256((Tm + 2) * 8));
+
+    FrameCounters *cnt = vol->d_counters;
+    hipStream_t stream = ctx->stream;
+
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256);
+
+    KeygenParams kp;
+    kp.W = W; kp.H = H;
+    kp.flInvX = 1.0f / cam->fx;
+    kp.flInvY = 1.0f / cam->fy;
+    kp.ppX = (0.5f - cam->cx) * kp.flInvX;                          // Integrator.cpp:95-101
+    kp.ppY = (cam->cy - 0.5f) * kp.flInvY;
+    kp.maxIndex = (float)(vol->desc.res - 1);
+    kp.vc = vol->consts;
+    for (int i = 0; i < 16; ++i)
+        kp.e2w.m[i] = e2w[i];
+    hipLaunchKernelGGL(k_keygen<K>, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       stream, d_depth, kp, (K *)s.candA, cnt);
+
+    // radix sort of the candidates (even pass count: result back in candA)
+    K *src = (K *)s.candA, *dst = (K *)s.candB;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        uint32_t *tot = s.rs_totals + pass * 256;
+        hipLaunchKernelGGL(k_rs_upsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
+                           &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kThreads), 0, stream, s.rs_counts, (int)Tb, tot);
+        hipLaunchKernelGGL(k_rs_downsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, dst,
+                           &cnt->n_cand, shift, s.rs_counts, (int)Tb);
+        K *t = src; src = dst; dst = t;
+    }
+    // unique (minus the maximum) -> E keys in candB
+    hipLaunchKernelGGL(k_unique_count<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
+                       &cnt->n_cand, s.uq_tiles);
+    hipLaunchKernelGGL(k_unique_scan, dim3(1), dim3(1024), 0, stream, s.uq_tiles, (int)Tb, cnt);
+    hipLaunchKernelGGL(k_unique_scatter<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
+                       &cnt->n_cand, s.uq_tiles, cnt, dst);
+    const K *Ekeys = dst;
+
+    // merge + update into the other buffer pair
+    const int cur = vol->cur, nxt = cur ^ 1;
+    hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((Tm + 1 + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, stream, (const K *)vol->keys[cur], vol->d_size, Ekeys, cnt,
+                       part_a, (int)Tm);
+    hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kThreads), 0, stream,
+                       (const K *)vol->keys[cur], Ekeys, part_a, cnt, tile_dups);
+    hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
+                       (uint64_t)vol->cap);
+    UpdateParams up;
+    for (int i = 0; i < 16; ++i)
+        up.w2c.m[i] = w2c[i];
+    up.vl = vol->consts.vl;
+    up.c0 = vol->consts.vlHalfPlusMin;
+    up.trunc = vol->consts.trunc;
+    up.ndcU = (float)W * 0.5f;                                     // Integrator.cpp:262
+    up.ndcV = (float)H * 0.5f;
+    up.maxU = (float)W;                                            // :265
+    up.maxV = (float)H;
+    up.W = W; up.H = H;
+    up.depth = d_depth;
+    hipLaunchKernelGGL((k_merge_update<K, V>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
+                       (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
+                       cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up);
+    KIFI_HIP(ctx, hipGetLastError());
+    vol->cur = nxt;
+
+    // async read-back of the counters to refresh the host size bound later
+    KIFI_HIP(ctx, hipMemcpyAsync(vol->h_counters, cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, stream));
+    KIFI_HIP(ctx, hipEventRecord(vol->size_event, stream));
+    vol->size_event_pending = true;
+    vol->size_bound += 8 * P;
+    return KIFI_OK;
+}
+
+kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+                               const float w2e[16])
+{
+    if (vol->desc.key_bits == 32) {
+        if (vol->desc.storage == KIFI_STORAGE_F32)
+            return integrate_impl<uint32_t, VoxF32>(vol, d_depth, cam, w2e);
+        return integrate_impl<uint32_t, uint32_t>(vol, d_depth, cam, w2e);
+    }
+    if (vol->desc.storage == KIFI_STORAGE_F32)
+        return integrate_impl<uint64_t, VoxF32>(vol, d_depth, cam, w2e);
+    return integrate_impl<uint64_t, uint32_t>(vol, d_depth, cam, w2e);
+}
+
+} // namespace kifi

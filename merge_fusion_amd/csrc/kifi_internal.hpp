@@ -1,0 +1,160 @@
+// kifi_internal.hpp -- shared device helpers and host-side state of libkifi_hip.so.
+//
+// CDNA4 (gfx950) only: 64-lane wavefronts, 64-bit ballots, LDS per workgroup.
+// Everything here is compiled with -ffp-contract=off so that binary32 results
+// match the oracle's source-order arithmetic bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "kifi.h"
+
+namespace kifi {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- wave64 ----
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
+
+// Lanes of the wave (among `active`) whose `bits` low bits of `d` equal this lane's.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, uint64_t active)
+{
+    uint64_t m = active;
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t on = __ballot((d >> b) & 1u);
+        m &= ((d >> b) & 1u) ? on : ~on;
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------ keys ----
+template <typename K> struct KeyTraits;
+template <> struct KeyTraits<uint32_t> {
+    static constexpr int kb = 10;       // bits per axis (util/math.h:442-457)
+    static constexpr int sort_bits = 32;
+};
+template <> struct KeyTraits<uint64_t> {
+    static constexpr int kb = 21;
+    static constexpr int sort_bits = 64;
+};
+
+template <typename K>
+__host__ __device__ __forceinline__ K pack_key(K x, K y, K z)
+{
+    constexpr int kb = KeyTraits<K>::kb;
+    return z << (2 * kb) | y << kb | x;
+}
+
+template <typename K>
+__host__ __device__ __forceinline__ void unpack_key(K k, uint32_t &x, uint32_t &y, uint32_t &z)
+{
+    constexpr int kb = KeyTraits<K>::kb;
+    constexpr K m = (K(1) << kb) - 1;
+    x = (uint32_t)(k & m);
+    y = (uint32_t)(k >> kb & m);
+    z = (uint32_t)(k >> (2 * kb));
+}
+
+// ------------------------------------------------------------ parameters ----
+// Host-precomputed binary32 constants, produced in the reference's order
+// (Integrator.cpp:41-42, 94-105; Volume.cpp:28-36) and passed by value.
+struct VolumeConsts {
+    int   res;
+    float vl;             // voxel length
+    float vlHalfPlusMin;  // vl * 0.5 + min
+    float vlInv;          // 1 / vl
+    float minOverNegVl;   // vlInv * -min
+    float trunc;
+};
+
+struct Mat4 {
+    float m[16];          // column storage
+};
+
+// (col0*x + col1*y) + (col2*z + col3*w), util/math.h:362-365, 381-397
+__device__ __forceinline__ void mat_vec(const Mat4 &M, const float v[4], float out[4])
+{
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        r[i] = (M.m[i] * v[0] + M.m[i + 4] * v[1]) + (M.m[i + 8] * v[2] + M.m[i + 12] * v[3]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        out[i] = r[i];
+}
+
+// Reset list for per-frame device counters (zeroed by a fill kernel, never by a
+// memset node, so that the integration sequence can be captured in a graph).
+struct FrameCounters {
+    uint32_t n_cand;      // candidates written by keygen (8 per valid pixel)
+    uint32_t n_keys;      // valid pixels (K)
+    uint32_t n_unique;    // unique candidates
+    uint32_t n_new;       // E = n_unique - 1 (the reference's x-pass drops the max)
+    uint64_t v_old;       // V0
+    uint64_t v_new;       // V1
+    uint32_t n_tiles;     // merge tiles this frame
+    uint32_t overflow;    // set when a device-side capacity check fails
+};
+
+} // namespace kifi
+
+// ------------------------------------------------------------ host state ----
+struct kifi_ctx {
+    int          device = 0;
+    hipStream_t  stream = nullptr;
+    std::string  err;
+    // reusable device scratch, grown on demand
+    void        *scratch = nullptr;
+    size_t       scratch_bytes = 0;
+    void        *out_scratch = nullptr;  // output staging (mesh, model maps)
+    size_t       out_bytes = 0;
+    uint16_t    *d_depth = nullptr;   // one frame in mm
+    size_t       depth_pixels = 0;
+    int          num_cus = 0;
+};
+
+struct kifi_volume {
+    kifi_ctx        *ctx = nullptr;
+    kifi_volume_desc desc{};
+    kifi::VolumeConsts consts{};
+    size_t           key_bytes = 4, val_bytes = 8;
+    void            *keys[2] = { nullptr, nullptr };
+    void            *vals[2] = { nullptr, nullptr };
+    int              cur = 0;         // which buffer pair holds the volume
+    size_t           cap = 0;         // capacity of each buffer (voxels)
+    size_t           size_bound = 0;  // host upper bound of the device-side size
+    kifi::FrameCounters *d_counters = nullptr;   // device
+    kifi::FrameCounters *h_counters = nullptr;   // pinned host mirror
+    uint64_t        *d_size = nullptr;           // device-resident volume size
+    hipEvent_t       size_event = nullptr;       // marks the last async size read-back
+    bool             size_event_pending = false;
+};
+
+namespace kifi {
+
+// error helpers
+kifi_status set_error(kifi_ctx *ctx, kifi_status st, const std::string &msg);
+kifi_status hip_error(kifi_ctx *ctx, hipError_t e, const char *what);
+
+#define KIFI_HIP(ctx, call)                                          \
+    do {                                                             \
+        hipError_t e_ = (call);                                      \
+        if (e_ != hipSuccess)                                        \
+            return ::kifi::hip_error((ctx), e_, #call);              \
+    } while (0)
+
+// scratch allocators (grow; contents not preserved across a growth)
+kifi_status ensure_scratch(kifi_ctx *ctx, size_t bytes);
+kifi_status ensure_out(kifi_ctx *ctx, size_t bytes);
+
+} // namespace kifi

@@ -1,0 +1,529 @@
+// kifi_mesh.hip -- marching cubes over the sorted sparse volume and the
+// z-buffered point splat (model prediction).
+//
+// Reference: Mesher::Generate<bool> / Mesher::Mesh (KinectFission/kifi/Mesher.cpp:14-45,
+// 85-229) and Renderer::Bin (Renderer.cpp:39-80).
+//
+// The reference sweeps voxels i = V-2 .. 0 with three monotone pointers whose
+// value for voxel i is min(V-2, upper_bound(key_i + delta) - 1); here every
+// voxel computes them independently with a wave-cooperative windowed search.
+// Outputs are placed by scans of per-voxel counts: the generation order of the
+// reference is the descending voxel order, and its vertex-ID order (3*key+axis)
+// is the ascending voxel order, so the vertex/index join needs no sort.
+#include <cfloat>
+
+#include "kifi_internal.hpp"
+#include "kifi_prims.hpp"
+
+namespace kifi {
+
+constexpr int kMT = 256;
+constexpr int kScanTile = 4096;
+
+// mc_tables.inc: kTriOffsets[257], kTriTable[2460] in __constant__ memory
+#include "mc_tables.inc"
+
+// ------------------------------------------------------------- device scan
+// Exclusive scan of n uint32 counts into out (may alias), total to *total.
+__global__ __launch_bounds__(kMT) void k_scan_reduce(const uint32_t *__restrict__ in, uint64_t n,
+                                                     uint32_t *__restrict__ tile_sums)
+{
+    __shared__ uint32_t tmp[kMT / kWave + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (int q = 0; q < kScanTile / kMT; ++q) {
+        const uint64_t i = base + (uint64_t)q * kMT + threadIdx.x;
+        if (i < n)
+            s += in[i];
+    }
+    s = block_reduce_sum<kMT>(s, tmp);
+    if (threadIdx.x == 0)
+        tile_sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t *__restrict__ tile_sums, uint64_t nt,
+                                                     uint64_t *__restrict__ total)
+{
+    block_scan_array<1024>(tile_sums, (size_t)nt, total);
+}
+
+__global__ __launch_bounds__(kMT) void k_scan_apply(const uint32_t *__restrict__ in, uint64_t n,
+                                                    const uint32_t *__restrict__ tile_sums,
+                                                    uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t tmp[kMT / kWave + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    constexpr int PER = kScanTile / kMT;
+    const uint64_t mine = base + (uint64_t)threadIdx.x * PER;
+    uint32_t loc[PER], s = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        loc[q] = mine + q < n ? in[mine + q] : 0u;
+        s += loc[q];
+    }
+    uint32_t tot;
+    uint32_t run = tile_sums[blockIdx.x] + block_exclusive_scan<kMT>(s, tmp, tot);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if (mine + q < n)
+            out[mine + q] = run;
+        run += loc[q];
+    }
+}
+
+static void device_scan(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n,
+                        uint32_t *tile_sums, uint64_t *total)
+{
+    const uint64_t nt = (n + kScanTile - 1) / kScanTile;
+    if (nt == 0) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, (uint64_t)0, total);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kMT), 0, st, in, n, tile_sums);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_sums, nt, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kMT), 0, st, in, n, tile_sums, out);
+}
+
+// ------------------------------------------------------------ voxel access
+struct VoxF32m { float sum, weight; };
+
+__device__ __forceinline__ float vweight(const VoxF32m &v) { return v.weight; }
+__device__ __forceinline__ float vdist(const VoxF32m &v) { return v.sum / v.weight; } // Voxel.h:39-44
+__device__ __forceinline__ float vweight(uint32_t v) { return (float)((v >> 16) & 0xffu); }
+__device__ __forceinline__ float vdist(uint32_t v)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+}
+
+template <typename K>
+__device__ __forceinline__ uint64_t upper_bound_g(const K *keys, uint64_t lo, uint64_t hi, K t)
+{
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] <= t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// min(V-2, upper_bound(t) - 1) for monotone targets across the wave's lanes:
+// lanes 0 and 63 search the whole array, the others their window.
+template <typename K>
+__device__ __forceinline__ int64_t wave_walk(const K *keys, uint64_t V, K t)
+{
+    const int lane = __lane_id();
+    uint64_t u = 0;
+    if (lane == 0 || lane == kWave - 1)
+        u = upper_bound_g(keys, 0, V, t);
+    const uint64_t lo = __shfl(u, 0, kWave), hi = __shfl(u, kWave - 1, kWave);
+    u = upper_bound_g(keys, lo, hi, t);
+    int64_t r = (int64_t)u - 1;
+    const int64_t cl = (int64_t)V - 2;
+    return r < cl ? r : cl;
+}
+
+struct MeshParams {
+    float vl, c0;
+};
+
+template <typename K>
+struct Corners {
+    int64_t c[8];
+};
+
+// Neighbour search and vertex flags of voxel i (Mesher.cpp:117-168).
+// All lanes of the wave must call this with ascending i.
+template <typename K, typename V, bool TRIS>
+__device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals, uint64_t n, uint64_t i_raw,
+                                                   Corners<K> &cn, int &lut)
+{
+    constexpr K d1 = 1, d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
+    const bool active = i_raw + 2 <= n;                 // i <= V-2
+    const uint64_t i = active ? i_raw : (n >= 2 ? n - 2 : 0);
+    const K k = keys[i];
+    const int64_t iTop = wave_walk<K>(keys, n, k + d2);           // :122-123
+    const int64_t iFront = wave_walk<K>(keys, n, k + d4);         // :125-126
+    int64_t iFrontTop = 0;
+    if (TRIS)
+        iFrontTop = wave_walk<K>(keys, n, k + d4 + d2);           // :174-175
+    lut = -1;
+    if (!active)
+        return 0;
+    const V v0 = vals[i];
+    if (vweight(v0) == 0.0f)                                       // :119-120
+        return 0;
+    cn.c[0] = (int64_t)i;
+    cn.c[1] = keys[i + 1] == k + d1 ? (int64_t)i + 1 : -1;         // :132
+    cn.c[2] = keys[iTop] == k + d2 ? iTop : -1;                    // :129
+    cn.c[4] = keys[iFront] == k + d4 ? iFront : -1;                // :130
+    const float dself = vdist(v0);
+    uint32_t flags = 0;
+    const int axis_corner[3] = { 1, 2, 4 };
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {                                  // :140-168
+        const int64_t c = cn.c[axis_corner[a]];
+        if (c >= 0) {
+            const V vn = vals[c];
+            if (vweight(vn) > 0.0f && dself * vdist(vn) <= 0.0f)
+                flags |= 1u << a;
+        }
+    }
+    if (TRIS) {
+        cn.c[6] = keys[iFrontTop] == k + d4 + d2 ? iFrontTop : -1; // :177
+        cn.c[3] = keys[iTop + 1] == k + d2 + d1 ? iTop + 1 : -1;   // :179
+        cn.c[5] = keys[iFront + 1] == k + d4 + d1 ? iFront + 1 : -1;
+        cn.c[7] = keys[iFrontTop + 1] == k + d4 + d2 + d1 ? iFrontTop + 1 : -1;
+        bool skip = false;
+        float d[8];
+        d[0] = dself;
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {                              // :183-188
+            if (cn.c[c] < 0) {
+                skip = true;
+                d[c] = 0.0f;
+                continue;
+            }
+            const V vc = vals[cn.c[c]];
+            if (vweight(vc) == 0.0f)
+                skip = true;
+            d[c] = skip ? 0.0f : vdist(vc);
+        }
+        if (!skip)                                                 // :191-199
+            lut = (d[0] < 0.0f) << 2 | (d[1] < 0.0f) << 3 | (d[2] < 0.0f) << 6 |
+                  (d[3] < 0.0f) << 7 | (d[4] < 0.0f) << 1 | (d[5] < 0.0f) << 0 |
+                  (d[6] < 0.0f) << 5 | (d[7] < 0.0f) << 4;
+    }
+    return flags;
+}
+
+// Pass 1: per-voxel vertex flags and counts.
+template <typename K, typename V, bool TRIS>
+__global__ __launch_bounds__(kMT) void k_mesh_count(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                    uint64_t n, uint8_t *__restrict__ vflags,
+                                                    uint32_t *__restrict__ cnt_v_fwd,
+                                                    uint32_t *__restrict__ cnt_v_rev,
+                                                    uint32_t *__restrict__ cnt_i_rev)
+{
+    const uint64_t m = n >= 2 ? n - 1 : 0;              // voxels 0 .. V-2
+    const uint64_t stride = (uint64_t)gridDim.x * kMT;
+    for (uint64_t base = (uint64_t)blockIdx.x * kMT; base < m; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        Corners<K> cn;
+        int lut;
+        const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, cn, lut);
+        if (i < m) {
+            const uint32_t nv = __popc(f);
+            vflags[i] = (uint8_t)f;
+            if (cnt_v_fwd)
+                cnt_v_fwd[i] = nv;
+            if (cnt_v_rev)
+                cnt_v_rev[m - 1 - i] = nv;
+            if (TRIS)
+                cnt_i_rev[m - 1 - i] = lut >= 0 ? (uint32_t)(kTriOffsets[lut + 1] - kTriOffsets[lut]) : 0u;
+        }
+    }
+}
+
+// Edge e of the cube -> (corner, axis), from localToGlobal (Mesher.cpp:206-218).
+__constant__ int kEdgeCorner[12] = { 4, 0, 0, 1, 6, 2, 2, 3, 5, 4, 0, 1 };
+__constant__ int kEdgeAxis[12] = { 0, 2, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1 };
+// the coordinate offsets of that corner (dx, dy, dz)
+__constant__ int kCornerOff[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1, 1, 0 },
+                                      { 0, 0, 1 }, { 1, 0, 1 }, { 0, 1, 1 }, { 1, 1, 1 } };
+
+// Pass 2: emit vertices and (optionally) indices at their scanned positions.
+template <typename K, typename V, bool TRIS>
+__global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                   uint64_t n, MeshParams mp,
+                                                   const uint8_t *__restrict__ vflags,
+                                                   const uint32_t *__restrict__ voff,   // fwd (TRIS) or rev
+                                                   const uint32_t *__restrict__ ioff_rev,
+                                                   float *__restrict__ verts, uint32_t *__restrict__ idx,
+                                                   uint64_t *__restrict__ raw_ids,
+                                                   unsigned long long *__restrict__ stall_id)
+{
+    const uint64_t m = n >= 2 ? n - 1 : 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kMT;
+    constexpr int kb = KeyTraits<K>::kb;
+    for (uint64_t base = (uint64_t)blockIdx.x * kMT; base < m; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        Corners<K> cn;
+        int lut;
+        const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, cn, lut);
+        if (i >= m)
+            continue;
+        if (f) {
+            const K k = keys[i];
+            uint32_t x0, y0, z0;
+            unpack_key<K>(k, x0, y0, z0);
+            // Volume::VoxelCenter(int, int, int), Volume.cpp:80-88
+            const float vx = (float)(int)x0 * mp.vl + mp.c0, vy = (float)(int)y0 * mp.vl + mp.c0,
+                        vz = (float)(int)z0 * mp.vl + mp.c0;
+            const V v0 = vals[i];
+            const float dself = vdist(v0);
+            uint32_t o = TRIS ? voff[i] : voff[m - 1 - i];
+            const int axis_corner[3] = { 1, 2, 4 };
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (!(f >> a & 1u))
+                    continue;
+                const float dn = vdist(vals[cn.c[axis_corner[a]]]);
+                const float t = fabsf(dself) / (fabsf(dself) + fabsf(dn)) * mp.vl; // :143/153/163
+                verts[3 * (size_t)o + 0] = a == 0 ? vx + t : vx;
+                verts[3 * (size_t)o + 1] = a == 1 ? vy + t : vy;
+                verts[3 * (size_t)o + 2] = a == 2 ? vz + t : vz;
+                ++o;
+            }
+        }
+        if (TRIS && lut >= 0) {
+            const K k = keys[i];
+            uint32_t x0, y0, z0;
+            unpack_key<K>(k, x0, y0, z0);
+            uint32_t o = ioff_rev[m - 1 - i];
+            for (int t = kTriOffsets[lut]; t < kTriOffsets[lut + 1]; ++t, ++o) {
+                const int e = kTriTable[t];
+                const int c = kEdgeCorner[e], a = kEdgeAxis[e];
+                // the reference's vertex ID: pack(x0+dx, y0+dy, z0+dz) * 3 + axis (OR-pack)
+                const K pk = pack_key<K>((K)x0 + kCornerOff[c][0], (K)y0 + kCornerOff[c][1],
+                                         (K)z0 + kCornerOff[c][2]);
+                int64_t vi = cn.c[c];
+                if (keys[vi] != pk) {        // only where x0+1 or y0+1 carries into the next field
+                    const uint64_t ub = upper_bound_g(keys, 0, n, pk);
+                    vi = (ub > 0 && keys[ub - 1] == pk) ? (int64_t)ub - 1 : -1;
+                }
+                // a vertex exists only for voxels <= V-2 that emitted along this axis
+                const bool have = vi >= 0 && (uint64_t)vi + 2 <= n && (vflags[vi] >> a & 1u);
+                const uint64_t rid = 3ull * (uint64_t)pk + (uint64_t)a;
+                raw_ids[o] = rid;
+                if (have) {
+                    const uint32_t fl = vflags[vi];
+                    idx[o] = voff[vi] + __popc(fl & ((1u << a) - 1u));
+                } else {
+                    idx[o] = (uint32_t)rid;
+                    atomicMin(stall_id, (unsigned long long)rid);
+                }
+                (void)kb;
+            }
+        }
+    }
+}
+
+// The reference's join stalls at the smallest unmatched index ID (Mesher.cpp:35-38):
+// every index whose ID is >= that ID keeps its raw ID.
+__global__ void k_mesh_stall_fix(const uint64_t *__restrict__ raw_ids, uint64_t ni,
+                                 const unsigned long long *__restrict__ stall_id,
+                                 uint32_t *__restrict__ idx)
+{
+    const unsigned long long s = *stall_id;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ni;
+         q += (uint64_t)gridDim.x * blockDim.x)
+        if (raw_ids[q] >= s)
+            idx[q] = (uint32_t)raw_ids[q];
+}
+
+// ---------------------------------------------------------------- Bin
+__device__ __forceinline__ bool to_int_checked(float f, int &out)
+{
+    if (!(f > -2147483648.0f && f < 2147483648.0f))
+        return false;
+    out = (int)f;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t orderable(float z)
+{
+    if (z == 0.0f)
+        z = 0.0f;                                  // -0 and +0 compare equal in the reference
+    const uint32_t b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, uint64_t n, unsigned long long v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Renderer::Bin (Renderer.cpp:59-79): nearest z (NDC) wins, the earlier point on ties.
+__global__ void k_bin_splat(const float *__restrict__ pts, uint64_t n, Mat4 w2c, int W, int H,
+                            unsigned long long *__restrict__ zbuf)
+{
+    const float hw = (float)W * 0.5f, hh = (float)H * 0.5f;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float p[4] = { pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], 1.0f };
+        mat_vec(w2c, p, p);
+        const float w = p[3];
+        const float x = p[0] / w, y = p[1] / w, z = p[2] / w;     // math.h:414-419
+        int u, yi;
+        if (!to_int_checked(x * hw + hw, u) || !to_int_checked(y * hh + hh, yi))
+            continue;
+        const long v = (long)H - 1 - yi;
+        if (u >= 0 && u < W && v >= 0 && v < H && z >= -1.0f && z <= 1.0f) {
+            const unsigned long long key = (unsigned long long)orderable(z) << 32 | (uint32_t)i;
+            atomicMin(&zbuf[(size_t)v * W + u], key);
+        }
+    }
+}
+
+__global__ void k_bin_resolve(const float *__restrict__ pts, const unsigned long long *__restrict__ zbuf,
+                              int W, int H, float *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= W * H)
+        return;
+    const unsigned long long k = zbuf[q];
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (k != ~0ull) {
+        const uint32_t i = (uint32_t)k;
+        x = pts[3 * (size_t)i];
+        y = pts[3 * (size_t)i + 1];
+        z = pts[3 * (size_t)i + 2];
+    }
+    out[3 * q] = x;
+    out[3 * q + 1] = y;
+    out[3 * q + 2] = z;
+}
+
+// =================================================================== host
+static size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <typename K, typename V>
+static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx,
+                             size_t icap, size_t *nverts, size_t *nidx, int *stalled)
+{
+    kifi_ctx *ctx = vol->ctx;
+    hipStream_t st = ctx->stream;
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    uint64_t n = 0;
+    KIFI_HIP(ctx, hipMemcpy(&n, vol->d_size, 8, hipMemcpyDeviceToHost));
+    if (stalled)
+        *stalled = 0;
+    if (n < 2) {
+        *nverts = 0;
+        if (nidx)
+            *nidx = 0;
+        return KIFI_OK;
+    }
+    const uint64_t m = n - 1;
+    const uint64_t nt = (m + kScanTile - 1) / kScanTile + 1;
+    size_t bytes = a256(m) + 3 * a256(m * 4) + a256(nt * 4) + 2 * a256(16);
+    kifi_status s = ensure_scratch(ctx, bytes);
+    if (s != KIFI_OK)
+        return s;
+    char *p = (char *)ctx->scratch;
+    uint8_t *vflags = (uint8_t *)p; p += a256(m);
+    uint32_t *cv_fwd = (uint32_t *)p; p += a256(m * 4);
+    uint32_t *cv_rev = (uint32_t *)p; p += a256(m * 4);
+    uint32_t *ci_rev = (uint32_t *)p; p += a256(m * 4);
+    uint32_t *tsum = (uint32_t *)p; p += a256(nt * 4);
+    uint64_t *totals = (uint64_t *)p; p += a256(16);
+    unsigned long long *stall = (unsigned long long *)p;
+
+    const K *keys = (const K *)vol->keys[vol->cur];
+    const V *vals = (const V *)vol->vals[vol->cur];
+    const unsigned grid = (unsigned)std::min<uint64_t>((m + kMT - 1) / kMT, (uint64_t)ctx->num_cus * 16);
+    if (tris)
+        hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, vflags,
+                           cv_fwd, (uint32_t *)nullptr, ci_rev);
+    else
+        hipLaunchKernelGGL((k_mesh_count<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, vflags,
+                           (uint32_t *)nullptr, cv_rev, (uint32_t *)nullptr);
+    uint32_t *voff = tris ? cv_fwd : cv_rev;
+    device_scan(st, voff, voff, m, tsum, totals);
+    if (tris)
+        device_scan(st, ci_rev, ci_rev, m, tsum, totals + 1);
+    uint64_t h_tot[2] = { 0, 0 };
+    KIFI_HIP(ctx, hipMemcpyAsync(h_tot, totals, tris ? 16 : 8, hipMemcpyDeviceToHost, st));
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    *nverts = h_tot[0];
+    if (nidx)
+        *nidx = tris ? h_tot[1] : 0;
+    if (h_tot[0] > vcap || (tris && h_tot[1] > icap))
+        return KIFI_ERR_CAPACITY;
+
+    // output buffers on the device (after the counts), then copy to the host
+    const size_t vb = a256(h_tot[0] * 12 + 4), ib = tris ? a256(h_tot[1] * 4 + 4) : 0,
+                 rb = tris ? a256(h_tot[1] * 8 + 8) : 0;
+    s = ensure_out(ctx, vb + ib + rb);
+    if (s != KIFI_OK)
+        return s;
+    char *q = (char *)ctx->out_scratch;
+    float *d_verts = (float *)q; q += vb;
+    uint32_t *d_idx = (uint32_t *)q; q += ib;
+    uint64_t *d_raw = (uint64_t *)q;
+    MeshParams mp{ vol->consts.vl, vol->consts.vlHalfPlusMin };
+    if (tris) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, stall, (uint64_t)1, ~0ull);
+        hipLaunchKernelGGL((k_mesh_emit<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, mp, vflags,
+                           cv_fwd, ci_rev, d_verts, d_idx, d_raw, stall);
+        unsigned long long h_stall = ~0ull;
+        KIFI_HIP(ctx, hipMemcpyAsync(&h_stall, stall, 8, hipMemcpyDeviceToHost, st));
+        KIFI_HIP(ctx, hipStreamSynchronize(st));
+        if (h_stall != ~0ull) {
+            hipLaunchKernelGGL(k_mesh_stall_fix, dim3(256), dim3(256), 0, st, d_raw, (uint64_t)h_tot[1],
+                               stall, d_idx);
+            if (stalled)
+                *stalled = 1;
+        }
+    } else {
+        hipLaunchKernelGGL((k_mesh_emit<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, mp, vflags,
+                           cv_rev, (uint32_t *)nullptr, d_verts, (uint32_t *)nullptr, (uint64_t *)nullptr,
+                           stall);
+    }
+    KIFI_HIP(ctx, hipGetLastError());
+    if (h_tot[0])
+        KIFI_HIP(ctx, hipMemcpyAsync(verts, d_verts, h_tot[0] * 12, hipMemcpyDeviceToHost, st));
+    if (tris && h_tot[1])
+        KIFI_HIP(ctx, hipMemcpyAsync(idx, d_idx, h_tot[1] * 4, hipMemcpyDeviceToHost, st));
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    return KIFI_OK;
+}
+
+kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
+                          size_t *nverts, size_t *nidx, int *stalled)
+{
+    if (vol->desc.key_bits == 32) {
+        if (vol->desc.storage == KIFI_STORAGE_F32)
+            return mesh_impl<uint32_t, VoxF32m>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+        return mesh_impl<uint32_t, uint32_t>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+    }
+    if (vol->desc.storage == KIFI_STORAGE_F32)
+        return mesh_impl<uint64_t, VoxF32m>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+    return mesh_impl<uint64_t, uint32_t>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+}
+
+kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,
+                     float *out)
+{
+    hipStream_t st = ctx->stream;
+    const size_t P = (size_t)W * H;
+    const size_t bytes = a256(n * 12 + 4) + a256(P * 8) + a256(P * 12);
+    kifi_status s = ensure_scratch(ctx, bytes);
+    if (s != KIFI_OK)
+        return s;
+    char *p = (char *)ctx->scratch;
+    float *d_pts = (float *)p; p += a256(n * 12 + 4);
+    unsigned long long *zb = (unsigned long long *)p; p += a256(P * 8);
+    float *d_out = (float *)p;
+    if (n)
+        KIFI_HIP(ctx, hipMemcpyAsync(d_pts, points, n * 12, hipMemcpyHostToDevice, st));
+    Mat4 m;
+    for (int i = 0; i < 16; ++i)
+        m.m[i] = w2c[i];
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, zb, (uint64_t)P, ~0ull);
+    if (n)
+        hipLaunchKernelGGL(k_bin_splat, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                           st, d_pts, (uint64_t)n, m, W, H, zb);
+    hipLaunchKernelGGL(k_bin_resolve, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, d_pts, zb, W, H,
+                       d_out);
+    KIFI_HIP(ctx, hipGetLastError());
+    KIFI_HIP(ctx, hipMemcpyAsync(out, d_out, P * 12, hipMemcpyDeviceToHost, st));
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    return KIFI_OK;
+}
+
+} // namespace kifi

@@ -1,0 +1,141 @@
+"""GPU parity: Integrator::Integrate on the MI355X vs the CPU oracle.
+
+Keys, merge result and voxel values are compared bit for bit (the HIP path
+uses the reference's binary32 operation order; -ffp-contract=off).
+Reference: KinectFission/kifi/Integrator.cpp:26-71.
+"""
+import numpy as np
+import pytest
+
+import merge_fusion_amd as mf
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+CAM = mf.Intrinsics(512, 424, 365.0, 365.0)
+
+
+def _vals_bits(vals, storage):
+    if storage == mf.STORAGE_F32:
+        if vals.dtype.names:
+            return np.stack([vals["sum"], vals["weight"]], 1).view(np.uint32)
+        return np.ascontiguousarray(vals, np.float32).view(np.uint32)
+    return np.ascontiguousarray(vals, np.uint32)
+
+
+def assert_volume_equal(gv, ov):
+    gk, gvals = gv.download()
+    ok, ovals = ov.read()
+    assert gk.shape == ok.shape, (gk.shape, ok.shape)
+    assert np.array_equal(gk.astype(np.uint64), ok), "voxel key sets differ"
+    st = gv.desc.storage
+    a, b = _vals_bits(gvals, st), _vals_bits(ovals, st)
+    diff = np.nonzero(np.any(a.reshape(len(gk), -1) != b.reshape(len(ok), -1), axis=1))[0]
+    assert diff.size == 0, "values differ at %d voxels, first %s: gpu %s oracle %s" % (
+        diff.size, diff[:3], gvals[diff[:3]], ovals[diff[:3]])
+
+
+def frame(kind, seed, traj, f, cam=CAM):
+    sc = mf.make_scene(kind, seed)
+    pose = mf.trajectory_pose(traj, seed, f)
+    d = mf.render(sc, cam, pose, noise_seed=1000 * seed + f)
+    return d, mf.invert_rigid(pose).astype(np.float32)
+
+
+@pytest.mark.parametrize("storage", [mf.STORAGE_F32, mf.STORAGE_F16U8])
+def test_plane_spheres_one_frame(ctx, storage):
+    """Config 1: one 512x424 frame into an empty 8 mm volume (res 1024, side 8.192 m)."""
+    d, w2e = frame(mf.SCENE_PLANE_SPHERES, 1, mf.TRAJ_STATIC, 0)
+    gv = mf.Volume(ctx, 1024, 8.192, 0.04, 32, storage)
+    ov = orc.Volume(1024, 8.192, 0.04, 10, storage)
+    gv.integrate(d, CAM, w2e)
+    ost = ov.integrate(d, CAM, w2e)
+    gst = gv.stats()
+    assert gst["keys"] == ost["keys"]
+    assert gst["new"] == ost["new"]
+    assert gst["size"] == ost["size"]
+    assert_volume_equal(gv, ov)
+
+
+@pytest.mark.parametrize("storage", [mf.STORAGE_F32, mf.STORAGE_F16U8])
+def test_room_trajectory_frames(ctx, storage):
+    """Config 2 shape: several frames of the room along the circle, 4 mm voxels."""
+    gv = mf.Volume(ctx, 1024, 4.096, 0.02, 32, storage)
+    ov = orc.Volume(1024, 4.096, 0.02, 10, storage)
+    for f in range(4):
+        d, w2e = frame(mf.SCENE_ROOM, 2, mf.TRAJ_CIRCLE, 3 * f)
+        gv.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    assert_volume_equal(gv, ov)
+
+
+@pytest.mark.parametrize("storage", [mf.STORAGE_F32, mf.STORAGE_F16U8])
+def test_64bit_keys(ctx, storage):
+    """64-bit keys (21 bits per axis): res 2048 at 2 mm, two frames."""
+    gv = mf.Volume(ctx, 2048, 4.096, 0.02, 64, storage)
+    ov = orc.Volume(2048, 4.096, 0.02, 21, storage)
+    for f in range(2):
+        d, w2e = frame(mf.SCENE_ROOM, 3, mf.TRAJ_CIRCLE, 5 * f)
+        gv.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    assert_volume_equal(gv, ov)
+
+
+def test_empty_and_all_invalid_frames(ctx):
+    gv = mf.Volume(ctx, 1024, 8.192, 0.04)
+    ov = orc.Volume(1024, 8.192, 0.04)
+    z = np.zeros((CAM.height, CAM.width), np.uint16)
+    w2e = np.eye(4, dtype=np.float32).reshape(16)
+    gv.integrate(z, CAM, w2e)
+    ov.integrate(z, CAM, w2e)
+    assert gv.size() == 0 == ov.size()
+    # one valid pixel: 7 keys (the eighth, the maximum, is dropped by the x pass)
+    z[200, 300] = 1500
+    gv.integrate(z, CAM, w2e)
+    ov.integrate(z, CAM, w2e)
+    assert gv.size() == 7 == ov.size()
+    assert_volume_equal(gv, ov)
+
+
+def test_small_ragged_frames(ctx):
+    """Frame widths that are multiples of 4 but not of the wave or tile sizes."""
+    cam = mf.Intrinsics(36, 20, 30.0, 30.0)
+    gv = mf.Volume(ctx, 256, 4.0, 0.05)
+    ov = orc.Volume(256, 4.0, 0.05)
+    sc = mf.make_scene(mf.SCENE_PLANE_SPHERES, 9)
+    for f in range(3):
+        pose = mf.trajectory_pose(mf.TRAJ_STATIC, 9, 0)
+        pose[12] += 0.01 * f
+        d = mf.render(sc, cam, pose, noise_seed=77 + f)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        gv.integrate(d, cam, w2e)
+        ov.integrate(d, cam, w2e)
+    assert_volume_equal(gv, ov)
+
+
+def test_upload_download_roundtrip_then_integrate(ctx):
+    """flat_map content survives upload/download; integration merges into it."""
+    ov = orc.Volume(1024, 4.096, 0.02)
+    d, w2e = frame(mf.SCENE_ROOM, 4, mf.TRAJ_CIRCLE, 0)
+    ov.integrate(d, CAM, w2e)
+    k, v = ov.read()
+    gv = mf.Volume(ctx, 1024, 4.096, 0.02)
+    gvals = np.zeros(len(k), gv.val_dtype)
+    gvals["sum"], gvals["weight"] = v[:, 0], v[:, 1]
+    gv.upload(k.astype(np.uint32), gvals)
+    assert_volume_equal(gv, ov)
+    d2, w2e2 = frame(mf.SCENE_ROOM, 4, mf.TRAJ_CIRCLE, 7)
+    gv.integrate(d2, CAM, w2e2)
+    ov.integrate(d2, CAM, w2e2)
+    assert_volume_equal(gv, ov)
+
+
+def test_update_covers_floor_v_over_4(ctx):
+    """Q2: the last V mod 4 voxels of the merged map are never updated."""
+    gv = mf.Volume(ctx, 1024, 8.192, 0.04)
+    d, w2e = frame(mf.SCENE_PLANE_SPHERES, 5, mf.TRAJ_STATIC, 0)
+    gv.integrate(d, CAM, w2e)
+    k, v = gv.download()
+    n = len(k)
+    assert np.all(v["weight"][n // 4 * 4:] == 0)
+    assert np.all(v["weight"][: n // 4 * 4] <= 1)
