@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Fused frames/s of the kifi fusion pipeline on one MI355X.
+
+Workload (the trajectory configuration): 200 seeded 512x424 uint16 frames of a
+procedural room (4 x 3 x 4 m box, 10 boxes, 6 spheres), camera on a circle of
+r = 1.2 m at 1.5 cm / 1 deg per frame with jitter, noise N(0, (1.5e-3 z^2)^2) m
+and 2 % invalid pixels; 4 mm voxels (res 1024, side 4.096 m, truncation 2 cm,
+binary32 sum + weight, 32-bit keys); per frame: 3-level point-to-plane ICP
+(10/5/4 iterations) against the model of the previous frame, integration at
+the tracked pose, model prediction; a mesh (Mesher::Mesh, vertices copied to
+the host) every 50 frames.  One step = one frame.  Frames are rendered on the
+host and copied to HBM before timing; the timed region covers frames
+[warmup, warmup + steps).
+
+    python bench.py [--gpus 1] [--steps 180] [--warmup 20]
+
+Prints one JSON line.  The CPU baseline is the single-threaded oracle
+(integration, model prediction, FP64 ICP) on a bounded sample of the same
+trajectory, timed on this host.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fused frames/s at 512x424, 4 mm voxels; ICP iters/frame, learned vs plain init"
+HBM_PEAK_GBS = 8000.0
+W, H, F = 512, 424, 365.0
+SEED = 2
+RES, SIDE, TRUNC = 1024, 4.096, 0.02
+S_K, S_V = 4, 8            # key and value bytes (32-bit keys, binary32 sum + weight)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=180)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--mesh-every", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain copy kernel of this size 20 times (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def render_frames(mf, cam, n):
+    scene = mf.make_scene(mf.SCENE_ROOM, SEED)
+    poses = [mf.trajectory_pose(mf.TRAJ_CIRCLE, SEED, f) for f in range(n)]
+    frames = np.stack([mf.render(scene, cam, poses[f], noise_seed=1000 + f) for f in range(n)])
+    return frames, poses
+
+
+def cpu_baseline(mf, cam, frames, poses, budget_s):
+    """Oracle pipeline on the first frames of the trajectory until the time budget is spent."""
+    import oracle as orc
+    vol = orc.Volume(RES, SIDE, TRUNC)
+    w2e0 = mf.invert_rigid(poses[0]).astype(np.float32)
+    vol.integrate(frames[0], cam, w2e0)
+    pts, nrm = orc.model_predict(vol, w2e0, cam)
+    model_w2e, pose = w2e0, poses[0]
+    times = []
+    t_start = time.perf_counter()
+    f = 1
+    while f < len(frames) and time.perf_counter() - t_start < budget_s:
+        t0 = time.perf_counter()
+        pose, _, _ = orc.icp_f64(frames[f], cam, pts, nrm, model_w2e.astype(np.float64), pose)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        vol.integrate(frames[f], cam, w2e)
+        pts, nrm = orc.model_predict(vol, w2e, cam)
+        model_w2e = w2e
+        times.append(time.perf_counter() - t0)
+        f += 1
+    ts = np.array(times)
+    med = float(np.median(ts[3:] if len(ts) > 5 else ts))
+    return {
+        "value": round(1.0 / med, 4),
+        "unit": "frames/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle pipeline (FP64 point-to-plane ICP 10/5/4, integration, model prediction) on frames "
+                  "1..%d of the same trajectory (volume 0 -> %d voxels), median frame time %.3f s; host %s, "
+                  "nproc %d" % (f - 1, vol.size(), med, platform.processor() or platform.machine(),
+                                os.cpu_count() or 0),
+    }
+
+
+def calibrate_copy(mf, mib):
+    ctx = mf.Context(0)
+    nbytes = mib << 20
+    a = ctx.device_upload(np.zeros(nbytes // 4, np.uint32))
+    b = ctx.device_upload(np.zeros(nbytes // 4, np.uint32))
+    for _ in range(20):
+        ctx.device_copy(b.address(), a.address(), nbytes)
+    ctx.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 20}))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the fusion path is not sharded); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import merge_fusion_amd as mf
+    if args.calibrate_copy_mib:
+        calibrate_copy(mf, args.calibrate_copy_mib)
+        return
+    cam = mf.Intrinsics(W, H, F, F)
+    n = args.warmup + args.steps
+    frames, poses = render_frames(mf, cam, n)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mf, cam, frames, poses, args.cpu_seconds)
+
+    ctx = mf.Context(0)
+    dframes = ctx.device_upload(frames)
+    fbytes = W * H * 2
+    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
+    mesh_buf = np.zeros((1 << 22, 3), np.float32)
+    mesh_counts, pose_err = [], []
+
+    def mesh(f):
+        nonlocal mesh_buf
+        v = pipe.volume
+        nv = v.mesh_into(mesh_buf)
+        if nv > len(mesh_buf):
+            mesh_buf = np.zeros((int(nv * 1.5), 3), np.float32)
+            nv = v.mesh_into(mesh_buf)
+        mesh_counts.append(int(nv))
+
+    def step(f):
+        if f == 0:
+            pipe.integrate_device(dframes.address(0), mf.invert_rigid(poses[0]).astype(np.float32))
+        else:
+            pipe.integrate_device(dframes.address(f * fbytes))
+        if (f + 1) % args.mesh_every == 0:
+            mesh(f)
+            pose_err.append((f, mf.pose_error(pipe.pose(), poses[f])))
+
+    for f in range(args.warmup):
+        step(f)
+    ctx.synchronize()
+    st0 = pipe.stats()
+    tot0 = pipe.volume.totals()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for f in range(args.warmup, n):
+        step(f)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.timing(False)
+    st1 = pipe.stats()
+    tot1 = pipe.volume.totals()
+    final_err = mf.pose_error(pipe.pose(), poses[n - 1])
+    stage = {name: ctx.timing_read(s) for name, s in (("merge_update", mf.STAGE_MERGE_UPDATE),
+                                                      ("sort", mf.STAGE_SORT), ("icp", mf.STAGE_ICP),
+                                                      ("model", mf.STAGE_MODEL), ("mesh", mf.STAGE_MESH))}
+    elapsed = t1 - t0
+    fps = args.steps / elapsed
+
+    # roofline of the dominant kernel: fused merge + TSDF update.  Algorithmic
+    # bytes per launch = read V0 keys+values + E new keys, write V1 keys+values,
+    # read the depth frame once.
+    launches = tot1["frames"] - tot0["frames"]
+    dv0, de, dv1 = (tot1[k] - tot0[k] for k in ("v_old", "new", "v_new"))
+    alg_bytes = (S_K + S_V) * (dv0 + dv1) + S_K * de + 2 * W * H * launches
+    mu_ms, mu_n = stage["merge_update"]
+    achieved = alg_bytes / launches / (mu_ms / mu_n * 1e-3) / 1e9 if mu_n else 0.0
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "merge_update_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
+
+    icp_frames = st1["frames"] - st0["frames"]
+    icp_iters = (st1["sum_iters"] - st0["sum_iters"]) / max(icp_frames, 1)
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "room trajectory, 512x424, 4 mm voxels (res 1024, side 4.096 m, trunc 2 cm), "
+                        "3-level point-to-plane ICP 10/5/4, mesh every %d frames, plain init" % args.mesh_every,
+            "frames": n,
+            "timed_frames": "%d..%d" % (args.warmup, n - 1),
+            "key_bits": 32,
+            "storage": "f32 sum + f32 weight",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_merge_update",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)),
+            "avg_launch_ms": round(mu_ms / mu_n, 5) if mu_n else None,
+            "launches": mu_n,
+        },
+        "cpu_baseline": cpu,
+        "icp_iters_per_frame": round(icp_iters, 3),
+        "icp_fallback_rate": round((st1["fallbacks"] - st0["fallbacks"]) / max(icp_frames, 1), 4),
+        "pose_error_m_deg": [[f, round(e[0], 5), round(e[1], 4)] for f, e in pose_err] +
+                            [[n - 1, round(final_err[0], 5), round(final_err[1], 4)]],
+        "volume_voxels_final": int(pipe.volume.size()),
+        "mesh_vertices": mesh_counts,
+        "stage_ms_per_frame": {k: round(v[0] / max(args.steps, 1), 4) for k, v in stage.items()},
+        "fps_with_h2d": None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,9 @@ kifi_status kifi_integrate_device(kifi_volume *vol, const uint16_t *depth_mm_dev
 /* Counts of the last integration: [0] K keygen keys, [1] 8-neighbourhood candidates,
  * [2] E keys handed to the merge, [3] volume size after the merge.  Synchronous. */
 kifi_status kifi_integrate_stats(kifi_volume *vol, size_t stats[4]);
+/* Running totals since the volume was created: [0] integrations, [1] sum of V0,
+ * [2] sum of E, [3] sum of V1, [4] sum of K.  Synchronous. */
+kifi_status kifi_integrate_totals(kifi_volume *vol, uint64_t totals[5]);
 
 /* ---- meshing: kifi::Mesher (Mesher.h:17-18, Mesher.cpp:14-45) ----
  * kifi_mesh: Mesh(volume, verts) -- float3 vertices in generation order.
@@ -110,6 +113,105 @@ kifi_status kifi_mesh_indexed(kifi_volume *vol, float *verts, size_t vcap, uint3
  * (host, 0 = empty) with the given worldToClip. Synchronous. */
 kifi_status kifi_bin(kifi_ctx *ctx, const float *points, size_t n, const float worldToClip[16],
                      int width, int height, float *out_points);
+
+/* ---- tracking: kifi::ICP::Align (ICP.h:18-27, ICP.cpp:9-265) ----
+ * The reference aligns with 4 fixed Horn point-to-point steps against a model
+ * point buffer.  Here the model is a device-resident point + normal map rendered
+ * from the volume (kifi_model_predict), and ICP is projective point-to-plane
+ * (KIFI_ICP_POINT_TO_PLANE, 6x6 normal equations) or point-to-point with Horn's
+ * closed form (KIFI_ICP_HORN) on a depth pyramid with a convergence test.
+ * Sums are binary32 with compensated reduction; solves and poses are FP64. */
+typedef struct kifi_model kifi_model;
+typedef struct kifi_pipeline kifi_pipeline;
+
+enum { KIFI_ICP_POINT_TO_PLANE = 0, KIFI_ICP_HORN = 1 };
+
+typedef struct {
+    int   mode;            /* KIFI_ICP_POINT_TO_PLANE or KIFI_ICP_HORN                 */
+    int   levels;          /* pyramid levels, 1..3 (level 0 = full resolution)         */
+    int   iters[3];        /* maximum iterations per level, index = level              */
+    float dist_thresh;     /* correspondence gate, metres                              */
+    float eps_rot;         /* convergence: |rotation increment| < eps_rot (rad) ...    */
+    float eps_trans;       /* ... and |translation increment| < eps_trans (m)          */
+    int   min_inliers;     /* fewer correspondences: stop (tracking failure)           */
+    float fallback_rms;    /* > 0: re-run plain ICP from the previous pose when the    */
+    int   fallback_iters;  /*       final RMS exceeds it or iterations reach the cap   */
+} kifi_icp_params;
+
+typedef struct {
+    int   iterations;      /* solves performed in the frame (both passes)              */
+    int   iters[3];        /* of the last pass, per level                              */
+    float rms;             /* RMS residual of the last association (m)                 */
+    int   inliers;         /* correspondences of the last association                  */
+    int   failed;          /* too few inliers or a singular system                     */
+    int   fallback;        /* the frame re-ran plain ICP from the previous pose        */
+    long long sum_iters;   /* running totals over tracked frames                       */
+    long long frames;
+    long long fallbacks;
+} kifi_icp_stats;
+
+kifi_status kifi_icp_params_default(kifi_icp_params *p);
+
+/* Model point/normal maps at the resolution of `cam` (device-resident). */
+kifi_status kifi_model_create(kifi_ctx *ctx, const kifi_intrinsics *cam, kifi_model **out);
+void        kifi_model_destroy(kifi_model *m);
+/* Renders the model of `vol` seen from worldToEye (column storage). Asynchronous. */
+kifi_status kifi_model_predict(kifi_model *m, kifi_volume *vol, const float worldToEye[16]);
+/* Copies the maps out: W*H float4 each (xyz + validity).  Synchronous. */
+kifi_status kifi_model_download(kifi_model *m, float *points, float *normals);
+/* ICP::Align: align a depth frame (host, mm) to the model starting from the guess
+ * eye->world pose; returns the pose and statistics.  Synchronous. */
+kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const double guess_eyeToWorld[16], const kifi_icp_params *params,
+                           double out_eyeToWorld[16], kifi_icp_stats *stats);
+/* Horn's closed-form rigid alignment of n correspondences (ICP.cpp:166-264):
+ * compensated centroid and cross-covariance reductions, FP64 4x4 eigen solve.
+ * out = the transform mapping src onto dst.  Synchronous. */
+kifi_status kifi_horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16]);
+
+/* ---- frame loop: kifi::Pipeline (Pipeline.h:19-41, Pipeline.cpp:26-68) ----
+ * Owns a volume, a model and the tracking state.  kifi_pipeline_integrate with a
+ * worldToEye integrates at that pose (the reference's #if 1 branch); with NULL
+ * it tracks the frame with ICP against the model of the previous frame (the
+ * #else branch), integrates at the tracked pose and renders the next model.
+ * Asynchronous: no host synchronisation per frame. */
+kifi_status kifi_pipeline_create(kifi_ctx *ctx, const kifi_volume_desc *vol, const kifi_intrinsics *cam,
+                                 const kifi_icp_params *params, kifi_pipeline **out);
+void        kifi_pipeline_destroy(kifi_pipeline *p);
+kifi_volume *kifi_pipeline_volume(kifi_pipeline *p);
+kifi_model  *kifi_pipeline_model(kifi_pipeline *p);
+kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth_mm, int depth_on_device,
+                                    const float worldToEye[16]);
+/* Initial guess (eye->world) for the next tracked frame instead of the previous
+ * pose, e.g. a learned prediction; plain ICP from the previous pose remains the
+ * fallback per kifi_icp_params. */
+kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double eyeToWorld[16]);
+kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double eyeToWorld[16]);
+kifi_status kifi_pipeline_pose(kifi_pipeline *p, double eyeToWorld[16]);     /* EyeToWorld(), sync */
+kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats);     /* sync */
+
+/* ---- measurement and staging (no reference counterpart; the reference times
+ * stages with a CPU stop_watch, util/stop_watch.cpp:23-58) ----
+ * Stage timers bracket the named kernels with HIP events on the context's
+ * stream while enabled; kifi_timing_read synchronises and returns the summed
+ * elapsed milliseconds and the number of timed launches, then resets. */
+enum {
+    KIFI_STAGE_MERGE_UPDATE = 0, /* fused merge + TSDF update kernel            */
+    KIFI_STAGE_SORT = 1,         /* keygen + radix sort + unique                */
+    KIFI_STAGE_ICP = 2,          /* depth pyramid + all ICP iterations          */
+    KIFI_STAGE_MODEL = 3,        /* model splat + resolve + normals             */
+    KIFI_STAGE_MESH = 4,         /* marching cubes                              */
+    KIFI_STAGE_COUNT = 5
+};
+kifi_status kifi_timing_enable(kifi_ctx *ctx, int enable);
+kifi_status kifi_timing_read(kifi_ctx *ctx, int stage, double *total_ms, long long *count);
+/* Device buffers for callers without their own allocator (frames resident in HBM). */
+kifi_status kifi_device_alloc(kifi_ctx *ctx, size_t bytes, void **out);
+kifi_status kifi_device_free(kifi_ctx *ctx, void *ptr);
+kifi_status kifi_memcpy_to_device(kifi_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Plain streaming copy kernel (16-byte vectors), the calibration reference for
+ * HBM byte counters.  bytes must be a multiple of 16.  Asynchronous. */
+kifi_status kifi_device_copy(kifi_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@ present, every call raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -29,7 +30,15 @@ HIP_EXPORTS = [
     "kifi_volume_create", "kifi_volume_destroy", "kifi_volume_clear", "kifi_volume_size",
     "kifi_volume_download", "kifi_volume_upload", "kifi_integrate", "kifi_integrate_device",
     "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_bin",
+    "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
+    "kifi_model_download", "kifi_icp_align", "kifi_horn_align", "kifi_pipeline_create",
+    "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
+    "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
+    "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_memcpy_to_device",
+    "kifi_device_copy", "kifi_integrate_totals",
 ]
+STAGE_MERGE_UPDATE, STAGE_SORT, STAGE_ICP, STAGE_MODEL, STAGE_MESH = range(5)
+ICP_POINT_TO_PLANE, ICP_HORN = 0, 1
 
 
 class KifiError(RuntimeError):
@@ -52,6 +61,27 @@ class VolumeDesc(ctypes.Structure):
                 ("key_bits", ctypes.c_int), ("storage", ctypes.c_int)]
 
 
+class IcpParams(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("levels", ctypes.c_int), ("iters", ctypes.c_int * 3),
+                ("dist_thresh", ctypes.c_float), ("eps_rot", ctypes.c_float), ("eps_trans", ctypes.c_float),
+                ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int)]
+
+    @classmethod
+    def default(cls):
+        p = cls()
+        load_hip().kifi_icp_params_default(ctypes.byref(p))
+        return p
+
+
+class IcpStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("iters", ctypes.c_int * 3), ("rms", ctypes.c_float),
+                ("inliers", ctypes.c_int), ("failed", ctypes.c_int), ("fallback", ctypes.c_int),
+                ("sum_iters", ctypes.c_longlong), ("frames", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k == "iters" else getattr(self, k)) for k, _ in self._fields_}
+
+
 class Scene(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("n_spheres", ctypes.c_int), ("n_boxes", ctypes.c_int),
                 ("spheres", (ctypes.c_double * 4) * 8), ("boxes", (ctypes.c_double * 6) * 16),
@@ -66,6 +96,10 @@ def _f32p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
+def _f64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
 def load_hip():
     """Load libkifi_hip.so (raises if it was not built)."""
     global _hip
@@ -75,6 +109,7 @@ def load_hip():
         lib = ctypes.CDLL(HIP_LIB_PATH)
         vp, sz, i32, f32p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
         szp = ctypes.POINTER(ctypes.c_size_t)
+        f64p = ctypes.POINTER(ctypes.c_double)
         sigs = {
             "kifi_ctx_create": ([i32, ctypes.POINTER(vp)], i32),
             "kifi_ctx_destroy": ([vp], None),
@@ -94,6 +129,31 @@ def load_hip():
             "kifi_mesh_indexed": ([vp, f32p, sz, ctypes.POINTER(ctypes.c_uint32), sz, szp, szp,
                                    ctypes.POINTER(ctypes.c_int)], i32),
             "kifi_bin": ([vp, f32p, sz, f32p, i32, i32, f32p], i32),
+            "kifi_icp_params_default": ([ctypes.POINTER(IcpParams)], i32),
+            "kifi_model_create": ([vp, ctypes.POINTER(Intrinsics), ctypes.POINTER(vp)], i32),
+            "kifi_model_destroy": ([vp], None),
+            "kifi_model_predict": ([vp, vp, f32p], i32),
+            "kifi_model_download": ([vp, f32p, f32p], i32),
+            "kifi_icp_align": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, ctypes.POINTER(IcpParams), f64p,
+                                ctypes.POINTER(IcpStats)], i32),
+            "kifi_horn_align": ([vp, f32p, f32p, sz, f32p], i32),
+            "kifi_pipeline_create": ([vp, ctypes.POINTER(VolumeDesc), ctypes.POINTER(Intrinsics),
+                                      ctypes.POINTER(IcpParams), ctypes.POINTER(vp)], i32),
+            "kifi_pipeline_destroy": ([vp], None),
+            "kifi_pipeline_volume": ([vp], vp),
+            "kifi_pipeline_model": ([vp], vp),
+            "kifi_pipeline_integrate": ([vp, vp, i32, f32p], i32),
+            "kifi_pipeline_set_guess": ([vp, f64p], i32),
+            "kifi_pipeline_set_pose": ([vp, f64p], i32),
+            "kifi_pipeline_pose": ([vp, f64p], i32),
+            "kifi_pipeline_stats": ([vp, ctypes.POINTER(IcpStats)], i32),
+            "kifi_timing_enable": ([vp, i32], i32),
+            "kifi_timing_read": ([vp, i32, f64p, ctypes.POINTER(ctypes.c_longlong)], i32),
+            "kifi_device_alloc": ([vp, sz, ctypes.POINTER(vp)], i32),
+            "kifi_device_free": ([vp, vp], i32),
+            "kifi_memcpy_to_device": ([vp, vp, vp, sz], i32),
+            "kifi_device_copy": ([vp, vp, vp, sz], i32),
+            "kifi_integrate_totals": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(lib, name)
@@ -179,6 +239,11 @@ class Context:
             raise KifiError("kifi_ctx_create failed (%d): no usable GPU?" % rc)
         self._h = h
         self._lib = lib
+        # objects created on this context; closed before the context is destroyed
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     @property
     def handle(self):
@@ -194,6 +259,13 @@ class Context:
 
     def close(self):
         if self._h:
+            # pipelines first (they own a volume and a model), then the rest
+            kids = sorted(list(self._children), key=lambda o: not isinstance(o, Pipeline))
+            for obj in kids:
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             self._lib.kifi_ctx_destroy(self._h)
             self._h = None
 
@@ -203,6 +275,29 @@ class Context:
         except Exception:
             pass
 
+    def timing(self, enable):
+        self.check(self._lib.kifi_timing_enable(self._h, 1 if enable else 0), "kifi_timing_enable")
+
+    def timing_read(self, stage):
+        """(total ms, launches) of a stage since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        self.check(self._lib.kifi_timing_read(self._h, stage, ctypes.byref(ms), ctypes.byref(n)), "kifi_timing_read")
+        return ms.value, n.value
+
+    def device_upload(self, array):
+        """Copy a numpy array into a new device buffer; returns DeviceBuffer."""
+        a = np.ascontiguousarray(array)
+        p = ctypes.c_void_p()
+        self.check(self._lib.kifi_device_alloc(self._h, a.nbytes, ctypes.byref(p)), "kifi_device_alloc")
+        buf = DeviceBuffer(self, p, a.nbytes)
+        self._adopt(buf)
+        self.check(self._lib.kifi_memcpy_to_device(self._h, p, a.ctypes.data, a.nbytes), "kifi_memcpy_to_device")
+        return buf
+
+    def device_copy(self, dst, src, nbytes):
+        self.check(self._lib.kifi_device_copy(self._h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes),
+                   "kifi_device_copy")
+
     def bin(self, points, world_to_clip, width, height):
         """Renderer::Bin: z-buffered splat into a (H, W, 3) float32 map."""
         pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
@@ -211,6 +306,27 @@ class Context:
         self.check(self._lib.kifi_bin(self._h, _f32p(pts), pts.shape[0], _f32p(m), width, height, _f32p(out)),
                    "kifi_bin")
         return out
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def address(self, offset=0):
+        return self.ptr.value + offset
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx._lib.kifi_device_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    close = free
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Volume:
@@ -223,6 +339,7 @@ class Volume:
         ctx.check(ctx._lib.kifi_volume_create(ctx.handle, ctypes.byref(self.desc), ctypes.byref(h)),
                   "kifi_volume_create")
         self._h = h
+        ctx._adopt(self)
         self.key_dtype = np.uint32 if key_bits == 32 else np.uint64
         self.val_dtype = np.dtype([("sum", np.float32), ("weight", np.float32)]) if storage == STORAGE_F32 \
             else np.uint32
@@ -281,6 +398,12 @@ class Volume:
                                                            ctypes.byref(cam), _f32p(m)),
                        "kifi_integrate_device")
 
+    def totals(self):
+        """Running sums since creation: integrations, V0, E, V1, K."""
+        t = (ctypes.c_uint64 * 5)()
+        self.ctx.check(self.ctx._lib.kifi_integrate_totals(self._h, t), "kifi_integrate_totals")
+        return {"frames": t[0], "v_old": t[1], "new": t[2], "v_new": t[3], "keys": t[4]}
+
     def stats(self):
         s = (ctypes.c_size_t * 4)()
         self.ctx.check(self.ctx._lib.kifi_integrate_stats(self._h, s), "kifi_integrate_stats")
@@ -297,6 +420,15 @@ class Volume:
         self.ctx.check(lib.kifi_mesh(self._h, _f32p(verts), n.value, ctypes.byref(n)), "kifi_mesh")
         return verts
 
+    def mesh_into(self, buf):
+        """Mesher::Mesh into a preallocated (N, 3) float32 buffer; returns the vertex count
+        (larger than len(buf) if the buffer was too small and nothing was written)."""
+        n = ctypes.c_size_t()
+        rc = self.ctx._lib.kifi_mesh(self._h, _f32p(buf), buf.shape[0], ctypes.byref(n))
+        if rc not in (KIFI_OK, KIFI_ERR_CAPACITY):
+            self.ctx.check(rc, "kifi_mesh")
+        return n.value
+
     def mesh_indexed(self):
         """Mesher::Mesh(volume, verts, idx): (verts sorted by vertex ID, indices, stalled)."""
         lib = self.ctx._lib
@@ -312,3 +444,141 @@ class Volume:
                                              ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(st)),
                        "kifi_mesh_indexed")
         return verts, idx, bool(st.value)
+
+
+class Model:
+    """Device-resident model point/normal maps (ICP's model input)."""
+
+    def __init__(self, ctx, cam, handle=None):
+        self.ctx, self.cam = ctx, cam
+        self._owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            ctx.check(ctx._lib.kifi_model_create(ctx.handle, ctypes.byref(cam), ctypes.byref(h)),
+                      "kifi_model_create")
+            handle = h
+        self._h = handle
+        ctx._adopt(self)
+
+    def close(self):
+        if self._h and self._owned:
+            self.ctx._lib.kifi_model_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def predict(self, volume, world_to_eye):
+        m = np.ascontiguousarray(world_to_eye, np.float32).reshape(16)
+        self.ctx.check(self.ctx._lib.kifi_model_predict(self._h, volume.handle, _f32p(m)), "kifi_model_predict")
+
+    def download(self):
+        P = self.cam.width * self.cam.height
+        pts = np.zeros((self.cam.height, self.cam.width, 4), np.float32)
+        nrm = np.zeros_like(pts)
+        self.ctx.check(self.ctx._lib.kifi_model_download(self._h, _f32p(pts), _f32p(nrm)), "kifi_model_download")
+        return pts, nrm
+
+    def icp_align(self, depth_mm, guess_e2w, params=None):
+        """ICP::Align: returns (eye->world float64[16], stats dict)."""
+        p = params if params is not None else IcpParams.default()
+        d = np.ascontiguousarray(depth_mm, np.uint16)
+        g = np.ascontiguousarray(guess_e2w, np.float64).reshape(16)
+        out = np.zeros(16, np.float64)
+        st = IcpStats()
+        self.ctx.check(self.ctx._lib.kifi_icp_align(self._h, d.ctypes.data, ctypes.byref(self.cam), _f64p(g),
+                                                    ctypes.byref(p), _f64p(out), ctypes.byref(st)),
+                       "kifi_icp_align")
+        return out, st.as_dict()
+
+
+def horn_align(ctx, src, dst):
+    s = np.ascontiguousarray(src, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dst, np.float32).reshape(-1, 3)
+    out = np.zeros(16, np.float32)
+    ctx.check(ctx._lib.kifi_horn_align(ctx.handle, _f32p(s), _f32p(d), s.shape[0], _f32p(out)), "kifi_horn_align")
+    return out
+
+
+class _BorrowedVolume(Volume):
+    def __init__(self, ctx, handle, desc):
+        self.ctx = ctx
+        self.desc = desc
+        self._h = handle
+        self.key_dtype = np.uint32 if desc.key_bits == 32 else np.uint64
+        self.val_dtype = np.dtype([("sum", np.float32), ("weight", np.float32)]) \
+            if desc.storage == STORAGE_F32 else np.uint32
+
+    def close(self):
+        self._h = None
+
+
+class Pipeline:
+    """kifi::Pipeline (Pipeline.h:19-41): volume + model + tracking, one call per frame."""
+
+    def __init__(self, ctx, cam, res=1024, side=4.096, trunc=0.02, key_bits=32, storage=STORAGE_F32,
+                 params=None):
+        self.ctx, self.cam = ctx, cam
+        self.params = params if params is not None else IcpParams.default()
+        desc = VolumeDesc(res, side, trunc, key_bits, storage)
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.kifi_pipeline_create(ctx.handle, ctypes.byref(desc), ctypes.byref(cam),
+                                                ctypes.byref(self.params), ctypes.byref(h)), "kifi_pipeline_create")
+        self._h = h
+        ctx._adopt(self)
+        self.volume = _BorrowedVolume(ctx, ctypes.c_void_p(ctx._lib.kifi_pipeline_volume(h)), desc)
+        self.model = Model(ctx, cam, ctypes.c_void_p(ctx._lib.kifi_pipeline_model(h)))
+        self.model._owned = False
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.kifi_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def integrate(self, depth_mm, world_to_eye=None):
+        """Integrate at world_to_eye, or track with ICP when it is None."""
+        d = np.ascontiguousarray(depth_mm, np.uint16)
+        m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
+        self.ctx.check(self.ctx._lib.kifi_pipeline_integrate(self._h, d.ctypes.data, 0, m), "kifi_pipeline_integrate")
+
+    def integrate_device(self, depth_ptr, world_to_eye=None):
+        m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
+        self.ctx.check(self.ctx._lib.kifi_pipeline_integrate(self._h, ctypes.c_void_p(depth_ptr), 1, m),
+                       "kifi_pipeline_integrate")
+
+    def set_guess(self, e2w):
+        g = np.ascontiguousarray(e2w, np.float64).reshape(16)
+        self.ctx.check(self.ctx._lib.kifi_pipeline_set_guess(self._h, _f64p(g)), "kifi_pipeline_set_guess")
+
+    def set_pose(self, e2w):
+        g = np.ascontiguousarray(e2w, np.float64).reshape(16)
+        self.ctx.check(self.ctx._lib.kifi_pipeline_set_pose(self._h, _f64p(g)), "kifi_pipeline_set_pose")
+
+    def pose(self):
+        out = np.zeros(16, np.float64)
+        self.ctx.check(self.ctx._lib.kifi_pipeline_pose(self._h, _f64p(out)), "kifi_pipeline_pose")
+        return out
+
+    def stats(self):
+        st = IcpStats()
+        self.ctx.check(self.ctx._lib.kifi_pipeline_stats(self._h, ctypes.byref(st)), "kifi_pipeline_stats")
+        return st.as_dict()
+
+
+def pose_error(a, b):
+    """(translation error m, rotation error deg) between two eye->world poses."""
+    A = np.asarray(a, np.float64).reshape(4, 4).T
+    B = np.asarray(b, np.float64).reshape(4, 4).T
+    dt = np.linalg.norm(A[:3, 3] - B[:3, 3])
+    R = A[:3, :3].T @ B[:3, :3]
+    ang = np.degrees(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1)))
+    return dt, ang

@@ -1,4 +1,5 @@
 // kifi_api.hip -- the extern "C" boundary of libkifi_hip.so (declared in include/kifi.h).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -8,7 +9,7 @@
 namespace kifi {
 
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                               const float w2e[16]);
+                               const float *w2e, const Mat4 *w2e_dev);
 kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
                           size_t *nverts, size_t *nidx, int *stalled);
 kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,
@@ -91,11 +92,134 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed)
     return KIFI_OK;
 }
 
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+static hipEvent_t pool_event(kifi_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+int timer_begin(kifi_ctx *ctx, int stage)
+{
+    if (!ctx->timing || stage < 0 || stage >= KIFI_STAGE_COUNT)
+        return -1;
+    hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+    if (!a || !b)
+        return -1;
+    (void)hipEventRecord(a, ctx->stream);
+    ctx->timers[stage].push_back({ a, b });
+    return (int)ctx->timers[stage].size() - 1;
+}
+
+void timer_end(kifi_ctx *ctx, int stage, int slot)
+{
+    if (slot < 0)
+        return;
+    (void)hipEventRecord(ctx->timers[stage][slot].second, ctx->stream);
+}
+
 } // namespace kifi
 
 using namespace kifi;
 
 extern "C" {
+
+kifi_status kifi_timing_enable(kifi_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return KIFI_ERR_ARG;
+    ctx->timing = enable != 0;
+    return KIFI_OK;
+}
+
+kifi_status kifi_timing_read(kifi_ctx *ctx, int stage, double *total_ms, long long *count)
+{
+    if (!ctx || stage < 0 || stage >= KIFI_STAGE_COUNT || !total_ms || !count)
+        return KIFI_ERR_ARG;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    for (auto &pr : ctx->timers[stage]) {
+        float ms = 0.0f;
+        KIFI_HIP(ctx, hipEventElapsedTime(&ms, pr.first, pr.second));
+        tot += ms;
+        ctx->event_pool.push_back(pr.first);
+        ctx->event_pool.push_back(pr.second);
+    }
+    *count = (long long)ctx->timers[stage].size();
+    *total_ms = tot;
+    ctx->timers[stage].clear();
+    return KIFI_OK;
+}
+
+kifi_status kifi_device_alloc(kifi_ctx *ctx, size_t bytes, void **out)
+{
+    if (!ctx || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    KIFI_HIP(ctx, hipSetDevice(ctx->device));
+    KIFI_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return KIFI_OK;
+}
+
+kifi_status kifi_device_free(kifi_ctx *ctx, void *ptr)
+{
+    if (!ctx)
+        return KIFI_ERR_ARG;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ptr)
+        KIFI_HIP(ctx, hipFree(ptr));
+    return KIFI_OK;
+}
+
+kifi_status kifi_device_copy(kifi_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src)) || bytes % 16 != 0)
+        return KIFI_ERR_ARG;
+    const size_t n = bytes / 16;
+    if (n) {
+        const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 32);
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, ctx->stream, (const uint4 *)src, (uint4 *)dst, n);
+        KIFI_HIP(ctx, hipGetLastError());
+    }
+    return KIFI_OK;
+}
+
+kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[5])
+{
+    if (!v || !totals)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FrameCounters c;
+    KIFI_HIP(ctx, hipMemcpy(&c, v->d_counters, sizeof c, hipMemcpyDeviceToHost));
+    totals[0] = c.tot_frames;
+    totals[1] = c.tot_v_old;
+    totals[2] = c.tot_new;
+    totals[3] = c.tot_v_new;
+    totals[4] = c.tot_keys;
+    return KIFI_OK;
+}
+
+kifi_status kifi_memcpy_to_device(kifi_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src)))
+        return KIFI_ERR_ARG;
+    KIFI_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KIFI_OK;
+}
 
 kifi_status kifi_ctx_create(int device, kifi_ctx **out)
 {
@@ -129,6 +253,13 @@ void kifi_ctx_destroy(kifi_ctx *ctx)
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->out_scratch) (void)hipFree(ctx->out_scratch);
     if (ctx->d_depth) (void)hipFree(ctx->d_depth);
+    for (auto &v : ctx->timers)
+        for (auto &pr : v) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : ctx->event_pool)
+        (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -180,6 +311,7 @@ kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *d, kifi_vo
     hipError_t e = hipMalloc(&v->d_counters, sizeof(FrameCounters));
     if (e == hipSuccess) e = hipHostMalloc(&v->h_counters, sizeof(FrameCounters), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&v->d_size, 8);
+    if (e == hipSuccess) e = hipMalloc(&v->d_mats, sizeof(FrameMats));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&v->size_event, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemcpy(v->d_size, &v->size_bound, 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(v->d_counters, 0, sizeof(FrameCounters));
@@ -210,6 +342,7 @@ void kifi_volume_destroy(kifi_volume *v)
     if (v->d_counters) (void)hipFree(v->d_counters);
     if (v->h_counters) (void)hipHostFree(v->h_counters);
     if (v->d_size) (void)hipFree(v->d_size);
+    if (v->d_mats) (void)hipFree(v->d_mats);
     if (v->size_event) (void)hipEventDestroy(v->size_event);
     delete v;
 }
@@ -310,7 +443,7 @@ kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, con
     kifi_status st = check_cam(v->ctx, cam);
     if (st != KIFI_OK)
         return st;
-    return integrate_dispatch(v, depth_dev, cam, worldToEye);
+    return integrate_dispatch(v, depth_dev, cam, worldToEye, nullptr);
 }
 
 kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_intrinsics *cam,
@@ -333,7 +466,7 @@ kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_
         ctx->depth_pixels = P;
     }
     KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream));
-    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye);
+    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye, nullptr);
 }
 
 kifi_status kifi_integrate_stats(kifi_volume *v, size_t stats[4])

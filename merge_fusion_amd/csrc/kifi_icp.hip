@@ -1,1 +1,1057 @@
-// placeholder
+// kifi_icp.hip -- model prediction and projective ICP on the GPU.
+//
+// Reference: ICP::Align / AlignStep (KinectFission/kifi/ICP.cpp:9-265) runs 4 fixed
+// Horn point-to-point steps over a 21x21 window search, against a model point
+// buffer from Mesher::Mesh + Renderer::Bin (Pipeline.cpp:36-55).  Here:
+//   - model prediction: one pass over the volume splats every near-surface voxel
+//     (|D| < dmax) at its estimated surface depth along the viewing ray into a
+//     z-buffer (64-bit atomicMin of depth | voxel index), then resolves points and
+//     normals (binary32, bit-exact with the oracle);
+//   - tracking: projective association on a 3-level depth pyramid, point-to-plane
+//     6x6 normal equations (or Horn's cross-covariance) accumulated in binary32 with
+//     compensated (TwoSum) reductions, solved on the device in FP64 (Cholesky /
+//     cyclic Jacobi for Horn's 4x4), pose composed in FP64.  Every iteration is two
+//     kernels and a device-side convergence flag, so a frame needs no host sync.
+#include <cfloat>
+
+#include "kifi_internal.hpp"
+#include "kifi_prims.hpp"
+#include "kifi_icp.hpp"
+
+namespace kifi {
+
+constexpr int kIT = 256;
+
+// ------------------------------------------------------------ voxel access
+struct VoxF32i { float sum, weight; };
+__device__ __forceinline__ float vw(const VoxF32i &v) { return v.weight; }
+__device__ __forceinline__ float vd(const VoxF32i &v) { return v.sum / v.weight; }
+__device__ __forceinline__ float vw(uint32_t v) { return (float)((v >> 16) & 0xffu); }
+__device__ __forceinline__ float vd(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu)); }
+
+__device__ __forceinline__ uint32_t orderable_f(float z)
+{
+    if (z == 0.0f)
+        z = 0.0f;
+    const uint32_t b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+template <typename K, typename V>
+__device__ __forceinline__ bool model_sample(const K *keys, const V *vals, uint64_t i, const ModelParams &mp,
+                                             const Mat4 &w2e, float &ex, float &ey, float &z, float &zs,
+                                             float c[3])
+{
+    const V val = vals[i];
+    if (!(vw(val) > 0.0f))
+        return false;
+    const float D = vd(val);
+    if (!(fabsf(D) < mp.dmax))
+        return false;
+    uint32_t x, y, zz;
+    unpack_key<K>(keys[i], x, y, zz);
+    c[0] = (float)x * mp.vl + mp.c0;
+    c[1] = (float)y * mp.vl + mp.c0;
+    c[2] = (float)zz * mp.vl + mp.c0;
+    const float p[4] = { c[0], c[1], c[2], 1.0f };
+    float e[4];
+    mat_vec(w2e, p, e);
+    const float depth = -e[2];
+    if (!(depth > mp.znear))
+        return false;
+    ex = e[0];
+    ey = e[1];
+    z = depth;
+    zs = depth + D;
+    return true;
+}
+
+template <typename K, typename V>
+__global__ __launch_bounds__(kIT) void k_model_splat(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                     const uint64_t *__restrict__ d_n, ModelParams mp,
+                                                     const ModelPose *__restrict__ pose,
+                                                     unsigned long long *__restrict__ zbuf)
+{
+    const uint64_t n = *d_n;
+    const Mat4 w2e = pose->w2e;
+    const int W = mp.W, H = mp.H;
+    for (uint64_t i = (uint64_t)blockIdx.x * kIT + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kIT) {
+        float ex, ey, z, zs, c[3];
+        if (!model_sample<K, V>(keys, vals, i, mp, w2e, ex, ey, z, zs, c))
+            continue;
+        const float u = mp.fx * ex / z + mp.cx;
+        const float v = mp.cy - mp.fy * ey / z;
+        if (!(u > -8.0f && u < (float)W + 8.0f && v > -8.0f && v < (float)H + 8.0f))
+            continue;
+        const float kf = ceilf(mp.vl * mp.fx / z);
+        const int k = kf < 1.0f ? 1 : (kf > 3.0f ? 3 : (int)kf);
+        const float half = 0.5f * (float)(k - 1);
+        const int u0 = (int)floorf(u - half), v0 = (int)floorf(v - half);
+        // nearest voxel centre wins (its depth is exact geometry, so the choice
+        // does not select on the noise of D); its surface estimate is zs
+        const unsigned long long key = (unsigned long long)orderable_f(z) << 32 | (uint32_t)i;
+        for (int dv = 0; dv < k; ++dv)
+            for (int du = 0; du < k; ++du) {
+                const int pu = u0 + du, pv = v0 + dv;
+                if (pu < 0 || pu >= W || pv < 0 || pv >= H)
+                    continue;
+                atomicMin(&zbuf[(size_t)pv * W + pu], key);
+            }
+    }
+}
+
+template <typename K, typename V>
+__global__ __launch_bounds__(kIT) void k_model_resolve(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                       ModelParams mp, const ModelPose *__restrict__ pose,
+                                                       const unsigned long long *__restrict__ zbuf,
+                                                       float4 *__restrict__ pts)
+{
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q >= mp.W * mp.H)
+        return;
+    const unsigned long long key = zbuf[q];
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (key != ~0ull) {
+        float ex, ey, z, zs, c[3];
+        model_sample<K, V>(keys, vals, (uint32_t)key, mp, pose->w2e, ex, ey, z, zs, c);
+        const float s = zs / z;
+        o.x = pose->cam[0] + (c[0] - pose->cam[0]) * s;
+        o.y = pose->cam[1] + (c[1] - pose->cam[1]) * s;
+        o.z = pose->cam[2] + (c[2] - pose->cam[2]) * s;
+        o.w = 1.0f;
+    }
+    pts[q] = o;
+}
+
+__global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict__ pts, int W, int H,
+                                                       const ModelPose *__restrict__ pose, float disc,
+                                                       float4 *__restrict__ nrm)
+{
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q >= W * H)
+        return;
+    const int y = q / W, x = q - y * W;
+    const float4 P = pts[q];
+    float4 N = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    constexpr int NB = 4;   // central-difference baseline in pixels
+    if (P.w != 0.0f && x >= NB && x < W - NB && y >= NB && y < H - NB) {
+        const float disc2 = disc * disc;
+        float4 nb[4];
+        bool ok = true;
+        const int dxs[4] = { NB, -NB, 0, 0 }, dys[4] = { 0, 0, NB, -NB };
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const float4 Q = pts[(size_t)(y + dys[d]) * W + (x + dxs[d])];
+            const float a = Q.x - P.x, b = Q.y - P.y, c = Q.z - P.z;
+            ok = ok && Q.w != 0.0f && (a * a + b * b) + c * c < disc2;
+            nb[d] = Q;
+        }
+        if (ok) {
+            const float4 r = nb[0], l = nb[1], dn = nb[2], up = nb[3];
+            const float ux = r.x - l.x, uy = r.y - l.y, uz = r.z - l.z;
+            const float vx = dn.x - up.x, vy = dn.y - up.y, vz = dn.z - up.z;
+            float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+            const float len = sqrtf((nx * nx + ny * ny) + nz * nz);
+            if (len > 0.0f) {
+                nx = nx / len; ny = ny / len; nz = nz / len;
+                const float wx = P.x - pose->cam[0], wy = P.y - pose->cam[1], wz = P.z - pose->cam[2];
+                if ((nx * wx + ny * wy) + nz * wz > 0.0f) {
+                    nx = -nx; ny = -ny; nz = -nz;
+                }
+                N = make_float4(nx, ny, nz, 1.0f);
+            }
+        }
+    }
+    nrm[q] = N;
+}
+
+__global__ void k_fill_zbuf(unsigned long long *__restrict__ p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = ~0ull;
+}
+
+// model pose from a world->eye matrix (host value or device pointer)
+__global__ void k_model_pose(Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev, Mat4 clip,
+                             ModelPose *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    ModelPose p;
+    p.w2e = w2e_dev ? *w2e_dev : w2e_val;
+    mat_mul(clip.m, p.w2e.m, p.proj.m);
+    Mat4 e2w = p.w2e;
+    invert_transform(e2w.m);
+    p.cam[0] = e2w.m[12];
+    p.cam[1] = e2w.m[13];
+    p.cam[2] = e2w.m[14];
+    *out = p;
+}
+
+// ------------------------------------------------------------ depth pyramid
+__global__ void k_depth_level0(const uint16_t *__restrict__ mm, int n, float *__restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = (float)mm[i] * 0.001f;                // DepthStream.cpp:89-90
+}
+
+// 2x2 mean of valid depths in the order (0,0),(1,0),(0,1),(1,1)
+__global__ void k_depth_down(const float *__restrict__ src, int W, int H, float *__restrict__ dst)
+{
+    const int w = W / 2, h = H / 2;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= w * h)
+        return;
+    const int y = q / w, x = q - y * w;
+    float s = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float d = src[(size_t)(2 * y + j) * W + 2 * x + i];
+            if (d > 0.0f) {
+                s += d;
+                c += 1.0f;
+            }
+        }
+    dst[q] = c > 0.0f ? s / c : 0.0f;
+}
+
+// ------------------------------------------------------------ compensated sums
+struct CSum {
+    float s, c;
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    const float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+__device__ __forceinline__ void cadd(CSum &x, float s2, float c2)
+{
+    float t, e;
+    two_sum(x.s, s2, t, e);
+    x.s = t;
+    x.c = x.c + c2 + e;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_csum(CSum (&v)[NV], CSum *lds /* [4][NV] */)
+{
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float s2 = __shfl_xor(v[j].s, o, kWave), c2 = __shfl_xor(v[j].c, o, kWave);
+            cadd(v[j], s2, c2);
+        }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            lds[w * NV + j] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 1; q < kIT / kWave; ++q)
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                cadd(v[j], lds[q * NV + j].s, lds[q * NV + j].c);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ association
+// NV = 29 for point-to-plane (21 upper-triangular A, 6 b, count, r^2),
+//      17 for Horn (sum p (3), sum q (3), sum p q^T (9), count, r^2) about origin o.
+template <int MODE>
+struct IcpNV;
+template <> struct IcpNV<KIFI_ICP_POINT_TO_PLANE> { static constexpr int nv = 29; };
+template <> struct IcpNV<KIFI_ICP_HORN> { static constexpr int nv = 17; };
+
+template <int MODE>
+__global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, const ModelPose *__restrict__ mpose,
+                                                   const float4 *__restrict__ mpts,
+                                                   const float4 *__restrict__ mnrm,
+                                                   const IcpState *__restrict__ st, float dist_thresh,
+                                                   CSum *__restrict__ partials)
+{
+    constexpr int NV = IcpNV<MODE>::nv;
+    __shared__ CSum lds[(kIT / kWave) * NV];
+    if (st->level_done)
+        return;
+    CSum acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+        acc[j] = CSum{ 0.0f, 0.0f };
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q < lv.W * lv.H) {
+        const int v = q / lv.W, u = q - v * lv.W;
+        const float d = lv.depth[q];
+        if (d > 0.0f) {
+            const float pe[4] = { ((float)u + 0.5f - lv.cx) / lv.fx * d, (lv.cy - (float)v - 0.5f) / lv.fy * d, -d,
+                                  1.0f };
+            float p[4];
+            mat_vec(st->pose_f, pe, p);
+            float e[4];
+            mat_vec(mpose->w2e, p, e);
+            const float z = -e[2];
+            if (z > 0.0f) {
+                const float fu = mp.fx * e[0] / z + mp.cx, fv = mp.cy - mp.fy * e[1] / z;
+                if (fu >= 0.0f && fu < (float)mp.W && fv >= 0.0f && fv < (float)mp.H) {
+                    const size_t mq = (size_t)(int)fv * mp.W + (int)fu;
+                    const float4 Q = mpts[mq], N = mnrm[mq];
+                    if (Q.w != 0.0f && N.w != 0.0f) {
+                        const float dx = Q.x - p[0], dy = Q.y - p[1], dz = Q.z - p[2];
+                        if ((dx * dx + dy * dy) + dz * dz <= dist_thresh * dist_thresh) {
+                            if (MODE == KIFI_ICP_POINT_TO_PLANE) {
+                                const float r = (N.x * dx + N.y * dy) + N.z * dz;
+                                const float J[6] = { p[1] * N.z - p[2] * N.y, p[2] * N.x - p[0] * N.z,
+                                                     p[0] * N.y - p[1] * N.x, N.x, N.y, N.z };
+                                // inverse variance of the sensor noise model sigma ~ z^2
+                                const float iz = 1.0f / d, wgt = iz * iz * iz * iz;
+                                int t = 0;
+#pragma unroll
+                                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                                    for (int j = 0; j <= i; ++j)
+                                        acc[t++].s = wgt * J[i] * J[j];
+#pragma unroll
+                                for (int i = 0; i < 6; ++i)
+                                    acc[21 + i].s = wgt * J[i] * r;
+                                acc[27].s = 1.0f;
+                                acc[28].s = r * r;
+                            } else {
+                                const float a[3] = { p[0] - mpose->cam[0], p[1] - mpose->cam[1], p[2] - mpose->cam[2] };
+                                const float b[3] = { Q.x - mpose->cam[0], Q.y - mpose->cam[1], Q.z - mpose->cam[2] };
+#pragma unroll
+                                for (int i = 0; i < 3; ++i) {
+                                    acc[i].s = a[i];
+                                    acc[3 + i].s = b[i];
+#pragma unroll
+                                    for (int j = 0; j < 3; ++j)
+                                        acc[6 + 3 * i + j].s = a[i] * b[j];
+                                }
+                                acc[15].s = 1.0f;
+                                acc[16].s = (dx * dx + dy * dy) + dz * dz;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    block_csum<NV>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            partials[(size_t)blockIdx.x * NV + j] = acc[j];
+}
+
+// ICP::AlignStep association (ICP.cpp:66-131): every 4th column, back-projected
+// in the reference's form, projected with EyeToClipRH * dstWorldToEye, nearest
+// model point of a 21 x 21 window within 10 cm (strictly smaller wins, scan order).
+// The window clamps to the model map (the reference hard-codes 640 x 480).
+// Sums about the model camera centre o: sum a, sum b, sum a b^T, count, sum |b-a|^2.
+__global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelParams mp,
+                                                          const ModelPose *__restrict__ mpose,
+                                                          const float4 *__restrict__ mpts,
+                                                          const IcpState *__restrict__ st,
+                                                          CSum *__restrict__ partials)
+{
+    constexpr int NV = IcpNV<KIFI_ICP_HORN>::nv;
+    __shared__ CSum lds[(kIT / kWave) * NV];
+    if (st->level_done)
+        return;
+    CSum acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+        acc[j] = CSum{ 0.0f, 0.0f };
+    const int cols = (lv.W + 3) / 4;
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q < cols * lv.H) {
+        const int y = q / cols, x = 4 * (q - y * cols);
+        const float d = lv.depth[(size_t)y * lv.W + x];
+        if (0.0f != d) {
+            const float flInvX = 1.0f / lv.fx, flInvY = 1.0f / lv.fy;     // :50-57
+            const float ppX = (0.5f - lv.cx) * flInvX, ppY = (lv.cy - 0.5f) * flInvY;
+            float pt[4] = { (float)x, -(float)y, -1.0f, 0.0f };          // :74-77
+            pt[0] = (pt[0] * flInvX + ppX) * d;
+            pt[1] = (pt[1] * flInvY + ppY) * d;
+            pt[2] = (pt[2] * 1.0f + 0.0f) * d;
+            pt[3] = 1.0f;
+            float p[4];
+            mat_vec(st->pose_f, pt, p);                                   // :79
+            float uv[4];
+            mat_vec(mpose->proj, p, uv);                                  // :81
+            const float w = uv[3];
+            const float ux = uv[0] / w, uy = uv[1] / w;
+            const float halfW = (float)mp.W * 0.5f, halfH = (float)mp.H * 0.5f;
+            const float fu = ux * halfW + halfW, fv = uy * halfH + halfH;
+            if (fu > -2147483648.0f && fu < 2147483648.0f && fv > -2147483648.0f && fv < 2147483648.0f) {
+                int u = (int)fu - 10;                                     // :100-104
+                int v = mp.H - 1 - (int)fv - 10;
+                u = u < 0 ? 0 : u;
+                v = v < 0 ? 0 : v;
+                const int yend = min(mp.H, v + 21), xend = min(mp.W, u + 21);
+                float mind = 0.01f;
+                float4 best = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int y1 = v; y1 < yend; ++y1)                         // :108-125
+                    for (int x1 = u; x1 < xend; ++x1) {
+                        const float4 Q = mpts[(size_t)y1 * mp.W + x1];
+                        if (Q.x == 0.0f && Q.y == 0.0f && Q.z == 0.0f)
+                            continue;
+                        const float dx = Q.x - p[0], dy = Q.y - p[1], dz = Q.z - p[2], dw = 1.0f - p[3];
+                        const float l2 = (dx * dx + dy * dy) + (dz * dz + dw * dw);
+                        if (l2 < mind) {
+                            mind = l2;
+                            best = Q;
+                        }
+                    }
+                if (mind < 0.01f) {                                       // :126-131
+                    const float a[3] = { p[0] - mpose->cam[0], p[1] - mpose->cam[1], p[2] - mpose->cam[2] };
+                    const float b[3] = { best.x - mpose->cam[0], best.y - mpose->cam[1], best.z - mpose->cam[2] };
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        acc[i].s = a[i];
+                        acc[3 + i].s = b[i];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                            acc[6 + 3 * i + j].s = a[i] * b[j];
+                    }
+                    acc[15].s = 1.0f;
+                    acc[16].s = mind;
+                }
+            }
+        }
+    }
+    block_csum<NV>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            partials[(size_t)blockIdx.x * NV + j] = acc[j];
+}
+
+// ---------------------------------------------------------------- FP64 solve
+__device__ bool cholesky6(const double *A, const double *b, double *x)
+{
+    double L[36];
+    for (int i = 0; i < 36; ++i)
+        L[i] = 0.0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = A[6 * i + j];
+            for (int k = 0; k < j; ++k)
+                s -= L[6 * i + k] * L[6 * j + k];
+            if (i == j) {
+                if (!(s > 0.0))
+                    return false;
+                L[6 * i + i] = sqrt(s);
+            } else {
+                L[6 * i + j] = s / L[6 * j + j];
+            }
+        }
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k)
+            s -= L[6 * i + k] * y[k];
+        y[i] = s / L[6 * i + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k)
+            s -= L[6 * k + i] * x[k];
+        x[i] = s / L[6 * i + i];
+    }
+    return true;
+}
+
+__device__ void twist_to_mat(const double xi[6], double *M)
+{
+    const double wx = xi[0], wy = xi[1], wz = xi[2];
+    const double th = sqrt(wx * wx + wy * wy + wz * wz);
+    const double K[9] = { 0, wz, -wy, -wz, 0, wx, wy, -wx, 0 };
+    double a, b;
+    if (th < 1e-12) {
+        a = 1.0;
+        b = 0.5;
+    } else {
+        a = sin(th) / th;
+        b = (1.0 - cos(th)) / (th * th);
+    }
+    for (int i = 0; i < 16; ++i)
+        M[i] = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double k2 = 0.0;
+            for (int k = 0; k < 3; ++k)
+                k2 += K[r + 3 * k] * K[k + 3 * c];
+            M[r + 4 * c] = (r == c ? 1.0 : 0.0) + a * K[r + 3 * c] + b * k2;
+        }
+    M[12] = xi[3];
+    M[13] = xi[4];
+    M[14] = xi[5];
+    M[15] = 1.0;
+}
+
+__device__ void mat_mul_d(const double *A, const double *B, double *C)
+{
+    double R[16];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 4; ++k)
+                s += A[r + 4 * k] * B[k + 4 * c];
+            R[r + 4 * c] = s;
+        }
+    for (int i = 0; i < 16; ++i)
+        C[i] = R[i];
+}
+
+// cyclic Jacobi eigen decomposition of a symmetric 4x4 (FP64); V columns = vectors
+__device__ void jacobi4(double A[4][4], double V[16], double ev[4])
+{
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+            V[r + 4 * c] = r == c ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) {
+                if (r != c) off += A[r][c] * A[r][c];
+                else diag += A[r][c] * A[r][c];
+            }
+        if (off <= 1e-30 * diag || off == 0.0)
+            break;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                const double apq = A[p][q];
+                if (apq == 0.0)
+                    continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 4; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const double vkp = V[k + 4 * p], vkq = V[k + 4 * q];
+                    V[k + 4 * p] = c * vkp - s * vkq;
+                    V[k + 4 * q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < 4; ++i)
+        ev[i] = A[i][i];
+}
+
+// Horn's closed form (ICP.cpp:199-264) from centred cross-covariance, FP64:
+// returns the rigid increment D mapping the src points onto the dst points.
+__device__ void horn_increment(const double S[9], const double ms[3], const double md[3], double D[16])
+{
+    const double Sxx = S[0], Sxy = S[1], Sxz = S[2], Syx = S[3], Syy = S[4], Syz = S[5], Szx = S[6],
+                 Szy = S[7], Szz = S[8];
+    double N[4][4] = { { Sxx + Syy + Szz, Syz - Szy, Szx - Sxz, Sxy - Syx },
+                       { Syz - Szy, Sxx - Syy - Szz, Sxy + Syx, Szx + Sxz },
+                       { Szx - Sxz, Sxy + Syx, Syy - Sxx - Szz, Syz + Szy },
+                       { Sxy - Syx, Szx + Sxz, Syz + Szy, Szz - Sxx - Syy } };
+    double V[16], ev[4];
+    jacobi4(N, V, ev);
+    int im = 0;
+    for (int i = 1; i < 4; ++i)
+        if (ev[i] > ev[im])
+            im = i;
+    double q[4] = { V[4 * im], V[4 * im + 1], V[4 * im + 2], V[4 * im + 3] };
+    const double len = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i)
+        q[i] /= len;
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    double R[9];
+    R[0] = 1 - 2 * (z * z + w * w); R[3] = 2 * (y * z - x * w); R[6] = 2 * (y * w + x * z);
+    R[1] = 2 * (y * z + x * w); R[4] = 1 - 2 * (y * y + w * w); R[7] = 2 * (z * w - x * y);
+    R[2] = 2 * (y * w - x * z); R[5] = 2 * (z * w + x * y); R[8] = 1 - 2 * (y * y + z * z);
+    for (int i = 0; i < 16; ++i)
+        D[i] = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            D[r + 4 * c] = R[r + 3 * c];
+    for (int r = 0; r < 3; ++r)
+        D[12 + r] = md[r] - (R[r] * ms[0] + R[r + 3] * ms[1] + R[r + 6] * ms[2]);
+    D[15] = 1.0;
+}
+
+// Reduce the block partials (compensated), solve in FP64, compose the pose.
+template <int MODE>
+__global__ __launch_bounds__(kIT) void k_icp_solve(const CSum *__restrict__ partials, int nblocks,
+                                                   IcpState *__restrict__ st, int level, IcpSolveParams sp,
+                                                   const ModelPose *__restrict__ mpose)
+{
+    constexpr int NV = IcpNV<MODE>::nv;
+    __shared__ CSum lds[(kIT / kWave) * NV];
+    if (st->level_done)
+        return;
+    CSum acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+        acc[j] = CSum{ 0.0f, 0.0f };
+    for (int b = threadIdx.x; b < nblocks; b += kIT)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            cadd(acc[j], partials[(size_t)b * NV + j].s, partials[(size_t)b * NV + j].c);
+    block_csum<NV>(acc, lds);
+    if (threadIdx.x != 0)
+        return;
+    double v[NV];
+    for (int j = 0; j < NV; ++j)
+        v[j] = (double)acc[j].s + (double)acc[j].c;
+    const double cnt = v[NV - 2], e2 = v[NV - 1];
+    st->rms = cnt > 0.0 ? (float)sqrt(e2 / cnt) : 0.0f;
+    st->inliers = (int)cnt;
+    if (cnt < (double)sp.min_inliers) {
+        st->level_done = 1;
+        st->failed = 1;
+        return;
+    }
+    double D[16], xi[6];
+    bool ok;
+    if (MODE == KIFI_ICP_POINT_TO_PLANE) {
+        double A[36], b[6];
+        int t = 0;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j <= i; ++j) {
+                A[6 * i + j] = v[t];
+                A[6 * j + i] = v[t];
+                ++t;
+            }
+        for (int i = 0; i < 6; ++i)
+            b[i] = v[21 + i];
+        ok = cholesky6(A, b, xi);
+        if (ok)
+            twist_to_mat(xi, D);
+    } else {
+        const double ms[3] = { v[0] / cnt, v[1] / cnt, v[2] / cnt }, md[3] = { v[3] / cnt, v[4] / cnt, v[5] / cnt };
+        double S[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                S[3 * i + j] = v[6 + 3 * i + j] - cnt * ms[i] * md[j];
+        double Dl[16];
+        horn_increment(S, ms, md, Dl);
+        // increment about the model camera origin o: D = T(o) Dl T(-o)
+        const double o[3] = { mpose->cam[0], mpose->cam[1], mpose->cam[2] };
+        for (int i = 0; i < 16; ++i)
+            D[i] = Dl[i];
+        for (int r = 0; r < 3; ++r)
+            D[12 + r] = Dl[12 + r] + o[r] - (Dl[r] * o[0] + Dl[r + 4] * o[1] + Dl[r + 8] * o[2]);
+        // twist magnitude for the convergence test: rotation angle and translation
+        const double tr = Dl[0] + Dl[5] + Dl[10];
+        const double ang = acos(fmin(1.0, fmax(-1.0, 0.5 * (tr - 1.0))));
+        xi[0] = ang; xi[1] = 0; xi[2] = 0;
+        xi[3] = D[12]; xi[4] = D[13]; xi[5] = D[14];
+        ok = true;
+    }
+    if (!ok) {
+        st->level_done = 1;
+        st->failed = 1;
+        return;
+    }
+    mat_mul_d(D, st->pose, st->pose);
+    for (int i = 0; i < 16; ++i)
+        st->pose_f.m[i] = (float)st->pose[i];
+    st->iters[level] += 1;
+    st->total_iters += 1;
+    const double rot = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    const double tr = sqrt(xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
+    if (rot < sp.eps_rot && tr < sp.eps_trans)
+        st->level_done = 1;
+}
+
+// state control kernels
+__global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const double *g = guess_dev ? guess_dev : guess_val.m;
+    for (int i = 0; i < 16; ++i) {
+        st->pose[i] = g[i];
+        st->guess[i] = g[i];
+        st->pose_f.m[i] = (float)g[i];
+    }
+    for (int l = 0; l < 4; ++l)
+        st->iters[l] = 0;
+    st->total_iters = 0;
+    st->level_done = 0;
+    st->failed = 0;
+    st->active = 1;
+    st->fallback = 0;
+    st->rms = 0.0f;
+    st->inliers = 0;
+}
+
+__global__ void k_icp_level_begin(IcpState *__restrict__ st)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        st->level_done = st->active ? 0 : 1;
+}
+
+// Fallback gate (no host round trip): when the first pass started from an
+// external guess and its residual or iteration count exceeds the limits, the
+// second pass re-runs plain ICP from the previous frame's pose.
+__global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, int max_iters)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const bool bad = st->failed || (max_rms > 0.0f && st->rms > max_rms) ||
+                     (max_iters > 0 && st->total_iters >= max_iters);
+    if (bad) {
+        st->fallback = 1;
+        st->active = 1;
+        st->failed = 0;
+        for (int i = 0; i < 16; ++i) {
+            st->pose[i] = st->prev[i];
+            st->pose_f.m[i] = (float)st->prev[i];
+        }
+    } else {
+        st->active = 0;
+    }
+}
+
+__global__ void k_icp_frame_end(IcpState *__restrict__ st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    for (int i = 0; i < 16; ++i)
+        st->prev[i] = st->pose[i];
+    st->sum_iters += st->total_iters;
+    st->frames_tracked += 1;
+    st->fallbacks += st->fallback;
+}
+
+// Sets the pose without tracking (first frame or a given pose).
+__global__ void k_icp_set_pose(IcpState *__restrict__ st, Pose64 p)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    for (int i = 0; i < 16; ++i) {
+        st->pose[i] = p.m[i];
+        st->prev[i] = p.m[i];
+        st->pose_f.m[i] = (float)p.m[i];
+    }
+}
+
+// world->eye binary32 pose given by the caller -> FP64 eye->world state
+__global__ void k_icp_set_pose_w2e(IcpState *__restrict__ st, Mat4 w2e)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    Mat4 e = w2e;
+    invert_transform(e.m);
+    for (int i = 0; i < 16; ++i) {
+        st->pose[i] = e.m[i];
+        st->prev[i] = e.m[i];
+        st->pose_f.m[i] = e.m[i];
+    }
+}
+
+// world->eye (float) of the tracked pose, as Pipeline.cpp:122-123 derives it
+__global__ void k_icp_w2e(const IcpState *__restrict__ st, Mat4 *__restrict__ w2e)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    Mat4 m = st->pose_f;
+    invert_transform(m.m);
+    *w2e = m;
+}
+
+// ====================================================================== host
+static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <typename K, typename V>
+static void launch_model(kifi_model *m, kifi_volume *vol, hipStream_t st)
+{
+    ModelParams mp = m->mp;
+    mp.vl = vol->consts.vl;
+    mp.c0 = vol->consts.vlHalfPlusMin;
+    m->mp = mp;
+    const int P = mp.W * mp.H;
+    const K *keys = (const K *)vol->keys[vol->cur];
+    const V *vals = (const V *)vol->vals[vol->cur];
+    hipLaunchKernelGGL(k_fill_zbuf, dim3((P + 255) / 256), dim3(256), 0, st, m->d_zbuf, P);
+    const unsigned grid = (unsigned)std::max(1, vol->ctx->num_cus * 8);
+    hipLaunchKernelGGL((k_model_splat<K, V>), dim3(grid), dim3(kIT), 0, st, keys, vals, vol->d_size, mp, m->d_pose,
+                       m->d_zbuf);
+    hipLaunchKernelGGL((k_model_resolve<K, V>), dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, keys, vals, mp,
+                       m->d_pose, m->d_zbuf, m->d_pts);
+    hipLaunchKernelGGL(k_model_normals, dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, m->d_pts, mp.W, mp.H,
+                       m->d_pose, m->disc, m->d_nrm);
+}
+
+kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev)
+{
+    kifi_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    Mat4 wv;
+    for (int i = 0; i < 16; ++i)
+        wv.m[i] = w2e_host ? w2e_host[i] : 0.0f;
+    Mat4 clip;
+    eye_to_clip_rh(m->cam.width, m->cam.height, m->cam.fx, m->cam.fy, m->cam.cx, m->cam.cy, m->cam.znear,
+                   m->cam.zfar, clip.m);
+    const int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    hipLaunchKernelGGL(k_model_pose, dim3(1), dim3(64), 0, st, wv, w2e_dev, clip, m->d_pose);
+    if (vol->desc.key_bits == 32) {
+        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint32_t, VoxF32i>(m, vol, st);
+        else launch_model<uint32_t, uint32_t>(m, vol, st);
+    } else {
+        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint64_t, VoxF32i>(m, vol, st);
+        else launch_model<uint64_t, uint32_t>(m, vol, st);
+    }
+    timer_end(ctx, KIFI_STAGE_MODEL, t_m);
+    KIFI_HIP(ctx, hipGetLastError());
+    return KIFI_OK;
+}
+
+kifi_status model_alloc(kifi_model *m)
+{
+    kifi_ctx *ctx = m->ctx;
+    const size_t P = (size_t)m->cam.width * m->cam.height;
+    KIFI_HIP(ctx, hipMalloc(&m->d_pts, P * sizeof(float4)));
+    KIFI_HIP(ctx, hipMalloc(&m->d_nrm, P * sizeof(float4)));
+    KIFI_HIP(ctx, hipMalloc(&m->d_zbuf, P * 8));
+    KIFI_HIP(ctx, hipMalloc(&m->d_pose, sizeof(ModelPose)));
+    KIFI_HIP(ctx, hipMemset(m->d_pts, 0, P * sizeof(float4)));
+    KIFI_HIP(ctx, hipMemset(m->d_nrm, 0, P * sizeof(float4)));
+    return KIFI_OK;
+}
+
+void model_free(kifi_model *m)
+{
+    if (m->d_pts) (void)hipFree(m->d_pts);
+    if (m->d_nrm) (void)hipFree(m->d_nrm);
+    if (m->d_zbuf) (void)hipFree(m->d_zbuf);
+    if (m->d_pose) (void)hipFree(m->d_pose);
+}
+
+kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H)
+{
+    const size_t P = (size_t)W * H;
+    size_t bytes = al256(P * 4) + al256(P / 4 * 4 + 4) + al256(P / 16 * 4 + 4);
+    const size_t nb = (P + kIT - 1) / kIT;
+    bytes += al256(nb * 29 * sizeof(CSum));
+    KIFI_HIP(ctx, hipMalloc(&w->buf, bytes));
+    char *p = (char *)w->buf;
+    w->lv[0] = (float *)p; p += al256(P * 4);
+    w->lv[1] = (float *)p; p += al256(P / 4 * 4 + 4);
+    w->lv[2] = (float *)p; p += al256(P / 16 * 4 + 4);
+    w->partials = p;
+    KIFI_HIP(ctx, hipMalloc(&w->d_state, sizeof(IcpState)));
+    KIFI_HIP(ctx, hipMemset(w->d_state, 0, sizeof(IcpState)));
+    w->W = W;
+    w->H = H;
+    return KIFI_OK;
+}
+
+void icp_free(IcpWork *w)
+{
+    if (w->buf) (void)hipFree(w->buf);
+    if (w->d_state) (void)hipFree(w->d_state);
+    w->buf = nullptr;
+    w->d_state = nullptr;
+}
+
+static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels);
+
+// Runs the whole pyramid schedule asynchronously on the context's stream.
+kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
+                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip)
+{
+    kifi_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const int W = cam->width, H = cam->height;
+    const int P = W * H;
+    Pose64 gv;
+    for (int i = 0; i < 16; ++i)
+        gv.m[i] = guess_host ? guess_host[i] : 0.0;
+    const int t_icp = timer_begin(ctx, KIFI_STAGE_ICP);
+    hipLaunchKernelGGL(k_icp_begin, dim3(1), dim3(64), 0, st, w->d_state, guess_dev, gv);
+    hipLaunchKernelGGL(k_depth_level0, dim3((P + 255) / 256), dim3(256), 0, st, d_depth_mm, P, w->lv[0]);
+    const int levels = ip->levels < 1 ? 1 : (ip->levels > 3 ? 3 : ip->levels);
+    int Wl[3] = { W, W / 2, W / 4 }, Hl[3] = { H, H / 2, H / 4 };
+    for (int l = 1; l < levels; ++l)
+        hipLaunchKernelGGL(k_depth_down, dim3((Wl[l] * Hl[l] + 255) / 256), dim3(256), 0, st, w->lv[l - 1],
+                           Wl[l - 1], Hl[l - 1], w->lv[l]);
+    icp_pass(m, w, cam, ip, levels);
+    if (ip->fallback_rms > 0.0f || ip->fallback_iters > 0) {
+        hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
+                           ip->fallback_iters);
+        icp_pass(m, w, cam, ip, levels);
+    }
+    hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state);
+    timer_end(ctx, KIFI_STAGE_ICP, t_icp);
+    KIFI_HIP(ctx, hipGetLastError());
+    return KIFI_OK;
+}
+
+void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st)
+{
+    if (e2w) {
+        Pose64 p;
+        for (int i = 0; i < 16; ++i)
+            p.m[i] = e2w[i];
+        hipLaunchKernelGGL(k_icp_set_pose, dim3(1), dim3(64), 0, st, w->d_state, p);
+    } else {
+        Mat4 m;
+        for (int i = 0; i < 16; ++i)
+            m.m[i] = w2e[i];
+        hipLaunchKernelGGL(k_icp_set_pose_w2e, dim3(1), dim3(64), 0, st, w->d_state, m);
+    }
+}
+
+static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels)
+{
+    hipStream_t st = m->ctx->stream;
+    const int W = cam->width, H = cam->height;
+    int Wl[3] = { W, W / 2, W / 4 }, Hl[3] = { H, H / 2, H / 4 };
+    IcpSolveParams sp;
+    sp.eps_rot = ip->eps_rot;
+    sp.eps_trans = ip->eps_trans;
+    sp.min_inliers = ip->min_inliers;
+    // Horn mode is the reference's ICP::Align: full resolution only
+    for (int l = ip->mode == KIFI_ICP_HORN ? 0 : levels - 1; l >= 0; --l) {
+        IcpLevel lv;
+        const float sc = 1.0f / (float)(1 << l);
+        lv.W = Wl[l];
+        lv.H = Hl[l];
+        lv.fx = cam->fx * sc;
+        lv.fy = cam->fy * sc;
+        lv.cx = cam->cx * sc;
+        lv.cy = cam->cy * sc;
+        lv.depth = w->lv[l];
+        const int nb = (lv.W * lv.H + kIT - 1) / kIT;
+        hipLaunchKernelGGL(k_icp_level_begin, dim3(1), dim3(64), 0, st, w->d_state);
+        for (int it = 0; it < ip->iters[l]; ++it) {
+            if (ip->mode == KIFI_ICP_HORN) {
+                const int nbw = (((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT;
+                hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
+                                   w->d_state, (CSum *)w->partials);
+                hipLaunchKernelGGL(k_icp_solve<KIFI_ICP_HORN>, dim3(1), dim3(kIT), 0, st, (const CSum *)w->partials,
+                                   nbw, w->d_state, l, sp, m->d_pose);
+            } else {
+                hipLaunchKernelGGL(k_icp_assoc<KIFI_ICP_POINT_TO_PLANE>, dim3(nb), dim3(kIT), 0, st, lv, m->mp,
+                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, ip->dist_thresh,
+                                   (CSum *)w->partials);
+                hipLaunchKernelGGL(k_icp_solve<KIFI_ICP_POINT_TO_PLANE>, dim3(1), dim3(kIT), 0, st,
+                                   (const CSum *)w->partials, nb, w->d_state, l, sp, m->d_pose);
+            }
+        }
+    }
+}
+
+void icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_icp_w2e, dim3(1), dim3(64), 0, st, w->d_state, d_out);
+}
+
+// ------------------------------------------------ Horn from correspondences
+// GPU twin of ICP.cpp:166-264 given explicit correspondences: two compensated
+// passes (centroids, then centred cross-covariance), FP64 Horn solve.
+__global__ __launch_bounds__(kIT) void k_horn_sums(const float *__restrict__ src, const float *__restrict__ dst,
+                                                   uint32_t n, const double *__restrict__ means, CSum *__restrict__ out)
+{
+    __shared__ CSum lds[(kIT / kWave) * 9];
+    CSum acc[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        acc[j] = CSum{ 0.0f, 0.0f };
+    float ms[3] = { 0, 0, 0 }, md[3] = { 0, 0, 0 };
+    if (means)
+        for (int i = 0; i < 3; ++i) {
+            ms[i] = (float)means[i];
+            md[i] = (float)means[3 + i];
+        }
+    for (uint32_t i = blockIdx.x * kIT + threadIdx.x; i < n; i += gridDim.x * kIT) {
+        float a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = src[3 * (size_t)i + k] - ms[k];
+            b[k] = dst[3 * (size_t)i + k] - md[k];
+        }
+        if (!means) {
+            for (int k = 0; k < 3; ++k) {
+                cadd(acc[k], a[k], 0.0f);
+                cadd(acc[3 + k], b[k], 0.0f);
+            }
+        } else {
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    cadd(acc[3 * r + c], a[r] * b[c], 0.0f);
+        }
+    }
+    block_csum<9>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 9; ++j)
+            out[(size_t)blockIdx.x * 9 + j] = acc[j];
+}
+
+__global__ __launch_bounds__(kIT) void k_horn_finish(const CSum *__restrict__ part, int nb, uint32_t n, int phase,
+                                                     double *__restrict__ means, float *__restrict__ out)
+{
+    __shared__ CSum lds[(kIT / kWave) * 9];
+    CSum acc[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+        acc[j] = CSum{ 0.0f, 0.0f };
+    for (int b = threadIdx.x; b < nb; b += kIT)
+        for (int j = 0; j < 9; ++j)
+            cadd(acc[j], part[(size_t)b * 9 + j].s, part[(size_t)b * 9 + j].c);
+    block_csum<9>(acc, lds);
+    if (threadIdx.x != 0)
+        return;
+    double v[9];
+    for (int j = 0; j < 9; ++j)
+        v[j] = (double)acc[j].s + (double)acc[j].c;
+    if (phase == 0) {
+        // centroids (ICP.cpp:166-167), rounded to binary32 as the reference keeps them
+        for (int j = 0; j < 6; ++j)
+            means[j] = (double)(float)(v[j] / (double)n);
+        return;
+    }
+    const double ms[3] = { means[0], means[1], means[2] }, md[3] = { means[3], means[4], means[5] };
+    double D[16];
+    horn_increment(v, ms, md, D);
+    for (int i = 0; i < 16; ++i)
+        out[i] = (float)D[i];
+}
+
+kifi_status horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16])
+{
+    hipStream_t st = ctx->stream;
+    const int nb = (int)std::min<size_t>((n + kIT - 1) / kIT, 1024);
+    const size_t bytes = al256(n * 12) * 2 + al256((size_t)nb * 9 * sizeof(CSum)) + al256(64) + al256(64);
+    kifi_status s = ensure_scratch(ctx, bytes);
+    if (s != KIFI_OK)
+        return s;
+    char *p = (char *)ctx->scratch;
+    float *ds = (float *)p; p += al256(n * 12);
+    float *dd = (float *)p; p += al256(n * 12);
+    CSum *part = (CSum *)p; p += al256((size_t)nb * 9 * sizeof(CSum));
+    double *means = (double *)p; p += al256(64);
+    float *dout = (float *)p;
+    KIFI_HIP(ctx, hipMemcpyAsync(ds, src, n * 12, hipMemcpyHostToDevice, st));
+    KIFI_HIP(ctx, hipMemcpyAsync(dd, dst, n * 12, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_horn_sums, dim3(nb), dim3(kIT), 0, st, ds, dd, (uint32_t)n, (const double *)nullptr, part);
+    hipLaunchKernelGGL(k_horn_finish, dim3(1), dim3(kIT), 0, st, part, nb, (uint32_t)n, 0, means, dout);
+    hipLaunchKernelGGL(k_horn_sums, dim3(nb), dim3(kIT), 0, st, ds, dd, (uint32_t)n, means, part);
+    hipLaunchKernelGGL(k_horn_finish, dim3(1), dim3(kIT), 0, st, part, nb, (uint32_t)n, 1, means, dout);
+    KIFI_HIP(ctx, hipGetLastError());
+    KIFI_HIP(ctx, hipMemcpyAsync(out, dout, 64, hipMemcpyDeviceToHost, st));
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    return KIFI_OK;
+}
+
+} // namespace kifi

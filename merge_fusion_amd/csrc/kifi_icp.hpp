@@ -1,0 +1,90 @@
+// kifi_icp.hpp -- device state shared by the model-prediction / ICP kernels and
+// the pipeline host code.
+#pragma once
+
+#include "kifi_internal.hpp"
+
+namespace kifi {
+
+struct Pose64 {
+    double m[16];   // eye->world, column storage
+};
+
+// world->eye of the model camera and its centre in world coordinates
+struct ModelPose {
+    Mat4  w2e;
+    Mat4  proj;     // EyeToClipRH() * w2e (the reference's association projection)
+    float cam[3];
+};
+
+struct ModelParams {
+    int   W, H;
+    float fx, fy, cx, cy;   // model camera intrinsics (full resolution)
+    float vl, c0;           // voxel length, VoxelCenter offset
+    float dmax;             // near-surface band |D| < dmax
+    float znear;
+};
+
+struct IcpLevel {
+    int   W, H;
+    float fx, fy, cx, cy;
+    const float *depth;     // metres
+};
+
+struct IcpSolveParams {
+    double eps_rot, eps_trans;
+    int    min_inliers;
+};
+
+// Device-resident tracking state of one frame (plus running totals).
+struct IcpState {
+    double pose[16];        // current eye->world estimate (FP64)
+    double guess[16];       // the initial guess of this frame
+    double prev[16];        // the previous frame's pose (fallback start)
+    Mat4   pose_f;          // binary32 copy used by the association kernels
+    int    iters[4];
+    int    total_iters;
+    int    level_done;
+    int    failed;
+    int    active;          // gate of the current pass
+    int    fallback;        // this frame re-ran plain ICP
+    float  rms;
+    int    inliers;
+    // running totals over frames
+    long long sum_iters;
+    long long frames_tracked;
+    long long fallbacks;
+};
+
+struct IcpWork {
+    void    *buf = nullptr;
+    float   *lv[3] = { nullptr, nullptr, nullptr };
+    void    *partials = nullptr;
+    IcpState *d_state = nullptr;
+    int      W = 0, H = 0;
+};
+
+} // namespace kifi
+
+struct kifi_model {
+    kifi_ctx           *ctx = nullptr;
+    kifi_intrinsics     cam{};
+    kifi::ModelParams   mp{};
+    float               disc = 0.15f;    // depth-discontinuity gate for normals (m)
+    float4             *d_pts = nullptr; // world points, w = valid
+    float4             *d_nrm = nullptr; // unit normals, w = valid
+    unsigned long long *d_zbuf = nullptr;
+    kifi::ModelPose    *d_pose = nullptr;
+};
+
+namespace kifi {
+kifi_status model_alloc(kifi_model *m);
+void        model_free(kifi_model *m);
+kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
+kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
+void        icp_free(IcpWork *w);
+kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
+                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip);
+void        icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st);
+kifi_status horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16]);
+} // namespace kifi

@@ -36,8 +36,24 @@ struct KeygenParams {
     int   W, H;
     float flInvX, flInvY, ppX, ppY, maxIndex;
     VolumeConsts vc;
-    Mat4  e2w;
+    const FrameMats *mats;   // device: e2w of this frame
 };
+
+// Integrator.cpp:41-42: worldToClip = EyeToClipRH() * worldToEye and
+// eyeToWorld = invert_transform(worldToEye), in binary32 on the device so that a
+// device-resident pose (tracking) and a host pose give the same bits.
+__global__ void k_frame_setup(Mat4 clip, Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev,
+                              FrameMats *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    FrameMats f;
+    f.w2e = w2e_dev ? *w2e_dev : w2e_val;
+    mat_mul(clip.m, f.w2e.m, f.w2c.m);
+    f.e2w = f.w2e;
+    invert_transform(f.e2w.m);
+    *out = f;
+}
 
 // Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
 // the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
@@ -64,7 +80,7 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict_
             q[1] = q[1] * d + 0.0f;
             q[2] = q[2] * d + 0.0f;
             q[3] = q[3] * d + 1.0f;
-            mat_vec(p.e2w, q, q);                              // :141-144
+            mat_vec(p.mats->e2w, q, q);                        // :141-144
             const float ix = q[0] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;   // :146-149
             const float iy = q[1] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
             const float iz = q[2] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
@@ -420,15 +436,21 @@ __global__ __launch_bounds__(1024) void k_merge_scan(uint32_t *__restrict__ tile
     if (threadIdx.x == 0) {
         const uint64_t v1 = cnt->v_old + cnt->n_new - total;
         cnt->v_new = v1;
-        if (v1 > cap)
+        if (v1 > cap) {
             cnt->overflow = 1;
-        else
+        } else {
             *d_size = v1;
+            cnt->tot_frames += 1;
+            cnt->tot_v_old += cnt->v_old;
+            cnt->tot_new += cnt->n_new;
+            cnt->tot_v_new += v1;
+            cnt->tot_keys += cnt->n_keys;
+        }
     }
 }
 
 struct UpdateParams {
-    Mat4  w2c;
+    const FrameMats *mats;   // device: w2c of this frame
     float vl, c0, trunc, ndcU, ndcV, maxU, maxV;
     int   W, H;
     const uint16_t *depth;
@@ -443,7 +465,7 @@ __device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, float
     unpack_key<K>(key, x, y, z);
     float k[4] = { (float)x * p.vl + p.c0, (float)y * p.vl + p.c0, (float)z * p.vl + p.c0,
                    1.0f };                                         // Volume.cpp:75-78
-    mat_vec(p.w2c, k, k);                                          // :290-293
+    mat_vec(p.mats->w2c, k, k);                                    // :290-293
     const float kw = k[3];
     float u = k[0] / kw, v = k[1] / kw, zz = k[2] / kw, ww = k[3] / kw;  // :305-308
     u = u * p.ndcU + p.ndcU;                                       // :310-313
@@ -559,52 +581,6 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
 }
 
 // ====================================================================== host
-static void host_mat_vec(const float m[16], const float v[4], float out[4])
-{
-    float r[4];
-    for (int i = 0; i < 4; ++i)
-        r[i] = (m[i] * v[0] + m[i + 4] * v[1]) + (m[i + 8] * v[2] + m[i + 12] * v[3]);
-    for (int i = 0; i < 4; ++i)
-        out[i] = r[i];
-}
-
-void host_mat_mul(const float m[16], const float n[16], float out[16])
-{
-    float r[16];
-    for (int c = 0; c < 4; ++c)
-        host_mat_vec(m, n + 4 * c, r + 4 * c);
-    for (int i = 0; i < 16; ++i)
-        out[i] = r[i];
-}
-
-void host_invert_transform(float m[16])          // util/math.h:615-625
-{
-    float t[4] = { -m[12], -m[13], -m[14], 1.0f };
-    m[12] = 0.0f; m[13] = 0.0f; m[14] = 0.0f; m[15] = 1.0f;
-    float s;
-    s = m[1]; m[1] = m[4]; m[4] = s;
-    s = m[2]; m[2] = m[8]; m[8] = s;
-    s = m[6]; m[6] = m[9]; m[9] = s;
-    float c3[4];
-    host_mat_vec(m, t, c3);
-    for (int i = 0; i < 4; ++i)
-        m[12 + i] = c3[i];
-}
-
-void host_eye_to_clip_rh(const kifi_intrinsics *c, float out[16])  // DepthSensorParams.cpp:47-63
-{
-    for (int i = 0; i < 16; ++i)
-        out[i] = 0.0f;
-    const float W = (float)c->width, H = (float)c->height;
-    out[0] = 2.0f * c->fx / W;
-    out[8] = W - 2.0f * c->cx;
-    out[5] = 2.0f * c->fy / H;
-    out[9] = H - 2.0f * c->cy;
-    out[10] = -(c->zfar + c->znear) / (c->zfar - c->znear);
-    out[14] = -2.0f * c->zfar * c->znear / (c->zfar - c->znear);
-    out[11] = -1.0f;
-}
-
 kifi_status grow_volume(kifi_volume *vol, size_t needed);
 
 // scratch layout for one integration (all offsets 256-byte aligned)
@@ -635,21 +611,13 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
 
 template <typename K, typename V>
 static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                                  const float w2e[16])
+                                  const float *w2e, const Mat4 *w2e_dev)
 {
     kifi_ctx *ctx = vol->ctx;
     const int W = cam->width, H = cam->height;
     const size_t P = (size_t)W * H;
     const int passes = KeyTraits<K>::sort_bits / 8;
     const size_t Tb = (8 * P + kRsTile - 1) / kRsTile;
-
-    // host-side binary32 constants in the reference's order
-    float clip[16], w2c[16], e2w[16];
-    host_eye_to_clip_rh(cam, clip);
-    host_mat_mul(clip, w2e, w2c);                                  // Integrator.cpp:41
-    for (int i = 0; i < 16; ++i)
-        e2w[i] = w2e[i];
-    host_invert_transform(e2w);                                    // :42
 
     // capacity: the volume can grow by at most 8P voxels per frame
     if (vol->size_event_pending && hipEventQuery(vol->size_event) == hipSuccess) {
@@ -685,6 +653,11 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     hipStream_t stream = ctx->stream;
 
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256);
+    Mat4 clip, wval;
+    eye_to_clip_rh(cam->width, cam->height, cam->fx, cam->fy, cam->cx, cam->cy, cam->znear, cam->zfar, clip.m);
+    for (int i = 0; i < 16; ++i)
+        wval.m[i] = w2e ? w2e[i] : 0.0f;
+    hipLaunchKernelGGL(k_frame_setup, dim3(1), dim3(64), 0, stream, clip, wval, w2e_dev, vol->d_mats);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;
@@ -694,8 +667,8 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     kp.ppY = (cam->cy - 0.5f) * kp.flInvY;
     kp.maxIndex = (float)(vol->desc.res - 1);
     kp.vc = vol->consts;
-    for (int i = 0; i < 16; ++i)
-        kp.e2w.m[i] = e2w[i];
+    kp.mats = vol->d_mats;
+    const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);
     hipLaunchKernelGGL(k_keygen<K>, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
                        stream, d_depth, kp, (K *)s.candA, cnt);
 
@@ -717,6 +690,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     hipLaunchKernelGGL(k_unique_scan, dim3(1), dim3(1024), 0, stream, s.uq_tiles, (int)Tb, cnt);
     hipLaunchKernelGGL(k_unique_scatter<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
                        &cnt->n_cand, s.uq_tiles, cnt, dst);
+    timer_end(ctx, KIFI_STAGE_SORT, t_sort);
     const K *Ekeys = dst;
 
     // merge + update into the other buffer pair
@@ -729,8 +703,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
                        (uint64_t)vol->cap);
     UpdateParams up;
-    for (int i = 0; i < 16; ++i)
-        up.w2c.m[i] = w2c[i];
+    up.mats = vol->d_mats;
     up.vl = vol->consts.vl;
     up.c0 = vol->consts.vlHalfPlusMin;
     up.trunc = vol->consts.trunc;
@@ -740,9 +713,11 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     up.maxV = (float)H;
     up.W = W; up.H = H;
     up.depth = d_depth;
+    const int t_mu = timer_begin(ctx, KIFI_STAGE_MERGE_UPDATE);
     hipLaunchKernelGGL((k_merge_update<K, V>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                        (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
                        cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up);
+    timer_end(ctx, KIFI_STAGE_MERGE_UPDATE, t_mu);
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
 
@@ -755,16 +730,16 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
 }
 
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                               const float w2e[16])
+                               const float *w2e, const Mat4 *w2e_dev)
 {
     if (vol->desc.key_bits == 32) {
         if (vol->desc.storage == KIFI_STORAGE_F32)
-            return integrate_impl<uint32_t, VoxF32>(vol, d_depth, cam, w2e);
-        return integrate_impl<uint32_t, uint32_t>(vol, d_depth, cam, w2e);
+            return integrate_impl<uint32_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev);
+        return integrate_impl<uint32_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev);
     }
     if (vol->desc.storage == KIFI_STORAGE_F32)
-        return integrate_impl<uint64_t, VoxF32>(vol, d_depth, cam, w2e);
-    return integrate_impl<uint64_t, uint32_t>(vol, d_depth, cam, w2e);
+        return integrate_impl<uint64_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev);
+    return integrate_impl<uint64_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev);
 }
 
 } // namespace kifi

@@ -10,6 +10,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "kifi.h"
 
@@ -82,16 +84,71 @@ struct Mat4 {
 };
 
 // (col0*x + col1*y) + (col2*z + col3*w), util/math.h:362-365, 381-397
-__device__ __forceinline__ void mat_vec(const Mat4 &M, const float v[4], float out[4])
+__host__ __device__ __forceinline__ void mat_vec(const float *m, const float v[4], float out[4])
 {
     float r[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        r[i] = (M.m[i] * v[0] + M.m[i + 4] * v[1]) + (M.m[i + 8] * v[2] + M.m[i + 12] * v[3]);
+        r[i] = (m[i] * v[0] + m[i + 4] * v[1]) + (m[i + 8] * v[2] + m[i + 12] * v[3]);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
         out[i] = r[i];
 }
+
+__host__ __device__ __forceinline__ void mat_vec(const Mat4 &M, const float v[4], float out[4])
+{
+    mat_vec(M.m, v, out);
+}
+
+// float4x4 * float4x4, util/math.h:367-377
+__host__ __device__ __forceinline__ void mat_mul(const float *m, const float *n, float *out)
+{
+    float r[16];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        mat_vec(m, n + 4 * c, r + 4 * c);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        out[i] = r[i];
+}
+
+// util/math.h:615-625
+__host__ __device__ __forceinline__ void invert_transform(float *m)
+{
+    float t[4] = { -m[12], -m[13], -m[14], 1.0f };
+    m[12] = 0.0f; m[13] = 0.0f; m[14] = 0.0f; m[15] = 1.0f;
+    float s;
+    s = m[1]; m[1] = m[4]; m[4] = s;
+    s = m[2]; m[2] = m[8]; m[8] = s;
+    s = m[6]; m[6] = m[9]; m[9] = s;
+    float c3[4];
+    mat_vec(m, t, c3);
+    m[12] = c3[0]; m[13] = c3[1]; m[14] = c3[2]; m[15] = c3[3];
+}
+
+// DepthSensorParams::EyeToClipRH, DepthSensorParams.cpp:47-63
+__host__ __device__ __forceinline__ void eye_to_clip_rh(int Wi, int Hi, float fx, float fy, float cx, float cy,
+                                                         float zn, float zf, float *out)
+{
+    for (int i = 0; i < 16; ++i)
+        out[i] = 0.0f;
+    const float W = (float)Wi, H = (float)Hi;
+    out[0] = 2.0f * fx / W;
+    out[8] = W - 2.0f * cx;
+    out[5] = 2.0f * fy / H;
+    out[9] = H - 2.0f * cy;
+    out[10] = -(zf + zn) / (zf - zn);
+    out[14] = -2.0f * zf * zn / (zf - zn);
+    out[11] = -1.0f;
+}
+
+// Per-frame matrices of an integration, computed on the device from worldToEye
+// in the reference's order (Integrator.cpp:41-42).
+struct FrameMats {
+    Mat4 w2e;   // worldToEye as given
+    Mat4 w2c;   // EyeToClipRH() * worldToEye
+    Mat4 e2w;   // invert_transform(worldToEye)
+};
 
 // Reset list for per-frame device counters (zeroed by a fill kernel, never by a
 // memset node, so that the integration sequence can be captured in a graph).
@@ -104,6 +161,8 @@ struct FrameCounters {
     uint64_t v_new;       // V1
     uint32_t n_tiles;     // merge tiles this frame
     uint32_t overflow;    // set when a device-side capacity check fails
+    // running totals over all integrations (never reset by a frame)
+    uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys;
 };
 
 } // namespace kifi
@@ -121,7 +180,17 @@ struct kifi_ctx {
     uint16_t    *d_depth = nullptr;   // one frame in mm
     size_t       depth_pixels = 0;
     int          num_cus = 0;
+    // stage timers (HIP event pairs recorded on `stream` while enabled)
+    bool         timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timers[KIFI_STAGE_COUNT];
+    std::vector<hipEvent_t> event_pool;
 };
+
+namespace kifi {
+// Records a start event for `stage` when timing is enabled; returns the slot.
+int  timer_begin(kifi_ctx *ctx, int stage);
+void timer_end(kifi_ctx *ctx, int stage, int slot);
+} // namespace kifi
 
 struct kifi_volume {
     kifi_ctx        *ctx = nullptr;
@@ -136,6 +205,7 @@ struct kifi_volume {
     kifi::FrameCounters *d_counters = nullptr;   // device
     kifi::FrameCounters *h_counters = nullptr;   // pinned host mirror
     uint64_t        *d_size = nullptr;           // device-resident volume size
+    kifi::FrameMats *d_mats = nullptr;           // matrices of the last integration
     hipEvent_t       size_event = nullptr;       // marks the last async size read-back
     bool             size_event_pending = false;
 };

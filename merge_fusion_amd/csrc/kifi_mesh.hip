@@ -426,6 +426,7 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     const K *keys = (const K *)vol->keys[vol->cur];
     const V *vals = (const V *)vol->vals[vol->cur];
     const unsigned grid = (unsigned)std::min<uint64_t>((m + kMT - 1) / kMT, (uint64_t)ctx->num_cus * 16);
+    const int t_mesh = timer_begin(ctx, KIFI_STAGE_MESH);
     if (tris)
         hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, vflags,
                            cv_fwd, (uint32_t *)nullptr, ci_rev);
@@ -436,6 +437,7 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     device_scan(st, voff, voff, m, tsum, totals);
     if (tris)
         device_scan(st, ci_rev, ci_rev, m, tsum, totals + 1);
+    timer_end(ctx, KIFI_STAGE_MESH, t_mesh);
     uint64_t h_tot[2] = { 0, 0 };
     KIFI_HIP(ctx, hipMemcpyAsync(h_tot, totals, tris ? 16 : 8, hipMemcpyDeviceToHost, st));
     KIFI_HIP(ctx, hipStreamSynchronize(st));
@@ -456,6 +458,7 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     uint32_t *d_idx = (uint32_t *)q; q += ib;
     uint64_t *d_raw = (uint64_t *)q;
     MeshParams mp{ vol->consts.vl, vol->consts.vlHalfPlusMin };
+    const int t_emit = timer_begin(ctx, KIFI_STAGE_MESH);
     if (tris) {
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, stall, (uint64_t)1, ~0ull);
         hipLaunchKernelGGL((k_mesh_emit<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, mp, vflags,
@@ -474,6 +477,7 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
                            cv_rev, (uint32_t *)nullptr, d_verts, (uint32_t *)nullptr, (uint64_t *)nullptr,
                            stall);
     }
+    timer_end(ctx, KIFI_STAGE_MESH, t_emit);
     KIFI_HIP(ctx, hipGetLastError());
     if (h_tot[0])
         KIFI_HIP(ctx, hipMemcpyAsync(verts, d_verts, h_tot[0] * 12, hipMemcpyDeviceToHost, st));

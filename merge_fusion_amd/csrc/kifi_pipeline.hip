@@ -1,0 +1,338 @@
+// kifi_pipeline.hip -- C ABI of model prediction, ICP, Horn and the frame loop.
+#include <cstddef>
+#include <cstring>
+#include <new>
+
+#include "kifi_icp.hpp"
+#include "kifi_internal.hpp"
+
+namespace kifi {
+kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+                               const float *w2e, const Mat4 *w2e_dev);
+void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st);
+} // namespace kifi
+
+using namespace kifi;
+
+struct kifi_pipeline {
+    kifi_ctx        *ctx = nullptr;
+    kifi_volume     *vol = nullptr;
+    kifi_model      *model = nullptr;
+    kifi_intrinsics  cam{};
+    kifi_icp_params  params{};
+    IcpWork          work;
+    Mat4            *d_w2e = nullptr;   // world->eye of the current frame (binary32)
+    uint16_t        *d_depth = nullptr; // staging for host frames
+    double          *d_guess = nullptr;
+    bool             have_guess = false;
+    double           guess[16];
+    long long        frame = 0;
+};
+
+static const double *prev_pose_dev(const IcpWork *w)
+{
+    return (const double *)((const char *)w->d_state + offsetof(IcpState, prev));
+}
+
+static kifi_status bad_cam(kifi_ctx *ctx, const kifi_intrinsics *cam)
+{
+    if (!cam || cam->width <= 0 || cam->height <= 0 || cam->width % 4 != 0 || cam->height % 4 != 0 ||
+        !(cam->fx > 0.0f) || !(cam->fy > 0.0f))
+        return set_error(ctx, KIFI_ERR_ARG, "invalid intrinsics (width and height must be multiples of 4)");
+    return KIFI_OK;
+}
+
+extern "C" {
+
+kifi_status kifi_icp_params_default(kifi_icp_params *p)
+{
+    if (!p)
+        return KIFI_ERR_ARG;
+    p->mode = KIFI_ICP_POINT_TO_PLANE;
+    p->levels = 3;
+    p->iters[0] = 10;   // 512 x 424
+    p->iters[1] = 5;    // 256 x 212
+    p->iters[2] = 4;    // 128 x 106
+    p->dist_thresh = 0.1f;
+    p->eps_rot = 2e-4f;
+    p->eps_trans = 2e-4f;
+    p->min_inliers = 64;
+    p->fallback_rms = 0.0f;
+    p->fallback_iters = 0;
+    return KIFI_OK;
+}
+
+kifi_status kifi_model_create(kifi_ctx *ctx, const kifi_intrinsics *cam, kifi_model **out)
+{
+    if (!ctx || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    kifi_model *m = new (std::nothrow) kifi_model;
+    if (!m)
+        return KIFI_ERR_NOMEM;
+    m->ctx = ctx;
+    m->cam = *cam;
+    m->mp.W = cam->width;
+    m->mp.H = cam->height;
+    m->mp.fx = cam->fx;
+    m->mp.fy = cam->fy;
+    m->mp.cx = cam->cx;
+    m->mp.cy = cam->cy;
+    m->mp.znear = 0.1f;
+    s = model_alloc(m);
+    if (s != KIFI_OK) {
+        model_free(m);
+        delete m;
+        return s;
+    }
+    *out = m;
+    return KIFI_OK;
+}
+
+void kifi_model_destroy(kifi_model *m)
+{
+    if (!m)
+        return;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    model_free(m);
+    delete m;
+}
+
+kifi_status kifi_model_predict(kifi_model *m, kifi_volume *vol, const float worldToEye[16])
+{
+    if (!m || !vol || !worldToEye || vol->ctx != m->ctx)
+        return KIFI_ERR_ARG;
+    m->mp.dmax = vol->consts.vl;   // near-surface band: |D| < one voxel
+    return model_predict(m, vol, worldToEye, nullptr);
+}
+
+kifi_status kifi_model_download(kifi_model *m, float *points, float *normals)
+{
+    if (!m)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = m->ctx;
+    const size_t P = (size_t)m->cam.width * m->cam.height;
+    if (points)
+        KIFI_HIP(ctx, hipMemcpyAsync(points, m->d_pts, P * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (normals)
+        KIFI_HIP(ctx, hipMemcpyAsync(normals, m->d_nrm, P * 16, hipMemcpyDeviceToHost, ctx->stream));
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KIFI_OK;
+}
+
+static void fill_stats(const IcpState &s, kifi_icp_stats *st)
+{
+    st->iterations = s.total_iters;
+    for (int l = 0; l < 3; ++l)
+        st->iters[l] = s.iters[l];
+    st->rms = s.rms;
+    st->inliers = s.inliers;
+    st->failed = s.failed;
+    st->fallback = s.fallback;
+    st->sum_iters = s.sum_iters;
+    st->frames = s.frames_tracked;
+    st->fallbacks = s.fallbacks;
+}
+
+kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const double guess[16], const kifi_icp_params *params, double out[16],
+                           kifi_icp_stats *stats)
+{
+    if (!m || !depth_mm || !guess || !params || !out)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = m->ctx;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    if (cam->width != m->cam.width || cam->height != m->cam.height)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_icp_align: frame size differs from the model's");
+    IcpWork w;
+    s = icp_alloc(&w, ctx, cam->width, cam->height);
+    if (s != KIFI_OK) {
+        icp_free(&w);
+        return s;
+    }
+    const size_t P = (size_t)cam->width * cam->height;
+    uint16_t *d_depth = nullptr;
+    hipError_t e = hipMalloc(&d_depth, P * 2);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        icp_free(&w);
+        if (d_depth) (void)hipFree(d_depth);
+        return hip_error(ctx, e, "kifi_icp_align upload");
+    }
+    icp_set_pose(&w, guess, nullptr, ctx->stream);   // previous pose for a fallback = the guess
+    s = icp_run(m, &w, d_depth, cam, guess, nullptr, params);
+    IcpState hs;
+    if (s == KIFI_OK) {
+        e = hipMemcpyAsync(&hs, w.d_state, sizeof hs, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess)
+            s = hip_error(ctx, e, "kifi_icp_align");
+    }
+    if (s == KIFI_OK) {
+        for (int i = 0; i < 16; ++i)
+            out[i] = hs.pose[i];
+        if (stats)
+            fill_stats(hs, stats);
+    }
+    (void)hipFree(d_depth);
+    icp_free(&w);
+    return s;
+}
+
+kifi_status kifi_horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16])
+{
+    if (!ctx || !src || !dst || !out || n < 3 || n >= 0xffffffffull)
+        return KIFI_ERR_ARG;
+    return horn_align(ctx, src, dst, n, out);
+}
+
+kifi_status kifi_pipeline_create(kifi_ctx *ctx, const kifi_volume_desc *vd, const kifi_intrinsics *cam,
+                                 const kifi_icp_params *params, kifi_pipeline **out)
+{
+    if (!ctx || !vd || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    kifi_pipeline *p = new (std::nothrow) kifi_pipeline;
+    if (!p)
+        return KIFI_ERR_NOMEM;
+    p->ctx = ctx;
+    p->cam = *cam;
+    if (params)
+        p->params = *params;
+    else
+        kifi_icp_params_default(&p->params);
+    s = kifi_volume_create(ctx, vd, &p->vol);
+    if (s == KIFI_OK)
+        s = kifi_model_create(ctx, cam, &p->model);
+    if (s == KIFI_OK)
+        s = icp_alloc(&p->work, ctx, cam->width, cam->height);
+    if (s == KIFI_OK) {
+        hipError_t e = hipMalloc(&p->d_w2e, sizeof(Mat4));
+        if (e == hipSuccess)
+            e = hipMalloc(&p->d_depth, (size_t)cam->width * cam->height * 2);
+        if (e == hipSuccess)
+            e = hipMalloc(&p->d_guess, 16 * sizeof(double));
+        if (e != hipSuccess)
+            s = hip_error(ctx, e, "kifi_pipeline_create");
+    }
+    if (s != KIFI_OK) {
+        kifi_pipeline_destroy(p);
+        return s;
+    }
+    // Pipeline.cpp:7-21: the first frame is integrated at the identity pose
+    const double I[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+    icp_set_pose(&p->work, I, nullptr, ctx->stream);
+    *out = p;
+    return KIFI_OK;
+}
+
+void kifi_pipeline_destroy(kifi_pipeline *p)
+{
+    if (!p)
+        return;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->vol) kifi_volume_destroy(p->vol);
+    if (p->model) kifi_model_destroy(p->model);
+    icp_free(&p->work);
+    if (p->d_w2e) (void)hipFree(p->d_w2e);
+    if (p->d_depth) (void)hipFree(p->d_depth);
+    if (p->d_guess) (void)hipFree(p->d_guess);
+    delete p;
+}
+
+kifi_volume *kifi_pipeline_volume(kifi_pipeline *p) { return p ? p->vol : nullptr; }
+kifi_model *kifi_pipeline_model(kifi_pipeline *p) { return p ? p->model : nullptr; }
+
+kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double e2w[16])
+{
+    if (!p || !e2w)
+        return KIFI_ERR_ARG;
+    std::memcpy(p->guess, e2w, sizeof p->guess);
+    p->have_guess = true;
+    return KIFI_OK;
+}
+
+kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double e2w[16])
+{
+    if (!p || !e2w)
+        return KIFI_ERR_ARG;
+    icp_set_pose(&p->work, e2w, nullptr, p->ctx->stream);
+    KIFI_HIP(p->ctx, hipGetLastError());
+    return KIFI_OK;
+}
+
+kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int on_device, const float w2e[16])
+{
+    if (!p || !depth)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = p->ctx;
+    hipStream_t st = ctx->stream;
+    const size_t P = (size_t)p->cam.width * p->cam.height;
+    const uint16_t *d_depth = depth;
+    if (!on_device) {
+        KIFI_HIP(ctx, hipMemcpyAsync(p->d_depth, depth, P * 2, hipMemcpyHostToDevice, st));
+        d_depth = p->d_depth;
+    }
+    kifi_status s = KIFI_OK;
+    if (w2e) {
+        // Pipeline.cpp:113-116: integrate at the given pose
+        icp_set_pose(&p->work, nullptr, w2e, st);
+    } else if (p->frame > 0) {
+        // Pipeline.cpp:118-120: track against the model of the previous frame
+        kifi_icp_params ip = p->params;
+        if (!p->have_guess) {
+            ip.fallback_rms = 0.0f;   // the guess is the previous pose: nothing to fall back to
+            ip.fallback_iters = 0;
+        }
+        s = icp_run(p->model, &p->work, d_depth, &p->cam, p->have_guess ? p->guess : nullptr,
+                    p->have_guess ? nullptr : prev_pose_dev(&p->work), &ip);
+        p->have_guess = false;
+        if (s != KIFI_OK)
+            return s;
+    }
+    // Pipeline.cpp:122-127: worldToEye = invert(eyeToWorld), integrate
+    icp_w2e(&p->work, p->d_w2e, st);
+    s = integrate_dispatch(p->vol, d_depth, &p->cam, nullptr, p->d_w2e);
+    if (s != KIFI_OK)
+        return s;
+    // Pipeline.cpp:129-133: the model for the next frame, seen from this pose
+    p->model->mp.dmax = p->vol->consts.vl;
+    s = model_predict(p->model, p->vol, nullptr, p->d_w2e);
+    ++p->frame;
+    return s;
+}
+
+kifi_status kifi_pipeline_pose(kifi_pipeline *p, double e2w[16])
+{
+    if (!p || !e2w)
+        return KIFI_ERR_ARG;
+    IcpState hs;
+    KIFI_HIP(p->ctx, hipMemcpyAsync(&hs, p->work.d_state, sizeof hs, hipMemcpyDeviceToHost, p->ctx->stream));
+    KIFI_HIP(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    for (int i = 0; i < 16; ++i)
+        e2w[i] = hs.pose[i];
+    return KIFI_OK;
+}
+
+kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats)
+{
+    if (!p || !stats)
+        return KIFI_ERR_ARG;
+    IcpState hs;
+    KIFI_HIP(p->ctx, hipMemcpyAsync(&hs, p->work.d_state, sizeof hs, hipMemcpyDeviceToHost, p->ctx->stream));
+    KIFI_HIP(p->ctx, hipStreamSynchronize(p->ctx->stream));
+    fill_stats(hs, stats);
+    return KIFI_OK;
+}
+
+} // extern "C"

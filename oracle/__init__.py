@@ -267,6 +267,106 @@ def kahan_sum_f32(x):
     return lib().orc_kahan_sum_f32(f32p(x), x.shape[0])
 
 
+def _icp_sigs(L):
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    L.orc_model_splat.argtypes = [ctypes.POINTER(OVolume), F32P, ctypes.POINTER(Camera), ctypes.c_float,
+                                  ctypes.c_float, U64P]
+    L.orc_model_splat.restype = None
+    L.orc_model_resolve.argtypes = [ctypes.POINTER(OVolume), F32P, F32P, U64P, i32, i32, ctypes.c_float,
+                                    ctypes.c_float, F32P]
+    L.orc_model_resolve.restype = None
+    L.orc_model_normals4.argtypes = [F32P, i32, i32, F32P, ctypes.c_float, F32P]
+    L.orc_model_normals4.restype = None
+    L.orc_depth_pyramid_level.argtypes = [F32P, i32, i32, F32P]
+    L.orc_depth_pyramid_level.restype = None
+    d = ctypes.c_double
+    L.orc_icp_level_f64.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P, F64P,
+                                    i32, d, d, d, F64P]
+    L.orc_icp_level_f64.restype = i32
+
+
+def model_predict(vol, world_to_eye, cam, dmax=None, znear=0.1, disc=0.15):
+    """Model point/normal maps (H, W, 4) of an oracle Volume seen from world_to_eye."""
+    L = lib()
+    if not hasattr(L, "_icp_ready"):
+        _icp_sigs(L)
+        L._icp_ready = True
+    w2e = np.ascontiguousarray(world_to_eye, np.float32).reshape(16)
+    c = camera(cam)
+    W, H = cam.width, cam.height
+    dm = vol.desc.vl if dmax is None else dmax
+    zb = np.zeros(W * H, np.uint64)
+    L.orc_model_splat(vol.h, f32p(w2e), ctypes.byref(c), dm, znear, u64p(zb))
+    e2w = invert_transform(w2e)
+    campos = np.ascontiguousarray(e2w[12:15], np.float32)
+    pts = np.zeros((H, W, 4), np.float32)
+    L.orc_model_resolve(vol.h, f32p(w2e), f32p(campos), u64p(zb), W, H, dm, znear, f32p(pts))
+    nrm = np.zeros_like(pts)
+    L.orc_model_normals4(f32p(pts), W, H, f32p(campos), disc, f32p(nrm))
+    return pts, nrm
+
+
+def depth_pyramid(depth_mm, levels=3):
+    """Level maps in metres, binary32, as the GPU builds them."""
+    L = lib()
+    if not hasattr(L, "_icp_ready"):
+        _icp_sigs(L)
+        L._icp_ready = True
+    d0 = (np.ascontiguousarray(depth_mm, np.uint16).astype(np.float32) * np.float32(0.001)).astype(np.float32)
+    out = [d0]
+    for _ in range(1, levels):
+        src = out[-1]
+        H, W = src.shape
+        dst = np.zeros((H // 2, W // 2), np.float32)
+        L.orc_depth_pyramid_level(f32p(np.ascontiguousarray(src)), W, H, f32p(dst))
+        out.append(dst)
+    return out
+
+
+def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dist_thresh=0.1,
+            eps_rot=2e-4, eps_trans=2e-4):
+    """FP64 point-to-plane ICP over the pyramid; returns (pose, total iterations, last rms)."""
+    L = lib()
+    if not hasattr(L, "_icp_ready"):
+        _icp_sigs(L)
+        L._icp_ready = True
+    pyr = depth_pyramid(depth_mm, len(iters))
+    pose = np.ascontiguousarray(guess_e2w, np.float64).reshape(16).copy()
+    W2E = np.ascontiguousarray(model_w2e, np.float64).reshape(16)
+    c = camera(cam)
+    P = np.ascontiguousarray(pts, np.float32)
+    N = np.ascontiguousarray(nrm, np.float32)
+    total, st = 0, np.zeros(2)
+    for l in range(len(iters) - 1, -1, -1):
+        d = pyr[l]
+        s = 1.0 / (1 << l)
+        n = L.orc_icp_level_f64(f32p(d), d.shape[1], d.shape[0], cam.fx * s, cam.fy * s, cam.cx * s, cam.cy * s,
+                                f32p(P), f32p(N), ctypes.byref(c), W2E.ctypes.data_as(F64P),
+                                pose.ctypes.data_as(F64P), iters[l], dist_thresh, eps_rot, eps_trans,
+                                st.ctypes.data_as(F64P))
+        total += n
+    return pose, total, st[0]
+
+
+def icp_align_ref(depth_mm, cam, model_pts4, model_e2w, guess_e2w, iters=4):
+    """ICP::Align restated (ICP.cpp:9-265): window search + Horn, binary32."""
+    L = lib()
+    if not hasattr(L, "_ref_ready"):
+        L.orc_icp_align_ref.argtypes = [F32P, ctypes.POINTER(Camera), F32P, F32P, ctypes.c_int, ctypes.c_int,
+                                        F32P, ctypes.c_int, F32P]
+        L.orc_icp_align_ref.restype = ctypes.c_size_t
+        L._ref_ready = True
+    d = (np.ascontiguousarray(depth_mm, np.uint16).astype(np.float32) * np.float32(0.001)).astype(np.float32)
+    m3 = np.ascontiguousarray(np.asarray(model_pts4, np.float32)[..., :3])
+    H, W = m3.shape[:2]
+    me = np.ascontiguousarray(model_e2w, np.float32).reshape(16)
+    g = np.ascontiguousarray(guess_e2w, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    n = L.orc_icp_align_ref(f32p(d), ctypes.byref(camera(cam)), f32p(g), f32p(m3), W, H, f32p(me), iters,
+                            f32p(out))
+    return out, n
+
+
 def partial_sum_exclusive(x):
     x = np.ascontiguousarray(x, np.uint32)
     out = np.zeros_like(x)

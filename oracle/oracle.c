@@ -867,6 +867,107 @@ static void kahan1_add(kahan1 *k, float x)
     k->s = t;
 }
 
+static void horn_from_sums(const kahan4 *ss, const kahan4 *ds, const float *src, const float *dst, size_t n,
+                           float out[16]);
+
+/* ICP::AlignStep, ICP.cpp:35-265, in full: every 4th column of the raw frame
+ * (depth in metres, W x H) is back-projected with the guess, projected into the
+ * model buffer (mW x mH float3, 0 = invalid) with EyeToClipRH * dstWorldToEye,
+ * matched to the nearest model point of a 21 x 21 window within 10 cm (first
+ * minimum in scan order), and Horn's closed form aligns the matches.  The
+ * window clamps to the model buffer (the reference hard-codes 640 x 480,
+ * ICP.cpp:108-109).  Returns the number of associations; out = T*R. */
+size_t orc_align_step(const float *depth, const orc_camera *cam, const float guess_e2w[16],
+                      const float *model, int mW, int mH, const float model_e2w[16], float out[16],
+                      float *scratch_src, float *scratch_dst)
+{
+    const int W = cam->width, H = cam->height;
+    float dstW2E[16];
+    memcpy(dstW2E, model_e2w, sizeof dstW2E);
+    orc_invert_transform(dstW2E);                                  /* :48 */
+    const float flInvX = 1.0f / cam->fx, flInvY = 1.0f / cam->fy;  /* :50-57 */
+    const float ppX = (0.5f - cam->cx) * flInvX, ppY = (cam->cy - 0.5f) * flInvY;
+    const float halfW = (float)mW * 0.5f, halfH = (float)mH * 0.5f; /* :59-60 */
+    float clip[16], proj[16];
+    orc_eye_to_clip_rh(cam, clip);
+    orc_mat_mul(clip, dstW2E, proj);                               /* :81 */
+    kahan4 ss = { { 0 }, { 0 } }, ds = { { 0 }, { 0 } };
+    size_t n = 0;
+    for (int y = 0; y < H; ++y)                                    /* :66-67 */
+        for (int x = 0; x < W; x += 4) {
+            float d = depth[(size_t)y * W + x];
+            if (0.0f == d)
+                continue;
+            float pt[4] = { (float)x, -(float)y, -1.0f, 0.0f };    /* :74-77 */
+            pt[0] = (pt[0] * flInvX + ppX) * d;
+            pt[1] = (pt[1] * flInvY + ppY) * d;
+            pt[2] = (pt[2] * 1.0f + 0.0f) * d;
+            pt[3] = 1.0f;
+            orc_mat_vec(guess_e2w, pt, pt);                        /* :79 */
+            float uv[4];
+            orc_mat_vec(proj, pt, uv);
+            float w = uv[3];
+            uv[0] = uv[0] / w; uv[1] = uv[1] / w;                  /* homogenize */
+            int u = to_int(uv[0] * halfW + halfW), v0 = to_int(uv[1] * halfH + halfH);
+            if (u == (-2147483647 - 1) || v0 == (-2147483647 - 1))
+                continue;
+            u -= 10;                                               /* :100-104 */
+            long v = (long)mH - 1 - v0 - 10;
+            if (u < 0) u = 0;
+            if (v < 0) v = 0;
+            float mind = 0.01f, p2[4] = { 0, 0, 0, 0 };
+            long yend = v + 21 < mH ? v + 21 : mH;
+            int xend = u + 21 < mW ? u + 21 : mW;
+            for (long y1 = v; y1 < yend; ++y1)                     /* :108-125 */
+                for (int x1 = u; x1 < xend; ++x1) {
+                    const float *q = model + 3 * ((size_t)y1 * mW + x1);
+                    if (q[0] == 0.0f && q[1] == 0.0f && q[2] == 0.0f)
+                        continue;
+                    float dx = q[0] - pt[0], dy = q[1] - pt[1], dz = q[2] - pt[2], dw = 1.0f - pt[3];
+                    float l2 = (dx * dx + dy * dy) + (dz * dz + dw * dw);
+                    if (l2 < mind) {
+                        mind = l2;
+                        p2[0] = q[0]; p2[1] = q[1]; p2[2] = q[2]; p2[3] = 1.0f;
+                    }
+                }
+            if (mind < 0.01f) {                                    /* :126-131 */
+                memcpy(scratch_src + 3 * n, pt, 3 * sizeof(float));
+                memcpy(scratch_dst + 3 * n, p2, 3 * sizeof(float));
+                float a[4] = { pt[0], pt[1], pt[2], pt[3] }, b[4] = { p2[0], p2[1], p2[2], 1.0f };
+                kahan4_add(&ss, a);
+                kahan4_add(&ds, b);
+                ++n;
+            }
+        }
+    if (n == 0) {
+        float I[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+        memcpy(out, I, sizeof I);
+        return 0;
+    }
+    horn_from_sums(&ss, &ds, scratch_src, scratch_dst, n, out);
+    return n;
+}
+
+/* ICP::Align, ICP.cpp:9-31: four AlignStep iterations, result = step * result. */
+size_t orc_icp_align_ref(const float *depth, const orc_camera *cam, const float guess_e2w[16],
+                         const float *model, int mW, int mH, const float model_e2w[16], int iters,
+                         float out_e2w[16])
+{
+    size_t cap = (size_t)cam->height * (cam->width / 4 + 1);
+    float *s = (float *)malloc(3 * cap * sizeof(float)), *d = (float *)malloc(3 * cap * sizeof(float));
+    float res[16], step[16];
+    memcpy(res, guess_e2w, sizeof res);
+    size_t n = 0;
+    for (int i = 0; i < iters; ++i) {
+        n = orc_align_step(depth, cam, res, model, mW, mH, model_e2w, step, s, d);
+        orc_mat_mul(step, res, res);
+    }
+    memcpy(out_e2w, res, sizeof res);
+    free(s);
+    free(d);
+    return n;
+}
+
 /* ICP::AlignStep from the correspondences on, ICP.cpp:166-264. */
 void orc_horn_step(const float *src, const float *dst, size_t n, float out[16])
 {
@@ -877,6 +978,13 @@ void orc_horn_step(const float *src, const float *dst, size_t n, float out[16])
         kahan4_add(&ss, a);
         kahan4_add(&ds, b);
     }
+    horn_from_sums(&ss, &ds, src, dst, n, out);
+}
+
+static void horn_from_sums(const kahan4 *ssp, const kahan4 *dsp, const float *src, const float *dst, size_t n,
+                           float out[16])
+{
+    kahan4 ss = *ssp, ds = *dsp;
     float nf = (float)n, sm[4], dm[4];
     for (int i = 0; i < 4; ++i) {                            /* :166-167 */
         sm[i] = ss.s[i] / nf;

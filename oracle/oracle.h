@@ -130,6 +130,15 @@ void orc_bin(const float *points, size_t n, const float worldToClip[16], int W, 
  * T*R).  The eigen solve uses the oracle's own FP64 cyclic Jacobi (the reference's
  * routine is not restated). out = T*R in column storage. */
 void orc_horn_step(const float *src, const float *dst, size_t n, float out[16]);
+/* ICP::AlignStep in full (ICP.cpp:35-265): window-search correspondences + Horn.
+ * model is mW x mH float3 (0 = invalid); scratch buffers hold H*(W/4+1) float3. */
+size_t orc_align_step(const float *depth_m, const orc_camera *cam, const float guess_e2w[16],
+                      const float *model, int mW, int mH, const float model_e2w[16], float out[16],
+                      float *scratch_src, float *scratch_dst);
+/* ICP::Align (ICP.cpp:9-31): `iters` AlignSteps (the reference runs 4). */
+size_t orc_icp_align_ref(const float *depth_m, const orc_camera *cam, const float guess_e2w[16],
+                         const float *model, int mW, int mH, const float model_e2w[16], int iters,
+                         float out_e2w[16]);
 /* Symmetric 4x4 eigen decomposition, FP64 cyclic Jacobi. vecs columns, column storage. */
 void orc_jacobi4(const double a[16], double evals[4], double evecs[16]);
 /* Kahan sum of binary32 values in order (numeric.h:52-74) */
@@ -137,18 +146,19 @@ float orc_kahan_sum_f32(const float *x, size_t n);
 /* exclusive scan (numeric.h:78-108) */
 void  orc_partial_sum_exclusive_u32(const uint32_t *in, size_t n, uint32_t *out);
 
-/* Point-to-plane projective ICP (no reference counterpart), FP64 oracle.
- * Model maps are W x H float3 (world coords, 0 = invalid) rendered at modelEyeToWorld.
- * pose_io: eye->world (column storage, double) initial guess in, result out.
- * iters[levels] coarse-to-fine schedule handled by the caller per level. Returns the
- * number of iterations run; stats: [0]=last rms, [1]=last inlier count. */
-int orc_icp_p2p_level(const float *depth_m_level, int Wl, int Hl, float fxl, float fyl,
-                      float cxl, float cyl, const float *model_pts, const float *model_nrm,
-                      int W, int H, const orc_camera *cam, const double modelWorldToEye[16],
-                      double pose_io[16], int max_iters, double dist_thresh, double eps,
-                      double stats[2]);
+/* ---- model prediction + point-to-plane ICP (oracle_icp.c; no reference counterpart) ---- */
+/* binary32, bit-exact twins of the HIP model kernels */
+void orc_model_splat(const orc_volume *v, const float w2e[16], const orc_camera *cam, float dmax,
+                     float znear, uint64_t *zbuf);
+void orc_model_resolve(const orc_volume *v, const float w2e[16], const float camPos[3],
+                       const uint64_t *zbuf, int W, int H, float dmax, float znear, float *pts);
+void orc_model_normals4(const float *pts, int W, int H, const float camPos[3], float disc, float *nrm);
 void orc_depth_pyramid_level(const float *src, int W, int H, float *dst);
-void orc_model_normals(const float *pts, int W, int H, const float camPosWorld[3], float *nrm);
+/* FP64 point-to-plane ICP on one pyramid level (the FP64 pose oracle) */
+int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                      const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                      double pose[16], int max_iters, double dist_thresh, double eps_rot,
+                      double eps_trans, double stats[2]);
 
 #ifdef __cplusplus
 }

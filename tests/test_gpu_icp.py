@@ -1,0 +1,131 @@
+"""GPU parity for model prediction, ICP and Horn.
+
+- model point/normal maps: bit-exact against the binary32 oracle restatement;
+- point-to-plane ICP (binary32 compensated sums, FP64 solve): pose within
+  1 mm / 0.1 deg of the FP64 oracle on identical inputs (stated tolerance);
+- Horn's closed form (ICP.cpp:166-264) from exact correspondences: recovers a
+  known rigid motion and agrees with the oracle's binary32 restatement.
+"""
+import numpy as np
+import pytest
+
+import merge_fusion_amd as mf
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+CAM = mf.Intrinsics(512, 424, 365.0, 365.0)
+POSE_TOL_M, POSE_TOL_DEG = 1e-3, 0.1
+
+
+def _room(ctx, frames, seed=2, storage=mf.STORAGE_F32):
+    sc = mf.make_scene(mf.SCENE_ROOM, seed)
+    gv = mf.Volume(ctx, 1024, 4.096, 0.02, 32, storage)
+    ov = orc.Volume(1024, 4.096, 0.02, 10, storage)
+    for f in frames:
+        pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, seed, f)
+        d = mf.render(sc, CAM, pose, noise_seed=10 + f)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        gv.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    return sc, gv, ov
+
+
+@pytest.mark.parametrize("storage", [mf.STORAGE_F32, mf.STORAGE_F16U8])
+def test_model_maps_bit_exact(ctx, storage):
+    sc, gv, ov = _room(ctx, [0, 1, 2], storage=storage)
+    w2e = mf.invert_rigid(mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)).astype(np.float32)
+    m = mf.Model(ctx, CAM)
+    m.predict(gv, w2e)
+    gp, gn = m.download()
+    op, on = orc.model_predict(ov, w2e, CAM)
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+    assert np.array_equal(gn.view(np.uint32), on.view(np.uint32))
+    assert gp[..., 3].mean() > 0.9 and gn[..., 3].mean() > 0.8
+
+
+@pytest.mark.parametrize("target", [3, 6, 9])
+def test_icp_matches_fp64_oracle(ctx, target):
+    sc, gv, ov = _room(ctx, [0, 1, 2])
+    p2 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)
+    w2e = mf.invert_rigid(p2).astype(np.float32)
+    m = mf.Model(ctx, CAM)
+    m.predict(gv, w2e)
+    pts, nrm = m.download()
+    pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, target)
+    d = mf.render(sc, CAM, pt, noise_seed=99 + target)
+    gpose, st = m.icp_align(d, p2)
+    opose, oit, orms = orc.icp_f64(d, CAM, pts, nrm, w2e.astype(np.float64), p2)
+    dt, da = mf.pose_error(gpose, opose)
+    assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da, st, oit)
+    gt_t, gt_a = mf.pose_error(gpose, pt)
+    assert gt_t < 0.01 and gt_a < 0.2, (gt_t, gt_a)
+    assert 1 <= st["iterations"] <= 19 and st["inliers"] > 10000 and not st["failed"]
+
+
+def test_icp_horn_mode_matches_reference_align(ctx):
+    """KIFI_ICP_HORN = the reference's ICP::Align (4 window-search Horn steps, ICP.cpp:9-265),
+    against its binary32 restatement; tolerance 1 mm / 0.1 deg (different reduction order)."""
+    sc, gv, ov = _room(ctx, [0, 1, 2])
+    p2 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)
+    w2e = mf.invert_rigid(p2).astype(np.float32)
+    m = mf.Model(ctx, CAM)
+    m.predict(gv, w2e)
+    pts, _ = m.download()
+    pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 3)
+    d = mf.render(sc, CAM, pt, noise_seed=7)
+    prm = mf.IcpParams.default()
+    prm.mode = mf.ICP_HORN
+    prm.levels = 1
+    prm.iters[0] = 4
+    prm.eps_rot = prm.eps_trans = 0.0
+    gpose, st = m.icp_align(d, p2, prm)
+    assert st["iterations"] == 4 and st["inliers"] > 10000
+    model_e2w = orc.invert_transform(w2e)
+    opose, n = orc.icp_align_ref(d, CAM, pts, model_e2w, p2.astype(np.float32))
+    dt, da = mf.pose_error(gpose, opose.astype(np.float64))
+    assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da)
+
+
+def _rigid(rng, ang_deg, trans):
+    ax = rng.standard_normal(3)
+    ax /= np.linalg.norm(ax)
+    a = np.radians(ang_deg)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = trans
+    return T
+
+
+@pytest.mark.parametrize("n", [100, 5000, 200000])
+def test_horn_closed_form(ctx, n):
+    """Known rigid motion of random points (the test ICP.cpp:134-164 sketches)."""
+    rng = np.random.default_rng(n)
+    src = rng.random((n, 3)).astype(np.float32) * 2.0 - 1.0
+    T = _rigid(rng, 7.0, [0.1, -0.05, 0.2])
+    dst = (src.astype(np.float64) @ T[:3, :3].T + T[:3, 3]).astype(np.float32)
+    g = mf.horn_align(ctx, src, dst).reshape(4, 4).T
+    assert np.abs(g - T).max() < 2e-5
+    o = orc.horn_step(src, dst).reshape(4, 4).T
+    assert np.abs(g - o).max() < 2e-5
+
+
+def test_pipeline_tracks_trajectory(ctx):
+    """Pipeline (Pipeline.cpp:26-56 tracking branch): ICP against the previous model."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+    p0 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 0)
+    for f in range(8):
+        pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f)
+        d = mf.render(sc, CAM, pt, noise_seed=300 + f)
+        if f == 0:
+            pipe.integrate(d, mf.invert_rigid(p0).astype(np.float32))
+        else:
+            pipe.integrate(d)
+    est = pipe.pose()
+    dt, da = mf.pose_error(est, mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 7))
+    st = pipe.stats()
+    assert dt < 0.02 and da < 0.5, (dt, da, st)
+    assert st["frames"] == 7 and 7 <= st["sum_iters"] <= 7 * 19
