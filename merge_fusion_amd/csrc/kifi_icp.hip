@@ -270,13 +270,43 @@ template <> struct IcpNV<KIFI_ICP_POINT_TO_PLANE> { static constexpr int nv = 29
 template <> struct IcpNV<KIFI_ICP_HORN> { static constexpr int nv = 17; };
 
 template <int MODE>
+__device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
+                           const ModelPose *mpose, CSum *lds);
+
+// Block partial -> device memory; the workgroup that takes the last ticket
+// reduces all partials and runs the FP64 solve (no spin-wait anywhere).
+template <int MODE, int NV>
+__device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CSum *lds, CSum *partials, IcpState *st, int level,
+                                            const IcpSolveParams &sp, const ModelPose *mpose)
+{
+    __shared__ bool last;
+    block_csum<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            partials[(size_t)blockIdx.x * NV + j] = acc[j];
+        __threadfence();
+        const unsigned t = atomicAdd(&st->ticket, 1u);
+        last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    icp_finish<MODE>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
+    if (threadIdx.x == 0)
+        st->ticket = 0;
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, const ModelPose *__restrict__ mpose,
                                                    const float4 *__restrict__ mpts,
                                                    const float4 *__restrict__ mnrm,
-                                                   const IcpState *__restrict__ st, float dist_thresh,
-                                                   CSum *__restrict__ partials)
+                                                   IcpState *__restrict__ st, float dist_thresh,
+                                                   CSum *__restrict__ partials, int level, IcpSolveParams sp)
 {
     constexpr int NV = IcpNV<MODE>::nv;
+    constexpr int MODE_OF_KERNEL = MODE;
     __shared__ CSum lds[(kIT / kWave) * NV];
     if (st->level_done)
         return;
@@ -341,11 +371,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
             }
         }
     }
-    block_csum<NV>(acc, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-            partials[(size_t)blockIdx.x * NV + j] = acc[j];
+    icp_publish<MODE_OF_KERNEL>(acc, lds, partials, st, level, sp, mpose);
 }
 
 // ICP::AlignStep association (ICP.cpp:66-131): every 4th column, back-projected
@@ -356,10 +382,12 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
 __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelParams mp,
                                                           const ModelPose *__restrict__ mpose,
                                                           const float4 *__restrict__ mpts,
-                                                          const IcpState *__restrict__ st,
-                                                          CSum *__restrict__ partials)
+                                                          IcpState *__restrict__ st,
+                                                          CSum *__restrict__ partials, int level,
+                                                          IcpSolveParams sp)
 {
     constexpr int NV = IcpNV<KIFI_ICP_HORN>::nv;
+    constexpr int MODE_OF_KERNEL = KIFI_ICP_HORN;
     __shared__ CSum lds[(kIT / kWave) * NV];
     if (st->level_done)
         return;
@@ -425,11 +453,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
             }
         }
     }
-    block_csum<NV>(acc, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-            partials[(size_t)blockIdx.x * NV + j] = acc[j];
+    icp_publish<MODE_OF_KERNEL>(acc, lds, partials, st, level, sp, mpose);
 }
 
 // ---------------------------------------------------------------- FP64 solve
@@ -589,23 +613,25 @@ __device__ void horn_increment(const double S[9], const double ms[3], const doub
 }
 
 // Reduce the block partials (compensated), solve in FP64, compose the pose.
+// Run by the whole last workgroup of an association kernel; the partials of the
+// other workgroups are read with device-scope loads after the ticket.
 template <int MODE>
-__global__ __launch_bounds__(kIT) void k_icp_solve(const CSum *__restrict__ partials, int nblocks,
-                                                   IcpState *__restrict__ st, int level, IcpSolveParams sp,
-                                                   const ModelPose *__restrict__ mpose)
+__device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
+                           const ModelPose *mpose, CSum *lds)
 {
     constexpr int NV = IcpNV<MODE>::nv;
-    __shared__ CSum lds[(kIT / kWave) * NV];
-    if (st->level_done)
-        return;
     CSum acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j)
         acc[j] = CSum{ 0.0f, 0.0f };
     for (int b = threadIdx.x; b < nblocks; b += kIT)
 #pragma unroll
-        for (int j = 0; j < NV; ++j)
-            cadd(acc[j], partials[(size_t)b * NV + j].s, partials[(size_t)b * NV + j].c);
+        for (int j = 0; j < NV; ++j) {
+            const float *pp = (const float *)&partials[(size_t)b * NV + j];
+            const float s = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float c = __hip_atomic_load(pp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            cadd(acc[j], s, c);
+        }
     block_csum<NV>(acc, lds);
     if (threadIdx.x != 0)
         return;
@@ -693,6 +719,7 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     st->fallback = 0;
     st->rms = 0.0f;
     st->inliers = 0;
+    st->ticket = 0;
 }
 
 __global__ void k_icp_level_begin(IcpState *__restrict__ st)
@@ -939,15 +966,11 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
             if (ip->mode == KIFI_ICP_HORN) {
                 const int nbw = (((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT;
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
-                                   w->d_state, (CSum *)w->partials);
-                hipLaunchKernelGGL(k_icp_solve<KIFI_ICP_HORN>, dim3(1), dim3(kIT), 0, st, (const CSum *)w->partials,
-                                   nbw, w->d_state, l, sp, m->d_pose);
+                                   w->d_state, (CSum *)w->partials, l, sp);
             } else {
                 hipLaunchKernelGGL(k_icp_assoc<KIFI_ICP_POINT_TO_PLANE>, dim3(nb), dim3(kIT), 0, st, lv, m->mp,
                                    m->d_pose, m->d_pts, m->d_nrm, w->d_state, ip->dist_thresh,
-                                   (CSum *)w->partials);
-                hipLaunchKernelGGL(k_icp_solve<KIFI_ICP_POINT_TO_PLANE>, dim3(1), dim3(kIT), 0, st,
-                                   (const CSum *)w->partials, nb, w->d_state, l, sp, m->d_pose);
+                                   (CSum *)w->partials, l, sp);
             }
         }
     }

@@ -50,6 +50,7 @@ struct IcpState {
     int    fallback;        // this frame re-ran plain ICP
     float  rms;
     int    inliers;
+    unsigned ticket;        // workgroups done in the current association launch
     // running totals over frames
     long long sum_iters;
     long long frames_tracked;

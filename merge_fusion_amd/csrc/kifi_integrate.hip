@@ -58,58 +58,78 @@ __global__ void k_frame_setup(Mat4 clip, Mat4 w2e_val, const Mat4 *__restrict__ 
 // Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
 // the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
 // 218, 242).  Output order is irrelevant: the sort that follows fixes it.
+// pixel_key: the key of one pixel (Integrator.cpp:109-178); false when it emits none.
+template <typename K>
+__device__ __forceinline__ bool pixel_key(const uint16_t *__restrict__ depth, const KeygenParams &p, int pix,
+                                          K &key)
+{
+    const int v = pix / p.W, u = pix - v * p.W;
+    const float d = (float)depth[pix] * 0.001f;                   // DepthStream.cpp:89-90
+    if (!(d > 0.0f))
+        return false;
+    float q[4] = { (float)u + 0.0f, 0.0f + -(float)v, 0.0f + -1.0f, 0.0f + 0.0f };
+    q[0] = q[0] * p.flInvX + p.ppX;                               // :126-129
+    q[1] = q[1] * p.flInvY + p.ppY;
+    q[2] = q[2] * 1.0f + 0.0f;
+    q[3] = q[3] * 1.0f + 0.0f;
+    q[0] = q[0] * d + 0.0f;                                       // :136-139
+    q[1] = q[1] * d + 0.0f;
+    q[2] = q[2] * d + 0.0f;
+    q[3] = q[3] * d + 1.0f;
+    mat_vec(p.mats->e2w, q, q);                                   // :141-144
+    const float ix = q[0] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;   // :146-149
+    const float iy = q[1] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
+    const float iz = q[2] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
+    const float iw = q[3] * 1.0f + 0.0f - 0.5f;
+    const bool valid = ix >= 0.0f && ix < p.maxIndex && iy >= 0.0f && iy < p.maxIndex && iz >= 0.0f &&
+                       iz < p.maxIndex && iw >= 0.0f && iw < p.maxIndex;   // :151-154
+    if (valid)
+        key = pack_key<K>((K)(uint32_t)ix, (K)(uint32_t)iy, (K)(uint32_t)iz);
+    return valid;
+}
+
+constexpr int kKgItems = 8;   // pixels per thread: one counter atomic per 2048 pixels
+
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict__ depth,
                                                      KeygenParams p, K *__restrict__ out,
                                                      FrameCounters *__restrict__ cnt)
 {
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    __shared__ uint32_t base_s;
     const int P = p.W * p.H;
-    const int pix = blockIdx.x * kThreads + threadIdx.x;
-    bool valid = false;
-    K key = 0;
-    if (pix < P) {
-        const int v = pix / p.W, u = pix - v * p.W;
-        const float d = (float)depth[pix] * 0.001f;           // DepthStream.cpp:89-90
-        if (d > 0.0f) {
-            float q[4] = { (float)u + 0.0f, 0.0f + -(float)v, 0.0f + -1.0f, 0.0f + 0.0f };
-            q[0] = q[0] * p.flInvX + p.ppX;                    // :126-129
-            q[1] = q[1] * p.flInvY + p.ppY;
-            q[2] = q[2] * 1.0f + 0.0f;
-            q[3] = q[3] * 1.0f + 0.0f;
-            q[0] = q[0] * d + 0.0f;                            // :136-139
-            q[1] = q[1] * d + 0.0f;
-            q[2] = q[2] * d + 0.0f;
-            q[3] = q[3] * d + 1.0f;
-            mat_vec(p.mats->e2w, q, q);                        // :141-144
-            const float ix = q[0] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;   // :146-149
-            const float iy = q[1] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
-            const float iz = q[2] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
-            const float iw = q[3] * 1.0f + 0.0f - 0.5f;
-            valid = ix >= 0.0f && ix < p.maxIndex && iy >= 0.0f && iy < p.maxIndex &&
-                    iz >= 0.0f && iz < p.maxIndex && iw >= 0.0f && iw < p.maxIndex; // :151-154
-            if (valid)
-                key = pack_key<K>((K)(uint32_t)ix, (K)(uint32_t)iy, (K)(uint32_t)iz);
-        }
+    K keys[kKgItems];
+    uint32_t flags = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j < kKgItems; ++j) {
+        const int pix = (blockIdx.x * kKgItems + j) * kThreads + threadIdx.x;
+        K k = 0;
+        const bool ok = pix < P && pixel_key<K>(depth, p, pix, k);
+        keys[j] = k;
+        flags |= (uint32_t)ok << j;
+        c += ok;
     }
-    // wave-aggregated append: one atomic per wavefront
-    const uint64_t m = __ballot(valid);
-    if (m == 0)
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<kThreads>(c, tmp, tot);
+    if (threadIdx.x == 0 && tot) {
+        base_s = atomicAdd(&cnt->n_cand, tot * 8u);
+        atomicAdd(&cnt->n_keys, tot);
+    }
+    __syncthreads();
+    if (!c)
         return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    const uint32_t total = popc64(m);
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) {
-        base = atomicAdd(&cnt->n_cand, total * 8u);
-        atomicAdd(&cnt->n_keys, total);
-    }
-    base = __shfl(base, leader, kWave);
-    if (valid) {
-        constexpr K dx = 1, dy = K(1) << KeyTraits<K>::kb, dz = K(1) << (2 * KeyTraits<K>::kb);
-        K *o = out + base + popc64(m & lanemask_lt()) * 8u;
+    constexpr K dx = 1, dy = K(1) << KeyTraits<K>::kb, dz = K(1) << (2 * KeyTraits<K>::kb);
+    K *o = out + base_s + ex * 8u;
+#pragma unroll
+    for (int j = 0; j < kKgItems; ++j) {
+        if (!(flags >> j & 1u))
+            continue;
+        const K key = keys[j];
         o[0] = key;           o[1] = key + dx;
         o[2] = key + dy;      o[3] = key + dy + dx;
         o[4] = key + dz;      o[5] = key + dz + dx;
         o[6] = key + dz + dy; o[7] = key + dz + dy + dx;
+        o += 8;
     }
 }
 
@@ -669,7 +689,8 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     kp.vc = vol->consts;
     kp.mats = vol->d_mats;
     const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);
-    hipLaunchKernelGGL(k_keygen<K>, dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_keygen<K>, dim3((unsigned)((P + kThreads * kKgItems - 1) / (kThreads * kKgItems))),
+                       dim3(kThreads), 0,
                        stream, d_depth, kp, (K *)s.candA, cnt);
 
     // radix sort of the candidates (even pass count: result back in candA)

@@ -1,0 +1,421 @@
+"""Pin the CPU oracle where the reference has no known answer: closed forms,
+invariants, brute force on small inputs and textbook special cases.
+
+Reference functions covered (KinectFission/kifi/...): Integrator.cpp:76-182
+(keygen), :186-250 (expand), algorithm.h:138-198 (set_union), Integrator.cpp:253-358
+(update), Mesher.cpp:85-229 (mesher), Renderer.cpp:39-80 (Bin), ICP.cpp:166-264
+(Horn), and the binary16 storage of this build.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import merge_fusion_amd as mf
+import oracle as orc
+
+L = orc.lib()
+
+
+def cam_of(w, h, f):
+    return orc.Camera(w, h, f, f, w * 0.5, h * 0.5, 0.4, 8.0)
+
+
+IDENT = np.eye(4, dtype=np.float32).reshape(16)
+
+
+# ------------------------------------------------------------------ keygen
+def test_keygen_fronto_parallel_plane_lands_in_one_slab():
+    """A plane at 1.5 m in front of an identity camera maps to z index 31
+    (res 256, side 4 m: vl = 1/64, index = z*64 + 128 - 0.5 = 31.5)."""
+    cam = cam_of(64, 48, 50.0)
+    depth = np.full((48, 64), np.float32(1500) * np.float32(0.001), np.float32)
+    desc = orc.volume_desc(256, 4.0, 0.02)
+    keys = orc.keygen(depth, cam, orc.invert_transform(IDENT), desc)
+    assert len(keys) == 64 * 48
+    assert np.all(keys >> 20 == 31)
+    x, y = keys & 0x3FF, (keys >> 10) & 0x3FF
+    # pixel u maps to x = (u + 0.5 - 32) / 50 * 1.5 m -> index ~ x*64 + 127.5
+    assert x.min() >= 125 - 62 and x.max() <= 128 + 62
+    assert len(np.unique(x)) > 50 and len(np.unique(y)) > 40
+
+
+def test_keygen_matches_fp64_away_from_voxel_boundaries():
+    rng = np.random.default_rng(5)
+    cam = cam_of(16, 12, 20.0)
+    mm = rng.integers(500, 3000, (12, 16)).astype(np.uint16)
+    mm[rng.random((12, 16)) < 0.1] = 0
+    depth = mm.astype(np.float32) * np.float32(0.001)
+    ang = 0.3
+    e2w = np.eye(4)
+    e2w[:3, :3] = [[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]]
+    e2w[:3, 3] = [0.1, -0.2, 0.3]
+    e2w32 = e2w.T.reshape(16).astype(np.float32)
+    desc = orc.volume_desc(256, 4.0, 0.02)
+    got = orc.keygen(depth, cam, e2w32, desc)
+    # FP64 brute force of the same geometry (pixel centre ray, depth along -z)
+    want, amb = [], 0
+    E = e2w32.astype(np.float64).reshape(4, 4).T
+    for v in range(12):
+        for u in range(16):
+            d = float(depth[v, u])
+            if d <= 0:
+                continue
+            pe = np.array([(u + 0.5 - 8) / 20 * d, (6 - v - 0.5) / 20 * d, -d, 1.0])
+            pw = E @ pe
+            idx = pw[:3] * 64.0 + 128.0 - 0.5
+            if np.any(np.abs(idx - np.round(idx)) < 1e-4):
+                amb += 1
+                continue
+            if np.all(idx >= 0) and np.all(idx < 255):
+                i = idx.astype(np.int64)
+                want.append(int(i[2]) << 20 | int(i[1]) << 10 | int(i[0]))
+    got_set = set(int(k) for k in got)
+    assert set(want) <= got_set
+    assert len(got) - len(want) <= amb
+
+
+def test_keygen_invariants_on_rendered_frame():
+    cam = mf.Intrinsics(512, 424, 365.0, 365.0)
+    sc = mf.make_scene(mf.SCENE_ROOM, 3)
+    pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, 3, 0)
+    mm = mf.render(sc, cam, pose, noise_seed=1)
+    d = mm.astype(np.float32) * np.float32(0.001)
+    desc = orc.volume_desc(1024, 4.096, 0.02)
+    keys = orc.keygen(d, orc.camera(cam), orc.invert_transform(mf.invert_rigid(pose).astype(np.float32)), desc)
+    assert 0 < len(keys) <= int((mm > 0).sum())
+    for s in (0, 10, 20):
+        assert ((keys >> s) & 0x3FF).max() < 1023
+
+
+# ------------------------------------------------------------------ expand
+@pytest.mark.parametrize("seed", range(4))
+def test_expand_equals_neighbourhood_minus_max(seed):
+    """ExpandChunks = {k + dx + dy + dz} over the 2x2x2 offsets, minus the maximum
+    (the x pass never emits the last element's +1, Integrator.cpp:231-245)."""
+    rng = np.random.default_rng(seed)
+    n = [1, 2, 50, 3000][seed]
+    xyz = rng.integers(0, 60, (n, 3))
+    keys = np.unique(xyz[:, 2] << 20 | xyz[:, 1] << 10 | xyz[:, 0]).astype(np.uint64)
+    got = orc.expand(keys)
+    full = set()
+    for k in keys.tolist():
+        for dz in (0, 1 << 20):
+            for dy in (0, 1 << 10):
+                for dx in (0, 1):
+                    full.add(k + dz + dy + dx)
+    full.discard(max(full))
+    assert got.tolist() == sorted(full)
+
+
+def test_expand_hand_cases():
+    assert orc.expand(np.array([], np.uint64)).tolist() == []
+    # one key: 7 of its 8 corners
+    assert len(orc.expand(np.array([5], np.uint64))) == 7
+
+
+# ------------------------------------------------------------------ merge
+@pytest.mark.parametrize("seed", range(3))
+def test_set_union_matches_dict_first_wins(seed):
+    rng = np.random.default_rng(seed)
+    k1 = np.unique(rng.integers(0, 5000, 800)).astype(np.uint64)
+    v1 = rng.integers(-100, 100, len(k1)).astype(np.int32)
+    k2 = np.unique(rng.integers(0, 5000, 600)).astype(np.uint64)
+    ko, vo = orc.set_union_u32(k1, v1, k2, 777)
+    ref = {int(k): int(v) for k, v in zip(k1, v1)}
+    for k in k2.tolist():
+        ref.setdefault(k, 777)
+    assert ko.tolist() == sorted(ref)
+    assert vo.tolist() == [ref[k] for k in sorted(ref)]
+
+
+# ------------------------------------------------------------------ update
+def _axis_volume(zs, res=256, side=4.0, trunc=0.02):
+    """Voxels on the optical axis column x = y = 128 at z indices zs (sorted)."""
+    v = orc.Volume(res, side, trunc)
+    keys = np.array([z << 20 | 128 << 10 | 128 for z in zs], np.uint64)
+    v.write(keys, np.zeros((len(keys), 2), np.float32))
+    return v, keys
+
+
+def test_update_projective_distance_along_the_axis():
+    """Voxel centres on the axis at z = (i + 0.5)/64 - 2; a plane at depth 1.5 m:
+    sd = 1.5 - (-z); updates iff sd >= -trunc, with min(sd, trunc)."""
+    zs = list(range(20, 44))                     # centres from -1.68 m to -1.30 m
+    vol, keys = _axis_volume(zs)
+    cam = cam_of(32, 24, 30.0)
+    mm = np.full((24, 32), 1500, np.uint16)
+    w2c = orc.mat_mul(orc.eye_to_clip(cam), IDENT)
+    depth = mm.astype(np.float32) * np.float32(0.001)
+    vol2 = orc.Volume(256, 4.0, 0.02)
+    vals = np.zeros((len(keys), 2), np.float32)
+    L.orc_update(orc.u64p(keys), vals.ctypes.data, len(keys), 0, orc.f32p(depth), ctypes.byref(cam),
+                 orc.f32p(w2c), ctypes.byref(vol2.desc))
+    for i, z in enumerate(zs):
+        zc = (z + 0.5) / 64.0 - 2.0
+        sd = 1.5 + zc
+        if i >= len(zs) // 4 * 4:
+            assert vals[i, 1] == 0
+        elif sd < -0.02 - 1e-6:
+            assert vals[i, 1] == 0, (z, sd)
+        elif sd >= -0.02 + 1e-6:
+            assert vals[i, 1] == 1
+            assert abs(vals[i, 0] - min(sd, 0.02)) < 2e-6, (z, sd, vals[i])
+
+
+def test_update_behind_camera_mirrored_voxel_gets_plus_trunc():
+    """Q3: no w > 0 test.  A voxel behind the camera whose mirrored projection
+    lands in the image receives Update(+trunc)."""
+    keys = np.array([z << 20 | 128 << 10 | 128 for z in (140, 141, 142, 143)], np.uint64)
+    vol = orc.Volume(256, 4.0, 0.02)
+    cam = cam_of(32, 24, 30.0)
+    depth = np.full((24, 32), 1.5, np.float32)
+    w2c = orc.mat_mul(orc.eye_to_clip(cam), IDENT)
+    vals = np.zeros((4, 2), np.float32)
+    L.orc_update(orc.u64p(keys), vals.ctypes.data, 4, 0, orc.f32p(depth), ctypes.byref(cam), orc.f32p(w2c),
+                 ctypes.byref(vol.desc))
+    assert np.all(vals[:, 1] == 1) and np.all(vals[:, 0] == np.float32(0.02))
+
+
+def test_update_weight_counts_and_fp64_agreement():
+    cam = mf.Intrinsics(512, 424, 365.0, 365.0)
+    sc = mf.make_scene(mf.SCENE_PLANE_SPHERES, 4)
+    pose = mf.trajectory_pose(mf.TRAJ_STATIC, 4, 0)
+    v = orc.Volume(1024, 8.192, 0.04)
+    for f in range(3):
+        v.integrate(mf.render(sc, cam, pose, noise_seed=f), cam, mf.invert_rigid(pose).astype(np.float32))
+    keys, vals = v.read()
+    assert vals[:, 1].max() <= 3 and np.all(vals[:, 1] == np.round(vals[:, 1]))
+    w = vals[:, 1] > 0
+    assert np.all(np.abs(vals[w, 0] / vals[w, 1]) <= 0.04 + 1e-6)
+    # FP64 restatement of the last frame's update on the same keys: same decisions
+    # up to boundary cases, and distances within binary32 rounding
+    mm = mf.render(sc, cam, pose, noise_seed=2)
+    d = mm.astype(np.float32) * np.float32(0.001)
+    w2c = orc.mat_mul(orc.eye_to_clip(cam), mf.invert_rigid(pose).astype(np.float32)).astype(np.float64)
+    sw = np.zeros(2 * len(keys))
+    L.orc_update_f64(orc.u64p(keys), sw.ctypes.data_as(orc.F64P), len(keys), orc.f32p(d),
+                     ctypes.byref(orc.camera(cam)), w2c.ctypes.data_as(orc.F64P), ctypes.byref(v.desc))
+    f32 = np.zeros((len(keys), 2), np.float32)
+    L.orc_update(orc.u64p(keys), f32.ctypes.data, len(keys), 0, orc.f32p(d), ctypes.byref(orc.camera(cam)),
+                 orc.f32p(w2c.astype(np.float32)), ctypes.byref(v.desc))
+    both = (f32[:, 1] > 0) & (sw[1::2] > 0)
+    assert both.sum() > 0.99 * max((f32[:, 1] > 0).sum(), (sw[1::2] > 0).sum())
+    diff = np.abs(f32[both, 0] - sw[0::2][both])
+    # a voxel whose centre projects within rounding of a pixel edge may read the
+    # neighbouring pixel's depth in one precision: rare, everything else agrees
+    assert np.median(diff) < 1e-6 and np.mean(diff > 1e-5) < 0.005
+
+
+# ------------------------------------------------------------------ binary16
+def test_f16_roundtrip_all_finite_halves():
+    h = np.arange(65536, dtype=np.uint16)
+    f = np.array([L.orc_f16_to_f32(int(x)) for x in h[::7]], np.float32)
+    ref = h[::7].view(np.float16).astype(np.float32)
+    fin = np.isfinite(ref)
+    assert np.array_equal(f[fin].view(np.uint32), ref[fin].view(np.uint32))
+    back = np.array([L.orc_f32_to_f16(float(x)) for x in f[fin]], np.uint16)
+    assert np.array_equal(back, h[::7][fin])
+
+
+def test_f32_to_f16_rounds_to_nearest_even():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.standard_normal(4000).astype(np.float32) * 1e-3,
+                        (rng.standard_normal(2000) * 1e-6).astype(np.float32),     # subnormal halves
+                        np.array([65504, 65520, 1e6, -1e6, 6e-8, 2.98e-8, 0.0, -0.0], np.float32)])
+    got = np.array([L.orc_f32_to_f16(float(v)) for v in x], np.uint16)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------------ mesher
+def _sphere_volume(res=64, side=2.0, r=0.53, c=(0.013, -0.021, 0.017), band=3):
+    vl = side / res
+    idx = np.arange(res)
+    centre = idx * vl + (vl * 0.5 - side * 0.5)
+    X, Y, Z = np.meshgrid(centre, centre, centre, indexing="ij")
+    D = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r
+    sel = np.abs(D) < band * vl
+    xi, yi, zi = np.nonzero(sel)
+    keys = (zi.astype(np.uint64) << 20) | (yi.astype(np.uint64) << 10) | xi.astype(np.uint64)
+    order = np.argsort(keys)
+    vals = np.zeros((len(keys), 2), np.float32)
+    vals[:, 0] = D[sel][order]
+    vals[:, 1] = 1.0
+    v = orc.Volume(res, side, 0.1)
+    v.write(keys[order], vals)
+    return v, np.array(c), r, vl
+
+
+def test_mesher_sphere_is_a_closed_genus0_surface():
+    v, c, r, vl = _sphere_volume()
+    verts, idx, stalled = v.mesh_indexed()
+    assert not stalled and len(idx) % 3 == 0 and idx.max() < len(verts)
+    tris = idx.reshape(-1, 3)
+    edges = {}
+    for t in tris:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            e = (min(a, b), max(a, b))
+            edges[e] = edges.get(e, 0) + 1
+    assert set(edges.values()) == {2}, "every edge shared by exactly two triangles"
+    used = np.unique(tris)
+    assert len(used) - len(edges) + len(tris) == 2, "Euler characteristic of a sphere"
+    dist = np.abs(np.linalg.norm(verts[used] - c, axis=1) - r)
+    assert dist.max() < vl
+
+
+def test_mesher_vertex_only_variant_matches_indexed_vertex_set():
+    v, c, r, vl = _sphere_volume(res=32, r=0.47)
+    a = v.mesh()
+    b, _, _ = v.mesh_indexed()
+    assert len(a) == len(b)
+    sa = a[np.lexsort(a.T)]
+    sb = b[np.lexsort(b.T)]
+    assert np.array_equal(sa, sb)
+
+
+# corner c of the cube at offsets (c&1, c>>1&1, c>>2&1) -> bit of lutIdx (Mesher.cpp:191-199)
+LUT_BIT = {0: 2, 1: 3, 2: 6, 3: 7, 4: 1, 5: 0, 6: 5, 7: 4}
+
+
+@pytest.mark.parametrize("case", range(0, 256, 1))
+def test_mesher_single_cube_every_case(case):
+    """A 2x2x2 volume with corner signs chosen so that lutIdx == case.  Every
+    triangle vertex must be an emitted edge vertex with a sign change, and the
+    triangle count must be the table's."""
+    offs = orc.lib().orc_tri_offsets()
+    ntri = (offs[case + 1] - offs[case]) // 3
+    keys, vals = [], []
+    for cz in range(2):
+        for cy in range(2):
+            for cx in range(2):
+                c = cx | cy << 1 | cz << 2
+                neg = case >> LUT_BIT[c] & 1
+                keys.append(cz << 20 | cy << 10 | cx)
+                vals.append((-0.3 - 0.01 * c if neg else 0.2 + 0.01 * c, 1.0))
+    # a trailing weight-0 voxel keeps corner 7 from being the last voxel, whose
+    # neighbour lookups the reference clamps away (Mesher.cpp:98-100, 117)
+    keys.append(5 << 20)
+    vals.append((0.0, 0.0))
+    order = np.argsort(keys)
+    v = orc.Volume(16, 1.6, 0.5)
+    v.write(np.array(keys, np.uint64)[order], np.array(vals, np.float32)[order])
+    verts, idx, stalled = v.mesh_indexed()
+    assert len(idx) == 3 * ntri
+    assert not stalled
+    if ntri:
+        assert idx.max() < len(verts)
+        # each referenced vertex lies strictly between two corners of opposite sign
+        for p in verts[np.unique(idx)]:
+            frac = (p - (0.05 - 0.8)) / 0.1           # voxel units from corner 0's centre
+            inner = np.abs(frac - np.round(frac)) > 1e-4
+            assert inner.sum() == 1
+
+
+def test_mesher_last_voxel_quirk_stalls_the_index_join():
+    """Q6 + Q8: in a bare 2x2x2 volume corner 7 is the last voxel, so corner 5's
+    +y neighbour is never found (the walk starts at V-2), edge 8 gets no vertex,
+    and the ID join stalls; every index from the stalled ID on keeps its raw ID."""
+    keys, vals = [], []
+    for cz in range(2):
+        for cy in range(2):
+            for cx in range(2):
+                c = cx | cy << 1 | cz << 2
+                keys.append(cz << 20 | cy << 10 | cx)
+                vals.append((-0.3 if c == 5 else 0.2, 1.0))      # lutIdx == 1
+    v = orc.Volume(16, 1.6, 0.5)
+    v.write(np.array(keys, np.uint64), np.array(vals, np.float32))
+    verts, idx, stalled = v.mesh_indexed()
+    assert stalled and len(idx) == 3
+    raw_edge8 = 3 * ((1 << 20) | 1) + 1                         # pack(x1, y0, z1) * 3 + 1
+    assert raw_edge8 in idx.tolist()
+
+
+# ------------------------------------------------------------------ Bin
+def test_bin_nearest_point_wins_brute_force():
+    rng = np.random.default_rng(2)
+    W, H = 40, 30
+    cam = cam_of(W, H, 30.0)
+    pts = np.stack([rng.uniform(-1, 1, 3000), rng.uniform(-0.8, 0.8, 3000), rng.uniform(-3.5, -0.6, 3000)], 1)
+    pts = pts.astype(np.float32)
+    w2c = orc.mat_mul(orc.eye_to_clip(cam), IDENT)
+    out = orc.bin_points(pts, w2c, W, H)
+    P = pts.astype(np.float64)
+    M = w2c.astype(np.float64).reshape(4, 4).T
+    clip = np.c_[P, np.ones(len(P))] @ M.T
+    ndc = clip[:, :3] / clip[:, 3:]
+    fu, fv = ndc[:, 0] * W / 2 + W / 2, ndc[:, 1] * H / 2 + H / 2
+    u, v = np.trunc(fu).astype(int), H - 1 - np.trunc(fv).astype(int)
+    ok = (u >= 0) & (u < W) & (v >= 0) & (v < H) & (np.abs(ndc[:, 2]) <= 1)
+    safe = (np.abs(fu - np.round(fu)) > 1e-3) & (np.abs(fv - np.round(fv)) > 1e-3)
+    best = {}
+    for i in np.nonzero(ok)[0]:
+        key = (v[i], u[i])
+        if key not in best or ndc[i, 2] < ndc[best[key], 2]:
+            best[key] = i
+    checked = 0
+    for (vv, uu), i in best.items():
+        cand = [j for j in np.nonzero(ok & (u == uu) & (v == vv))[0]]
+        if not all(safe[j] for j in cand):
+            continue
+        assert np.array_equal(out[vv, uu], pts[i])
+        checked += 1
+    assert checked > 500
+    assert (np.abs(out).sum(-1) > 0).sum() >= checked
+
+
+# ------------------------------------------------------------------ Horn / eigen
+def test_jacobi_known_spectrum():
+    rng = np.random.default_rng(3)
+    lam = np.array([3.0, -1.0, 0.5, 7.25])
+    ev, V = orc.jacobi4(np.diag(lam))
+    assert np.allclose(sorted(ev), sorted(lam), atol=0, rtol=0)
+    Q, _ = np.linalg.qr(rng.standard_normal((4, 4)))
+    A = Q @ np.diag(lam) @ Q.T
+    ev, V = orc.jacobi4(A.T.reshape(16))
+    assert np.allclose(sorted(ev), sorted(lam), atol=1e-12)
+    for i in range(4):
+        assert np.linalg.norm(A @ V[:, i] - ev[i] * V[:, i]) < 1e-12
+
+
+@pytest.mark.parametrize("ang", [0.5, 10.0, 90.0])
+def test_horn_recovers_known_rigid_motion(ang):
+    """The test sketched at ICP.cpp:134-164: random points, known (R, t)."""
+    rng = np.random.default_rng(int(ang * 10))
+    src = rng.random((500, 3)).astype(np.float32)
+    ax = rng.standard_normal(3)
+    ax /= np.linalg.norm(ax)
+    a = np.radians(ang)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+    t = np.array([0.3, -0.1, 0.05])
+    dst = (src.astype(np.float64) @ R.T + t).astype(np.float32)
+    T = orc.horn_step(src, dst).reshape(4, 4).T
+    assert np.abs(T[:3, :3] - R).max() < 1e-5
+    assert np.abs(T[:3, 3] - t).max() < 1e-5
+
+
+# ------------------------------------------------------------------ storage precision
+def test_f16_storage_topology_equals_f32_and_distance_error_bounded():
+    """The binary16 + uint8 layout keeps the same voxel set; distances stay
+    within binary16 resolution of the binary32 sum/weight."""
+    cam = mf.Intrinsics(512, 424, 365.0, 365.0)
+    sc = mf.make_scene(mf.SCENE_ROOM, 6)
+    a = orc.Volume(1024, 4.096, 0.02, storage=orc.STORE_F32)
+    b = orc.Volume(1024, 4.096, 0.02, storage=orc.STORE_F16U8)
+    for f in range(3):
+        pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, 6, 2 * f)
+        d = mf.render(sc, cam, pose, noise_seed=f)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        a.integrate(d, cam, w2e)
+        b.integrate(d, cam, w2e)
+    ka, va = a.read()
+    kb, vb = b.read()
+    assert np.array_equal(ka, kb)
+    w = (vb >> 16) & 0xFF
+    assert np.array_equal(w.astype(np.float32), va[:, 1])
+    m = w > 0
+    d16 = np.array([L.orc_voxel_distance_f16u8(int(x)) for x in vb[m][::50]], np.float32)
+    d32 = (va[m, 0] / va[m, 1])[::50]
+    assert np.abs(d16 - d32).max() < 2e-5      # binary16 spacing at 0.02 m is 1.5e-5
