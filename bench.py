@@ -175,11 +175,22 @@ def main():
     alg_bytes = (S_K + S_V) * (dv0 + dv1) + S_K * de + 2 * W * H * launches
     mu_ms, mu_n = stage["merge_update"]
     achieved = alg_bytes / launches / (mu_ms / mu_n * 1e-3) / 1e9 if mu_n else 0.0
-    traffic = None
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary (copy-kernel calibrated).  When it was measured on these very
+    # launches (same warm-up and step count) it is used as is; otherwise its
+    # measured traffic/algorithmic ratio is applied and the note says so.
+    traffic, traffic_note = None, None
     pmc_path = os.path.join(ROOT, "profiles", "merge_update_pmc.json")
-    if os.path.exists(pmc_path):
+    if os.path.exists(pmc_path) and launches:
         with open(pmc_path) as fh:
-            traffic = json.load(fh).get("traffic_bytes_per_launch")
+            pmc = json.load(fh)
+        if pmc.get("warmup") == args.warmup and pmc.get("steps") == args.steps:
+            traffic = int(pmc["traffic_bytes_per_launch"])
+            traffic_note = "measured on these launches (%s)" % pmc.get("source")
+        elif pmc.get("traffic_over_alg"):
+            traffic = int(pmc["traffic_over_alg"] * alg_bytes / launches)
+            traffic_note = "measured traffic/algorithmic ratio %.3f (%s, %s) x this run's algorithmic bytes" % (
+                pmc["traffic_over_alg"], pmc.get("launches"), pmc.get("source"))
 
     icp_frames = st1["frames"] - st0["frames"]
     icp_iters = (st1["sum_iters"] - st0["sum_iters"]) / max(icp_frames, 1)
@@ -212,6 +223,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_note": traffic_note,
             "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)),
             "avg_launch_ms": round(mu_ms / mu_n, 5) if mu_n else None,
             "launches": mu_n,

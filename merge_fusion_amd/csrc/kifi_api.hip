@@ -9,7 +9,7 @@
 namespace kifi {
 
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                               const float *w2e, const Mat4 *w2e_dev);
+                               const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
                           size_t *nverts, size_t *nidx, int *stalled);
 kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,
@@ -443,7 +443,7 @@ kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, con
     kifi_status st = check_cam(v->ctx, cam);
     if (st != KIFI_OK)
         return st;
-    return integrate_dispatch(v, depth_dev, cam, worldToEye, nullptr);
+    return integrate_dispatch(v, depth_dev, cam, worldToEye, nullptr, nullptr);
 }
 
 kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_intrinsics *cam,
@@ -466,7 +466,7 @@ kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_
         ctx->depth_pixels = P;
     }
     KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream));
-    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye, nullptr);
+    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye, nullptr, nullptr);
 }
 
 kifi_status kifi_integrate_stats(kifi_volume *v, size_t stats[4])

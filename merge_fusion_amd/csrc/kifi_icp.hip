@@ -292,6 +292,7 @@ __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CSum *lds, CSum *pa
     __syncthreads();
     if (!last)
         return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);       // device scope: see every workgroup's partial
     __threadfence();
     icp_finish<MODE>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
     if (threadIdx.x == 0)
@@ -314,8 +315,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
 #pragma unroll
     for (int j = 0; j < NV; ++j)
         acc[j] = CSum{ 0.0f, 0.0f };
-    const int q = blockIdx.x * kIT + threadIdx.x;
-    if (q < lv.W * lv.H) {
+    for (int q = blockIdx.x * kIT + threadIdx.x; q < lv.W * lv.H; q += gridDim.x * kIT) {
         const int v = q / lv.W, u = q - v * lv.W;
         const float d = lv.depth[q];
         if (d > 0.0f) {
@@ -345,25 +345,28 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
                                 for (int i = 0; i < 6; ++i)
 #pragma unroll
                                     for (int j = 0; j <= i; ++j)
-                                        acc[t++].s = wgt * J[i] * J[j];
+                                    {
+                                        cadd(acc[t], wgt * J[i] * J[j], 0.0f);
+                                        ++t;
+                                    }
 #pragma unroll
                                 for (int i = 0; i < 6; ++i)
-                                    acc[21 + i].s = wgt * J[i] * r;
-                                acc[27].s = 1.0f;
-                                acc[28].s = r * r;
+                                    cadd(acc[21 + i], wgt * J[i] * r, 0.0f);
+                                cadd(acc[27], 1.0f, 0.0f);
+                                cadd(acc[28], r * r, 0.0f);
                             } else {
                                 const float a[3] = { p[0] - mpose->cam[0], p[1] - mpose->cam[1], p[2] - mpose->cam[2] };
                                 const float b[3] = { Q.x - mpose->cam[0], Q.y - mpose->cam[1], Q.z - mpose->cam[2] };
 #pragma unroll
                                 for (int i = 0; i < 3; ++i) {
-                                    acc[i].s = a[i];
-                                    acc[3 + i].s = b[i];
+                                    cadd(acc[i], a[i], 0.0f);
+                                    cadd(acc[3 + i], b[i], 0.0f);
 #pragma unroll
                                     for (int j = 0; j < 3; ++j)
-                                        acc[6 + 3 * i + j].s = a[i] * b[j];
+                                        cadd(acc[6 + 3 * i + j], a[i] * b[j], 0.0f);
                                 }
-                                acc[15].s = 1.0f;
-                                acc[16].s = (dx * dx + dy * dy) + dz * dz;
+                                cadd(acc[15], 1.0f, 0.0f);
+                                cadd(acc[16], (dx * dx + dy * dy) + dz * dz, 0.0f);
                             }
                         }
                     }
@@ -396,8 +399,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
     for (int j = 0; j < NV; ++j)
         acc[j] = CSum{ 0.0f, 0.0f };
     const int cols = (lv.W + 3) / 4;
-    const int q = blockIdx.x * kIT + threadIdx.x;
-    if (q < cols * lv.H) {
+    for (int q = blockIdx.x * kIT + threadIdx.x; q < cols * lv.H; q += gridDim.x * kIT) {
         const int y = q / cols, x = 4 * (q - y * cols);
         const float d = lv.depth[(size_t)y * lv.W + x];
         if (0.0f != d) {
@@ -441,14 +443,14 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
                     const float b[3] = { best.x - mpose->cam[0], best.y - mpose->cam[1], best.z - mpose->cam[2] };
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
-                        acc[i].s = a[i];
-                        acc[3 + i].s = b[i];
+                        cadd(acc[i], a[i], 0.0f);
+                        cadd(acc[3 + i], b[i], 0.0f);
 #pragma unroll
                         for (int j = 0; j < 3; ++j)
-                            acc[6 + 3 * i + j].s = a[i] * b[j];
+                            cadd(acc[6 + 3 * i + j], a[i] * b[j], 0.0f);
                     }
-                    acc[15].s = 1.0f;
-                    acc[16].s = mind;
+                    cadd(acc[15], 1.0f, 0.0f);
+                    cadd(acc[16], mind, 0.0f);
                 }
             }
         }
@@ -624,13 +626,13 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 #pragma unroll
     for (int j = 0; j < NV; ++j)
         acc[j] = CSum{ 0.0f, 0.0f };
+    // plain loads: the caller's device-scope acquire fence after the ticket has
+    // invalidated this CU's L1, so they read the other workgroups' partials from L2
     for (int b = threadIdx.x; b < nblocks; b += kIT)
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            const float *pp = (const float *)&partials[(size_t)b * NV + j];
-            const float s = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float c = __hip_atomic_load(pp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            cadd(acc[j], s, c);
+            const CSum p = partials[(size_t)b * NV + j];
+            cadd(acc[j], p.s, p.c);
         }
     block_csum<NV>(acc, lds);
     if (threadIdx.x != 0)
@@ -801,44 +803,79 @@ __global__ void k_icp_w2e(const IcpState *__restrict__ st, Mat4 *__restrict__ w2
 static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 
 template <typename K, typename V>
-static void launch_model(kifi_model *m, kifi_volume *vol, hipStream_t st)
+static void launch_model(kifi_model *m, kifi_volume *vol, hipStream_t st, bool splat)
 {
-    ModelParams mp = m->mp;
-    mp.vl = vol->consts.vl;
-    mp.c0 = vol->consts.vlHalfPlusMin;
-    m->mp = mp;
+    const ModelParams mp = m->mp;
     const int P = mp.W * mp.H;
     const K *keys = (const K *)vol->keys[vol->cur];
     const V *vals = (const V *)vol->vals[vol->cur];
-    hipLaunchKernelGGL(k_fill_zbuf, dim3((P + 255) / 256), dim3(256), 0, st, m->d_zbuf, P);
-    const unsigned grid = (unsigned)std::max(1, vol->ctx->num_cus * 8);
-    hipLaunchKernelGGL((k_model_splat<K, V>), dim3(grid), dim3(kIT), 0, st, keys, vals, vol->d_size, mp, m->d_pose,
-                       m->d_zbuf);
+    if (splat) {
+        const unsigned grid = (unsigned)std::max(1, vol->ctx->num_cus * 8);
+        hipLaunchKernelGGL((k_model_splat<K, V>), dim3(grid), dim3(kIT), 0, st, keys, vals, vol->d_size, mp,
+                           m->d_pose, m->d_zbuf);
+    }
     hipLaunchKernelGGL((k_model_resolve<K, V>), dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, keys, vals, mp,
                        m->d_pose, m->d_zbuf, m->d_pts);
     hipLaunchKernelGGL(k_model_normals, dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, m->d_pts, mp.W, mp.H,
                        m->d_pose, m->disc, m->d_nrm);
 }
 
-kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev)
+// Model pose of this prediction and a cleared z-buffer.
+void model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev)
 {
-    kifi_ctx *ctx = m->ctx;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = m->ctx->stream;
     Mat4 wv;
     for (int i = 0; i < 16; ++i)
         wv.m[i] = w2e_host ? w2e_host[i] : 0.0f;
     Mat4 clip;
     eye_to_clip_rh(m->cam.width, m->cam.height, m->cam.fx, m->cam.fy, m->cam.cx, m->cam.cy, m->cam.znear,
                    m->cam.zfar, clip.m);
-    const int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    m->mp.vl = vol->consts.vl;
+    m->mp.c0 = vol->consts.vlHalfPlusMin;
+    m->mp.dmax = vol->consts.vl;                 // near-surface band: |D| < one voxel
+    const int P = m->mp.W * m->mp.H;
     hipLaunchKernelGGL(k_model_pose, dim3(1), dim3(64), 0, st, wv, w2e_dev, clip, m->d_pose);
+    hipLaunchKernelGGL(k_fill_zbuf, dim3((P + 255) / 256), dim3(256), 0, st, m->d_zbuf, P);
+}
+
+// Splat target for the fused voxel pass of an integration.
+SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev)
+{
+    SplatParams sp;
+    sp.w2e = w2e_dev;
+    sp.zbuf = m->d_zbuf;
+    sp.W = m->mp.W;
+    sp.H = m->mp.H;
+    sp.fx = m->mp.fx;
+    sp.fy = m->mp.fy;
+    sp.cx = m->mp.cx;
+    sp.cy = m->mp.cy;
+    sp.vl = m->mp.vl;
+    sp.c0 = m->mp.c0;
+    sp.dmax = m->mp.dmax;
+    sp.znear = m->mp.znear;
+    return sp;
+}
+
+// Resolve + normals (and the splat itself unless it was fused).
+void model_end(kifi_model *m, kifi_volume *vol, bool splat)
+{
+    hipStream_t st = m->ctx->stream;
     if (vol->desc.key_bits == 32) {
-        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint32_t, VoxF32i>(m, vol, st);
-        else launch_model<uint32_t, uint32_t>(m, vol, st);
+        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint32_t, VoxF32i>(m, vol, st, splat);
+        else launch_model<uint32_t, uint32_t>(m, vol, st, splat);
     } else {
-        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint64_t, VoxF32i>(m, vol, st);
-        else launch_model<uint64_t, uint32_t>(m, vol, st);
+        if (vol->desc.storage == KIFI_STORAGE_F32) launch_model<uint64_t, VoxF32i>(m, vol, st, splat);
+        else launch_model<uint64_t, uint32_t>(m, vol, st, splat);
     }
+}
+
+kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev)
+{
+    kifi_ctx *ctx = m->ctx;
+    const int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    model_begin(m, vol, w2e_host, w2e_dev);
+    model_end(m, vol, true);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
     KIFI_HIP(ctx, hipGetLastError());
     return KIFI_OK;
@@ -960,11 +997,14 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
         lv.cx = cam->cx * sc;
         lv.cy = cam->cy * sc;
         lv.depth = w->lv[l];
-        const int nb = (lv.W * lv.H + kIT - 1) / kIT;
+        // grid-stride: at most one workgroup per CU, so the last workgroup's
+        // reduction covers few partials
+        const int cap = std::max(1, m->ctx->num_cus);
+        const int nb = std::min((lv.W * lv.H + kIT - 1) / kIT, cap);
         hipLaunchKernelGGL(k_icp_level_begin, dim3(1), dim3(64), 0, st, w->d_state);
         for (int it = 0; it < ip->iters[l]; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
-                const int nbw = (((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT;
+                const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, cap);
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
                                    w->d_state, (CSum *)w->partials, l, sp);
             } else {
@@ -974,6 +1014,20 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
             }
         }
     }
+}
+
+__global__ void k_store_mat(Mat4 v, Mat4 *__restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *out = v;
+}
+
+void store_w2e(const float *w2e, Mat4 *d_out, hipStream_t st)
+{
+    Mat4 m;
+    for (int i = 0; i < 16; ++i)
+        m.m[i] = w2e[i];
+    hipLaunchKernelGGL(k_store_mat, dim3(1), dim3(64), 0, st, m, d_out);
 }
 
 void icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st)

@@ -82,10 +82,14 @@ namespace kifi {
 kifi_status model_alloc(kifi_model *m);
 void        model_free(kifi_model *m);
 kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
+void        model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
+SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev);
+void        model_end(kifi_model *m, kifi_volume *vol, bool splat);
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 void        icp_free(IcpWork *w);
 kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip);
 void        icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st);
+void        store_w2e(const float *w2e, Mat4 *d_out, hipStream_t st);
 kifi_status horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16]);
 } // namespace kifi

@@ -487,6 +487,11 @@ __device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, float
                    1.0f };                                         // Volume.cpp:75-78
     mat_vec(p.mats->w2c, k, k);                                    // :290-293
     const float kw = k[3];
+    // |x/w| or |y/w| above 1.01 lands outside [0,W) x [0,H) whatever the rounding,
+    // so such voxels (most of the volume) skip the four divisions; NaN passes on
+    const float lim = 1.01f * fabsf(kw);
+    if (fabsf(k[0]) > lim || fabsf(k[1]) > lim)
+        return false;
     float u = k[0] / kw, v = k[1] / kw, zz = k[2] / kw, ww = k[3] / kw;  // :305-308
     u = u * p.ndcU + p.ndcU;                                       // :310-313
     v = v * p.ndcV + p.ndcV;
@@ -523,15 +528,76 @@ __device__ __forceinline__ void voxel_apply(uint32_t &v, float s)
     v = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)dn) | (wn << 16);
 }
 
+// Model prediction folded into the voxel pass of a tracked frame: every
+// near-surface voxel (|D| < dmax after its update) is splatted into the model
+// z-buffer at the frame's pose, keyed by its index in the merged volume.  Same
+// arithmetic as the stand-alone model splat (kifi_icp.hip).
+__device__ __forceinline__ float vox_weight(const VoxF32 &v) { return v.weight; }
+__device__ __forceinline__ float vox_dist(const VoxF32 &v) { return v.sum / v.weight; }
+__device__ __forceinline__ float vox_weight(uint32_t v) { return (float)((v >> 16) & 0xffu); }
+__device__ __forceinline__ float vox_dist(uint32_t v)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+}
+
+__device__ __forceinline__ uint32_t orderable_z(float z)
+{
+    if (z == 0.0f)
+        z = 0.0f;
+    const uint32_t b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 template <typename K, typename V>
+__device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+{
+    if (!(vox_weight(val) > 0.0f))
+        return;
+    const float D = vox_dist(val);
+    if (!(fabsf(D) < sp.dmax))
+        return;
+    uint32_t x, y, zz;
+    unpack_key<K>(key, x, y, zz);
+    const float p[4] = { (float)x * sp.vl + sp.c0, (float)y * sp.vl + sp.c0, (float)zz * sp.vl + sp.c0, 1.0f };
+    float e[4];
+    mat_vec(w2e, p, e);
+    const float z = -e[2];
+    if (!(z > sp.znear))
+        return;
+    // multiply-only rejection well outside the image (2 % + 1 px slack, rounding-safe)
+    const float fxe = sp.fx * e[0], fye = sp.fy * e[1];
+    if (fxe > ((float)sp.W + 8.0f - sp.cx) * z * 1.02f + 1.0f || fxe < (-8.0f - sp.cx) * z * 1.02f - 1.0f ||
+        -fye > ((float)sp.H + 8.0f - sp.cy) * z * 1.02f + 1.0f || -fye < (-8.0f - sp.cy) * z * 1.02f - 1.0f)
+        return;
+    const float u = fxe / z + sp.cx;
+    const float v = sp.cy - fye / z;
+    if (!(u > -8.0f && u < (float)sp.W + 8.0f && v > -8.0f && v < (float)sp.H + 8.0f))
+        return;
+    const float kf = ceilf(sp.vl * sp.fx / z);
+    const int k = kf < 1.0f ? 1 : (kf > 3.0f ? 3 : (int)kf);
+    const float half = 0.5f * (float)(k - 1);
+    const int u0 = (int)floorf(u - half), v0 = (int)floorf(v - half);
+    const unsigned long long zk = (unsigned long long)orderable_z(z) << 32 | (uint32_t)o;
+    for (int dv = 0; dv < k; ++dv)
+        for (int du = 0; du < k; ++du) {
+            const int pu = u0 + du, pv = v0 + dv;
+            if (pu < 0 || pu >= sp.W || pv < 0 || pv >= sp.H)
+                continue;
+            unsigned long long *slot = &sp.zbuf[(size_t)pv * sp.W + pu];
+            if (zk < *slot)                 // slots only decrease: skipping is exact
+                atomicMin(slot, zk);
+        }
+}
+
+template <typename K, typename V, bool SPLAT>
 __global__ __launch_bounds__(kThreads) void k_merge_update(
     const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B,
     const uint64_t *__restrict__ part_a, const uint32_t *__restrict__ tile_base_dups,
-    const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, UpdateParams up)
+    const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, UpdateParams up,
+    SplatParams sp)
 {
     __shared__ K sA[kMgTile];
     __shared__ K sB[kMgTile];
-    __shared__ uint32_t sDup[kMgTile + 1];
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
@@ -545,58 +611,60 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads)
         sB[j] = B[mt.b0 + j];
     __syncthreads();
-    // dup flags of B, then their exclusive prefix (16 consecutive per thread)
+    // drop the new keys already in the volume (A wins ties; the A element just
+    // before the tile can match the first B element), compacting sB in place
     const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
     constexpr int PER = kMgTile / kThreads;
+    K kept[PER];
     uint32_t fl = 0, c = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t j = threadIdx.x * PER + q;
-        bool dup = false;
+        bool keep = false;
         if (j < mt.nb) {
             const K b = sB[j];
             const uint32_t lb = lds_lower_bound(sA, mt.na, b);
-            dup = (lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b);
+            keep = !((lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b));
+            kept[q] = b;
         }
-        fl |= (uint32_t)dup << q;
-        c += dup;
+        fl |= (uint32_t)keep << q;
+        c += keep;
     }
-    uint32_t dtot;
-    uint32_t run = block_exclusive_scan<kThreads>(c, tmp, dtot);
+    uint32_t nbk;
+    uint32_t pos = block_exclusive_scan<kThreads>(c, tmp, nbk);   // ends with a barrier
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        sDup[threadIdx.x * PER + q] = run;
-        run += fl >> q & 1u;
-    }
-    if (threadIdx.x == 0)
-        sDup[kMgTile] = dtot;
+    for (int q = 0; q < PER; ++q)
+        if (fl >> q & 1u)
+            sB[pos++] = kept[q];
     __syncthreads();
+    Mat4 w2e;
+    if (SPLAT)
+        w2e = *sp.w2e;
     // old voxels: keep their value (first range wins), then update
     for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads) {
         const K k = sA[i];
-        const uint32_t lb = lds_lower_bound(sB, mt.nb, k);
-        const uint64_t o = obase + i + lb - sDup[lb];
+        const uint64_t o = obase + i + lds_lower_bound(sB, nbk, k);
         V val = Av[mt.a0 + i];
         float s;
         if (o < upd_end && voxel_sample<K>(k, up, s))
             voxel_apply(val, s);
         O[o] = k;
         Ov[o] = val;
+        if (SPLAT)
+            splat_voxel<K, V>(k, val, o, sp, w2e);
     }
     // new voxels: Voxel() default, then update
-    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads) {
-        const bool dup = (j + 1 <= mt.nb) && (sDup[j + 1] != sDup[j]);
-        if (dup)
-            continue;
+    for (uint32_t j = threadIdx.x; j < nbk; j += kThreads) {
         const K k = sB[j];
-        const uint32_t lb = lds_lower_bound(sA, mt.na, k);
-        const uint64_t o = obase + (j - sDup[j]) + lb;
+        const uint64_t o = obase + j + lds_lower_bound(sA, mt.na, k);
         V val{};
         float s;
         if (o < upd_end && voxel_sample<K>(k, up, s))
             voxel_apply(val, s);
         O[o] = k;
         Ov[o] = val;
+        if (SPLAT)
+            splat_voxel<K, V>(k, val, o, sp, w2e);
     }
 }
 
@@ -631,7 +699,7 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
 
 template <typename K, typename V>
 static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                                  const float *w2e, const Mat4 *w2e_dev)
+                                  const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat)
 {
     kifi_ctx *ctx = vol->ctx;
     const int W = cam->width, H = cam->height;
@@ -735,9 +803,14 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     up.W = W; up.H = H;
     up.depth = d_depth;
     const int t_mu = timer_begin(ctx, KIFI_STAGE_MERGE_UPDATE);
-    hipLaunchKernelGGL((k_merge_update<K, V>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
-                       (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
-                       cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up);
+    if (splat)
+        hipLaunchKernelGGL((k_merge_update<K, V, true>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
+                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, *splat);
+    else
+        hipLaunchKernelGGL((k_merge_update<K, V, false>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
+                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, SplatParams{});
     timer_end(ctx, KIFI_STAGE_MERGE_UPDATE, t_mu);
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
@@ -751,16 +824,16 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
 }
 
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                               const float *w2e, const Mat4 *w2e_dev)
+                               const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat)
 {
     if (vol->desc.key_bits == 32) {
         if (vol->desc.storage == KIFI_STORAGE_F32)
-            return integrate_impl<uint32_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev);
-        return integrate_impl<uint32_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev);
+            return integrate_impl<uint32_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev, splat);
+        return integrate_impl<uint32_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev, splat);
     }
     if (vol->desc.storage == KIFI_STORAGE_F32)
-        return integrate_impl<uint64_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev);
-    return integrate_impl<uint64_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev);
+        return integrate_impl<uint64_t, VoxF32>(vol, d_depth, cam, w2e, w2e_dev, splat);
+    return integrate_impl<uint64_t, uint32_t>(vol, d_depth, cam, w2e, w2e_dev, splat);
 }
 
 } // namespace kifi

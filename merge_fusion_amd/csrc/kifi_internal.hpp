@@ -150,6 +150,14 @@ struct FrameMats {
     Mat4 e2w;   // invert_transform(worldToEye)
 };
 
+// Optional model splat folded into the merge + update voxel pass.
+struct SplatParams {
+    const Mat4 *w2e;                 // device: world->eye of this frame
+    unsigned long long *zbuf;        // W*H, cleared to ~0
+    int   W, H;
+    float fx, fy, cx, cy, vl, c0, dmax, znear;
+};
+
 // Reset list for per-frame device counters (zeroed by a fill kernel, never by a
 // memset node, so that the integration sequence can be captured in a graph).
 struct FrameCounters {

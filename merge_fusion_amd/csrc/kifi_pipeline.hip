@@ -8,7 +8,7 @@
 
 namespace kifi {
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
-                               const float *w2e, const Mat4 *w2e_dev);
+                               const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st);
 } // namespace kifi
 
@@ -300,16 +300,26 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
         if (s != KIFI_OK)
             return s;
     }
-    // Pipeline.cpp:122-127: worldToEye = invert(eyeToWorld), integrate
-    icp_w2e(&p->work, p->d_w2e, st);
-    s = integrate_dispatch(p->vol, d_depth, &p->cam, nullptr, p->d_w2e);
+    // Pipeline.cpp:122-127: worldToEye = invert(eyeToWorld), integrate; the
+    // model for the next frame (Pipeline.cpp:129-133, seen from this pose) is
+    // splatted inside the integration's voxel pass, then resolved
+    if (w2e)
+        store_w2e(w2e, p->d_w2e, st);         // the given worldToEye as is (Pipeline.cpp:116)
+    else
+        icp_w2e(&p->work, p->d_w2e, st);
+    int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    model_begin(p->model, p->vol, nullptr, p->d_w2e);
+    timer_end(ctx, KIFI_STAGE_MODEL, t_m);
+    const SplatParams sp = model_splat_params(p->model, p->d_w2e);
+    s = integrate_dispatch(p->vol, d_depth, &p->cam, nullptr, p->d_w2e, &sp);
     if (s != KIFI_OK)
         return s;
-    // Pipeline.cpp:129-133: the model for the next frame, seen from this pose
-    p->model->mp.dmax = p->vol->consts.vl;
-    s = model_predict(p->model, p->vol, nullptr, p->d_w2e);
+    t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    model_end(p->model, p->vol, false);
+    timer_end(ctx, KIFI_STAGE_MODEL, t_m);
+    KIFI_HIP(ctx, hipGetLastError());
     ++p->frame;
-    return s;
+    return KIFI_OK;
 }
 
 kifi_status kifi_pipeline_pose(kifi_pipeline *p, double e2w[16])

@@ -112,6 +112,29 @@ def test_horn_closed_form(ctx, n):
     assert np.abs(g - o).max() < 2e-5
 
 
+def test_pipeline_given_poses_matches_oracle_volume_and_model(ctx):
+    """Pipeline::Integrate with given poses (Pipeline.cpp:113-116) integrates at
+    exactly those poses; its model, splatted inside the fused voxel pass, equals
+    the oracle's stand-alone model prediction bit for bit."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+    ov = orc.Volume(1024, 4.096, 0.02)
+    for f in (0, 1, 2):
+        pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f)
+        d = mf.render(sc, CAM, pose, noise_seed=10 + f)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        pipe.integrate(d, w2e)
+        ov.integrate(d, CAM, w2e)
+    gk, gv = pipe.volume.download()
+    ok, oval = ov.read()
+    assert np.array_equal(gk.astype(np.uint64), ok)
+    assert np.array_equal(np.stack([gv["sum"], gv["weight"]], 1).view(np.uint32), oval.view(np.uint32))
+    gp, gn = pipe.model.download()
+    op, on = orc.model_predict(ov, w2e, CAM)
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+    assert np.array_equal(gn.view(np.uint32), on.view(np.uint32))
+
+
 def test_pipeline_tracks_trajectory(ctx):
     """Pipeline (Pipeline.cpp:26-56 tracking branch): ICP against the previous model."""
     sc = mf.make_scene(mf.SCENE_ROOM, 2)
