@@ -10,8 +10,10 @@
 //   - tracking: projective association on a 3-level depth pyramid, point-to-plane
 //     6x6 normal equations (or Horn's cross-covariance) accumulated in binary32 with
 //     compensated (TwoSum) reductions, solved on the device in FP64 (Cholesky /
-//     cyclic Jacobi for Horn's 4x4), pose composed in FP64.  Every iteration is two
-//     kernels and a device-side convergence flag, so a frame needs no host sync.
+//     cyclic Jacobi for Horn's 4x4), pose composed in FP64.  Every iteration is one
+//     kernel: the workgroup that finishes last reduces the partials and solves, and
+//     a device-side convergence flag turns the level's remaining launches into
+//     early exits, so a frame needs no host sync.
 #include <cfloat>
 
 #include "kifi_internal.hpp"
@@ -237,27 +239,46 @@ __device__ __forceinline__ void cadd(CSum &x, float s2, float c2)
     x.c = x.c + c2 + e;
 }
 
+// Compensated block reduction through an LDS transpose: every thread stores its
+// NV pairs as one row; kSlices x NV threads then each fold a 1/kSlices column
+// slice sequentially, and NV threads fold the slices into lds.result.  Every
+// step is parallel over the values: a butterfly of cross-lane shuffles would
+// chain 6 x 2 x NV dependent permutes per wave (about 9 us for NV = 29), and a
+// single thread folding all slices about 5 us.  Result in lds.result.
+constexpr int kSlices = 8;
 template <int NV>
-__device__ __forceinline__ void block_csum(CSum (&v)[NV], CSum *lds /* [4][NV] */)
+struct CsumLds {
+    CSum rows[kIT * NV];
+    CSum slices[kSlices * NV];
+    CSum result[NV];
+};
+
+template <int NV>
+__device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds)
 {
+    static_assert(kSlices * NV <= kIT, "one thread per (slice, value)");
+    constexpr int kRows = kIT / kSlices;
 #pragma unroll
     for (int j = 0; j < NV; ++j)
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const float s2 = __shfl_xor(v[j].s, o, kWave), c2 = __shfl_xor(v[j].c, o, kWave);
-            cadd(v[j], s2, c2);
-        }
-    const int w = threadIdx.x / kWave;
-    if (__lane_id() == 0)
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-            lds[w * NV + j] = v[j];
+        lds.rows[threadIdx.x * NV + j] = v[j];
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int q = 1; q < kIT / kWave; ++q)
+    if (threadIdx.x < kSlices * NV) {
+        const int j = threadIdx.x % NV, sl = threadIdx.x / NV;
+        CSum a = lds.rows[(sl * kRows) * NV + j];
+        for (int r = 1; r < kRows; ++r) {
+            const CSum b = lds.rows[(sl * kRows + r) * NV + j];
+            cadd(a, b.s, b.c);
+        }
+        lds.slices[sl * NV + j] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        CSum a = lds.slices[threadIdx.x];
 #pragma unroll
-            for (int j = 0; j < NV; ++j)
-                cadd(v[j], lds[q * NV + j].s, lds[q * NV + j].c);
+        for (int sl = 1; sl < kSlices; ++sl)
+            cadd(a, lds.slices[sl * NV + threadIdx.x].s, lds.slices[sl * NV + threadIdx.x].c);
+        lds.result[threadIdx.x] = a;
+    }
     __syncthreads();
 }
 
@@ -271,29 +292,33 @@ template <> struct IcpNV<KIFI_ICP_HORN> { static constexpr int nv = 17; };
 
 template <int MODE>
 __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
-                           const ModelPose *mpose, CSum *lds);
+                           const ModelPose *mpose, CsumLds<IcpNV<MODE>::nv> &lds);
 
 // Block partial -> device memory; the workgroup that takes the last ticket
 // reduces all partials and runs the FP64 solve (no spin-wait anywhere).
+// Ordering: each workgroup's partial store is released at agent scope before
+// its ticket increment (gfx950: L2 write-back, no invalidate); only the last
+// workgroup acquires (one L2 invalidate per launch).  A full __threadfence()
+// here would invalidate the XCD's L2 once per workgroup and make every other
+// workgroup re-fetch the model maps from HBM.
 template <int MODE, int NV>
-__device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CSum *lds, CSum *partials, IcpState *st, int level,
+__device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, CSum *partials, IcpState *st, int level,
                                             const IcpSolveParams &sp, const ModelPose *mpose)
 {
     __shared__ bool last;
     block_csum<NV>(acc, lds);
+    if (threadIdx.x < NV)
+        partials[(size_t)blockIdx.x * NV + threadIdx.x] = lds.result[threadIdx.x];
+    __syncthreads();     // the stores of threads 0..NV-1 precede thread 0's release
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-            partials[(size_t)blockIdx.x * NV + j] = acc[j];
-        __threadfence();
-        const unsigned t = atomicAdd(&st->ticket, 1u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned t = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = t == gridDim.x - 1;
     }
     __syncthreads();
     if (!last)
         return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);       // device scope: see every workgroup's partial
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // see every workgroup's partial
     icp_finish<MODE>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
     if (threadIdx.x == 0)
         st->ticket = 0;
@@ -308,7 +333,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
 {
     constexpr int NV = IcpNV<MODE>::nv;
     constexpr int MODE_OF_KERNEL = MODE;
-    __shared__ CSum lds[(kIT / kWave) * NV];
+    __shared__ CsumLds<NV> lds;
     if (st->level_done)
         return;
     CSum acc[NV];
@@ -391,7 +416,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
 {
     constexpr int NV = IcpNV<KIFI_ICP_HORN>::nv;
     constexpr int MODE_OF_KERNEL = KIFI_ICP_HORN;
-    __shared__ CSum lds[(kIT / kWave) * NV];
+    __shared__ CsumLds<NV> lds;
     if (st->level_done)
         return;
     CSum acc[NV];
@@ -619,7 +644,7 @@ __device__ void horn_increment(const double S[9], const double ms[3], const doub
 // other workgroups are read with device-scope loads after the ticket.
 template <int MODE>
 __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
-                           const ModelPose *mpose, CSum *lds)
+                           const ModelPose *mpose, CsumLds<IcpNV<MODE>::nv> &lds)
 {
     constexpr int NV = IcpNV<MODE>::nv;
     CSum acc[NV];
@@ -639,7 +664,7 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
         return;
     double v[NV];
     for (int j = 0; j < NV; ++j)
-        v[j] = (double)acc[j].s + (double)acc[j].c;
+        v[j] = (double)lds.result[j].s + (double)lds.result[j].c;
     const double cnt = v[NV - 2], e2 = v[NV - 1];
     st->rms = cnt > 0.0 ? (float)sqrt(e2 / cnt) : 0.0f;
     st->inliers = (int)cnt;
@@ -1041,7 +1066,7 @@ void icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st)
 __global__ __launch_bounds__(kIT) void k_horn_sums(const float *__restrict__ src, const float *__restrict__ dst,
                                                    uint32_t n, const double *__restrict__ means, CSum *__restrict__ out)
 {
-    __shared__ CSum lds[(kIT / kWave) * 9];
+    __shared__ CsumLds<9> lds;
     CSum acc[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j)
@@ -1070,15 +1095,14 @@ __global__ __launch_bounds__(kIT) void k_horn_sums(const float *__restrict__ src
         }
     }
     block_csum<9>(acc, lds);
-    if (threadIdx.x == 0)
-        for (int j = 0; j < 9; ++j)
-            out[(size_t)blockIdx.x * 9 + j] = acc[j];
+    if (threadIdx.x < 9)
+        out[(size_t)blockIdx.x * 9 + threadIdx.x] = lds.result[threadIdx.x];
 }
 
 __global__ __launch_bounds__(kIT) void k_horn_finish(const CSum *__restrict__ part, int nb, uint32_t n, int phase,
                                                      double *__restrict__ means, float *__restrict__ out)
 {
-    __shared__ CSum lds[(kIT / kWave) * 9];
+    __shared__ CsumLds<9> lds;
     CSum acc[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j)
@@ -1091,7 +1115,7 @@ __global__ __launch_bounds__(kIT) void k_horn_finish(const CSum *__restrict__ pa
         return;
     double v[9];
     for (int j = 0; j < 9; ++j)
-        v[j] = (double)acc[j].s + (double)acc[j].c;
+        v[j] = (double)lds.result[j].s + (double)lds.result[j].c;
     if (phase == 0) {
         // centroids (ICP.cpp:166-167), rounded to binary32 as the reference keeps them
         for (int j = 0; j < 6; ++j)
