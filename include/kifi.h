@@ -164,6 +164,17 @@ kifi_status kifi_model_download(kifi_model *m, float *points, float *normals);
 kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
                            const double guess_eyeToWorld[16], const kifi_icp_params *params,
                            double out_eyeToWorld[16], kifi_icp_stats *stats);
+/* ICP::Align itself (ICP.h:18-27, ICP.cpp:9-265): `iterations` window-search
+ * Horn steps (the reference runs 4) of the raw frame (host, mm) against a host
+ * point buffer synth_xyz (synth_width x synth_height float3 world points, all
+ * zero = empty) rendered at synthEyeToWorld; poses are eye->world, column
+ * storage.  The 21 x 21 window clamps to the buffer (the reference clamps to
+ * 640 x 480).  With no correspondence the guess is returned and stats->failed
+ * is set (the reference divides by zero).  Synchronous. */
+kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                                  const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
+                                  const float synthEyeToWorld[16], int iterations, float out[16],
+                                  kifi_icp_stats *stats);
 /* Horn's closed-form rigid alignment of n correspondences (ICP.cpp:166-264):
  * compensated centroid and cross-covariance reductions, FP64 4x4 eigen solve.
  * out = the transform mapping src onto dst.  Synchronous. */

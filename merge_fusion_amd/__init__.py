@@ -35,7 +35,7 @@ HIP_EXPORTS = [
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_memcpy_to_device",
-    "kifi_device_copy", "kifi_integrate_totals",
+    "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
 ]
 STAGE_MERGE_UPDATE, STAGE_SORT, STAGE_ICP, STAGE_MODEL, STAGE_MESH = range(5)
 ICP_POINT_TO_PLANE, ICP_HORN = 0, 1
@@ -154,6 +154,8 @@ def load_hip():
             "kifi_memcpy_to_device": ([vp, vp, vp, sz], i32),
             "kifi_device_copy": ([vp, vp, vp, sz], i32),
             "kifi_integrate_totals": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+            "kifi_icp_align_points": ([vp, vp, ctypes.POINTER(Intrinsics), f32p, f32p, i32, i32, f32p, i32, f32p,
+                                       ctypes.POINTER(IcpStats)], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(lib, name)
@@ -501,6 +503,22 @@ def horn_align(ctx, src, dst):
     out = np.zeros(16, np.float32)
     ctx.check(ctx._lib.kifi_horn_align(ctx.handle, _f32p(s), _f32p(d), s.shape[0], _f32p(out)), "kifi_horn_align")
     return out
+
+
+def icp_align_points(ctx, depth_mm, cam, guess_e2w, synth_xyz, synth_e2w, iterations=4):
+    """ICP::Align (ICP.cpp:9-265) against a host point buffer: (pose float32[16], stats)."""
+    d = np.ascontiguousarray(depth_mm, np.uint16)
+    if d.shape != (cam.height, cam.width):
+        raise KifiError("depth frame must be (height, width) of the camera")
+    syn = np.ascontiguousarray(np.asarray(synth_xyz, np.float32)[..., :3])
+    g = np.ascontiguousarray(guess_e2w, np.float32).reshape(16)
+    se = np.ascontiguousarray(synth_e2w, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    st = IcpStats()
+    ctx.check(ctx._lib.kifi_icp_align_points(ctx.handle, d.ctypes.data, ctypes.byref(cam), _f32p(g), _f32p(syn),
+                                             syn.shape[1], syn.shape[0], _f32p(se), iterations, _f32p(out),
+                                             ctypes.byref(st)), "kifi_icp_align_points")
+    return out, st.as_dict()
 
 
 class _BorrowedVolume(Volume):

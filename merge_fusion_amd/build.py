@@ -70,13 +70,23 @@ def build_synth():
     return out
 
 
+def build_cpp_tests():
+    """The C++ drop-in test program (host code only: g++ against the ABI)."""
+    src = os.path.join(PKG, "cpp", "tests", "test_dropin.cpp")
+    out = os.path.join(PKG, "cpp", "tests", "test_dropin")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wno-unused-result", "-I", os.path.join(PKG, "cpp"), "-I", INCLUDE,
+          src, "-o", out, "-L", PKG, "-lkifi_hip", "-lkifi_synth", "-Wl,-rpath,$ORIGIN/../..",
+          "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"])
+    return out
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
 def build_all(oracle=True):
-    outs = [build_synth(), build_hip()]
+    outs = [build_synth(), build_hip(), build_cpp_tests()]
     if oracle:
         outs.append(build_oracle())
     return outs

@@ -2,7 +2,7 @@
 //
 // Reference: ICP::Align / AlignStep (KinectFission/kifi/ICP.cpp:9-265) runs 4 fixed
 // Horn point-to-point steps over a 21x21 window search, against a model point
-// buffer from Mesher::Mesh + Renderer::Bin (Pipeline.cpp:36-55).  Here:
+// buffer from Mesher::Mesh + Renderer::Bin (Pipeline.cpp:48-52).  Here:
 //   - model prediction: one pass over the volume splats every near-surface voxel
 //     (|D| < dmax) at its estimated surface depth along the viewing ray into a
 //     z-buffer (64-bit atomicMin of depth | voxel index), then resolves points and
@@ -15,6 +15,7 @@
 //     a device-side convergence flag turns the level's remaining launches into
 //     early exits, so a frame needs no host sync.
 #include <cfloat>
+#include <vector>
 
 #include "kifi_internal.hpp"
 #include "kifi_prims.hpp"
@@ -814,7 +815,7 @@ __global__ void k_icp_set_pose_w2e(IcpState *__restrict__ st, Mat4 w2e)
     }
 }
 
-// world->eye (float) of the tracked pose, as Pipeline.cpp:122-123 derives it
+// world->eye (float) of the tracked pose, as Pipeline.cpp:41-42 derives it
 __global__ void k_icp_w2e(const IcpState *__restrict__ st, Mat4 *__restrict__ w2e)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
@@ -903,6 +904,31 @@ kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host
     model_end(m, vol, true);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
     KIFI_HIP(ctx, hipGetLastError());
+    return KIFI_OK;
+}
+
+// A caller's point buffer as the model (ICP::Align's synthDepthBuffer and
+// synthEyeToWorld, ICP.cpp:38-48): W*H float3 world points, all-zero = empty
+// (ICP.cpp:112-113).  The association projection is EyeToClipRH of the raw
+// camera times the buffer's worldToEye (ICP.cpp:48, 81), as in the reference.
+kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, const kifi_intrinsics *proj_cam)
+{
+    kifi_ctx *ctx = m->ctx;
+    const size_t P = (size_t)m->mp.W * m->mp.H;
+    std::vector<float4> tmp(P);
+    for (size_t i = 0; i < P; ++i) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        tmp[i] = make_float4(x, y, z, (x == 0.0f && y == 0.0f && z == 0.0f) ? 0.0f : 1.0f);
+    }
+    KIFI_HIP(ctx, hipMemcpyAsync(m->d_pts, tmp.data(), P * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    Mat4 wv, clip;
+    for (int i = 0; i < 16; ++i)
+        wv.m[i] = w2e[i];
+    eye_to_clip_rh(proj_cam->width, proj_cam->height, proj_cam->fx, proj_cam->fy, proj_cam->cx, proj_cam->cy,
+                   proj_cam->znear, proj_cam->zfar, clip.m);
+    hipLaunchKernelGGL(k_model_pose, dim3(1), dim3(64), 0, ctx->stream, wv, (const Mat4 *)nullptr, clip, m->d_pose);
+    KIFI_HIP(ctx, hipGetLastError());
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));     // tmp is released on return
     return KIFI_OK;
 }
 

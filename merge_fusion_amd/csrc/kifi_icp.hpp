@@ -85,6 +85,7 @@ kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host
 void        model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
 SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev);
 void        model_end(kifi_model *m, kifi_volume *vol, bool splat);
+kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, const kifi_intrinsics *proj_cam);
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 void        icp_free(IcpWork *w);
 kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,

@@ -186,6 +186,89 @@ kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_i
     return s;
 }
 
+kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                                  const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
+                                  const float synthEyeToWorld[16], int iterations, float out[16],
+                                  kifi_icp_stats *stats)
+{
+    if (!ctx || !depth_mm || !guess || !synth_xyz || !synthEyeToWorld || !out || iterations < 0)
+        return KIFI_ERR_ARG;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    if (synth_width <= 0 || synth_height <= 0)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_icp_align_points: empty point buffer");
+    // the model is the caller's buffer: its size, projected with the raw camera
+    kifi_intrinsics mcam = *cam;
+    mcam.width = synth_width;
+    mcam.height = synth_height;
+    kifi_model m;
+    m.ctx = ctx;
+    m.cam = mcam;
+    m.mp.W = synth_width;
+    m.mp.H = synth_height;
+    m.mp.fx = cam->fx;
+    m.mp.fy = cam->fy;
+    m.mp.cx = cam->cx;
+    m.mp.cy = cam->cy;
+    IcpWork w;
+    uint16_t *d_depth = nullptr;
+    const size_t P = (size_t)cam->width * cam->height;
+    s = model_alloc(&m);
+    if (s == KIFI_OK) {
+        float w2e[16];
+        for (int i = 0; i < 16; ++i)
+            w2e[i] = synthEyeToWorld[i];
+        invert_transform(w2e);                       // dstWorldToEye (ICP.cpp:48)
+        s = model_set_points(&m, synth_xyz, w2e, cam);
+    }
+    if (s == KIFI_OK)
+        s = icp_alloc(&w, ctx, cam->width, cam->height);
+    if (s == KIFI_OK) {
+        hipError_t e = hipMalloc(&d_depth, P * 2);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess)
+            s = hip_error(ctx, e, "kifi_icp_align_points upload");
+    }
+    IcpState hs;
+    if (s == KIFI_OK) {
+        // ICP::Align (ICP.cpp:9-31): `iterations` Horn steps, no convergence test
+        kifi_icp_params ip;
+        kifi_icp_params_default(&ip);
+        ip.mode = KIFI_ICP_HORN;
+        ip.levels = 1;
+        ip.iters[0] = iterations;
+        ip.eps_rot = 0.0f;
+        ip.eps_trans = 0.0f;
+        ip.min_inliers = 1;
+        double g[16];
+        for (int i = 0; i < 16; ++i)
+            g[i] = guess[i];
+        icp_set_pose(&w, g, nullptr, ctx->stream);
+        s = icp_run(&m, &w, d_depth, cam, g, nullptr, &ip);
+        if (s == KIFI_OK) {
+            hipError_t e = hipMemcpyAsync(&hs, w.d_state, sizeof hs, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess)
+                s = hip_error(ctx, e, "kifi_icp_align_points");
+        }
+    }
+    if (s == KIFI_OK) {
+        for (int i = 0; i < 16; ++i)
+            out[i] = (float)hs.pose[i];
+        if (stats)
+            fill_stats(hs, stats);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (d_depth)
+        (void)hipFree(d_depth);
+    icp_free(&w);
+    model_free(&m);
+    return s;
+}
+
 kifi_status kifi_horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16])
 {
     if (!ctx || !src || !dst || !out || n < 3 || n >= 0xffffffffull)
@@ -229,7 +312,9 @@ kifi_status kifi_pipeline_create(kifi_ctx *ctx, const kifi_volume_desc *vd, cons
         kifi_pipeline_destroy(p);
         return s;
     }
-    // Pipeline.cpp:7-21: the first frame is integrated at the identity pose
+    // the first tracked frame is integrated at the start pose; the reference
+    // starts at identity moved 2 m along z (Pipeline.cpp:19-21), kifi_pipeline_set_pose
+    // sets any other
     const double I[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
     icp_set_pose(&p->work, I, nullptr, ctx->stream);
     *out = p;
@@ -285,10 +370,10 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
     }
     kifi_status s = KIFI_OK;
     if (w2e) {
-        // Pipeline.cpp:113-116: integrate at the given pose
+        // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
     } else if (p->frame > 0) {
-        // Pipeline.cpp:118-120: track against the model of the previous frame
+        // Pipeline.cpp:37-39: track against the model of the previous frame
         kifi_icp_params ip = p->params;
         if (!p->have_guess) {
             ip.fallback_rms = 0.0f;   // the guess is the previous pose: nothing to fall back to
@@ -300,11 +385,11 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
         if (s != KIFI_OK)
             return s;
     }
-    // Pipeline.cpp:122-127: worldToEye = invert(eyeToWorld), integrate; the
-    // model for the next frame (Pipeline.cpp:129-133, seen from this pose) is
+    // Pipeline.cpp:41-46: worldToEye = invert(eyeToWorld), integrate; the
+    // model for the next frame (Pipeline.cpp:48-52, seen from this pose) is
     // splatted inside the integration's voxel pass, then resolved
     if (w2e)
-        store_w2e(w2e, p->d_w2e, st);         // the given worldToEye as is (Pipeline.cpp:116)
+        store_w2e(w2e, p->d_w2e, st);         // the given worldToEye as is (Pipeline.cpp:35)
     else
         icp_w2e(&p->work, p->d_w2e, st);
     int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);

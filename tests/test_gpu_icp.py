@@ -113,7 +113,7 @@ def test_horn_closed_form(ctx, n):
 
 
 def test_pipeline_given_poses_matches_oracle_volume_and_model(ctx):
-    """Pipeline::Integrate with given poses (Pipeline.cpp:113-116) integrates at
+    """Pipeline::Integrate with given poses (Pipeline.cpp:32-35) integrates at
     exactly those poses; its model, splatted inside the fused voxel pass, equals
     the oracle's stand-alone model prediction bit for bit."""
     sc = mf.make_scene(mf.SCENE_ROOM, 2)
@@ -136,7 +136,7 @@ def test_pipeline_given_poses_matches_oracle_volume_and_model(ctx):
 
 
 def test_pipeline_tracks_trajectory(ctx):
-    """Pipeline (Pipeline.cpp:26-56 tracking branch): ICP against the previous model."""
+    """Pipeline (Pipeline.cpp:36-55 tracking branch): ICP against the previous model."""
     sc = mf.make_scene(mf.SCENE_ROOM, 2)
     pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
     p0 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 0)
@@ -152,3 +152,26 @@ def test_pipeline_tracks_trajectory(ctx):
     st = pipe.stats()
     assert dt < 0.02 and da < 0.5, (dt, da, st)
     assert st["frames"] == 7 and 7 <= st["sum_iters"] <= 7 * 19
+
+
+def test_icp_align_points_is_reference_align(ctx):
+    """kifi_icp_align_points is ICP::Align (ICP.cpp:9-265) on the caller's point
+    buffer, fed by the reference's own chain (Pipeline.cpp:47-52): mesh vertices
+    binned into a 1024 x 768 buffer with EyeToClipRH() * worldToEye; 4 Horn steps
+    against the binary32 restatement, tolerance 1 mm / 0.1 deg (reduction order)."""
+    sc, gv, ov = _room(ctx, [0, 1, 2])
+    p2 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)
+    w2e = mf.invert_rigid(p2).astype(np.float32)
+    verts = gv.mesh()
+    synth = ctx.bin(verts, orc.mat_mul(orc.eye_to_clip(CAM), w2e), 1024, 768)
+    assert (np.abs(synth).sum(-1) > 0).mean() > 0.3
+    pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 3)
+    d = mf.render(sc, CAM, pt, noise_seed=11)
+    e2w = p2.astype(np.float32)
+    gpose, st = mf.icp_align_points(ctx, d, CAM, e2w, synth, e2w, iterations=4)
+    assert st["iterations"] == 4 and st["inliers"] > 10000 and not st["failed"]
+    opose, n = orc.icp_align_ref(d, CAM, synth, e2w, e2w)
+    dt, da = mf.pose_error(gpose.astype(np.float64), opose.astype(np.float64))
+    assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da)
+    # and it moves the guess towards the true pose
+    assert mf.pose_error(gpose.astype(np.float64), pt)[0] < mf.pose_error(p2, pt)[0]
