@@ -1,5 +1,5 @@
 // test_dropin.cpp -- the reference's class API (KinectFission/kifi/*.h) used the
-// way its own callers use it (UnitTests/Integrator.cpp:15-60, Pipeline.cpp:26-68),
+// way its own callers use it (UnitTests/Integrator.cpp:15-55, Pipeline.cpp:26-68),
 // compiled as ordinary C++ against kifi/kifi.hpp and libkifi_hip.so.
 //
 // Writes what each call produced into the output directory given as argv[1]
