@@ -16,7 +16,10 @@ host and copied to HBM before timing; the timed region covers frames
 
 Prints one JSON line.  The CPU baseline is the single-threaded oracle
 (integration, model prediction, FP64 ICP) on a bounded sample of the same
-trajectory, timed on this host.
+trajectory, timed on this host.  "fps_with_h2d" repeats the timed frames handed
+over from host memory.  "learned_vs_plain" trains the twist-prediction network
+on this GPU (seeded pairs, bounded budget) and compares plain and learned
+initial poses on a 1000-frame fast trajectory rendered on the device.
 """
 import argparse
 import json
@@ -46,6 +49,10 @@ def parse():
     ap.add_argument("--mesh-every", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-initialisation comparison")
+    ap.add_argument("--fast-frames", type=int, default=1000, help="frames of the fast trajectory")
+    ap.add_argument("--train-pairs", type=int, default=60000)
+    ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain copy kernel of this size 20 times (HBM counter calibration)")
     return ap.parse_args()
@@ -89,6 +96,116 @@ def cpu_baseline(mf, cam, frames, poses, budget_s):
                   "1..%d of the same trajectory (volume 0 -> %d voxels), median frame time %.3f s; host %s, "
                   "nproc %d" % (f - 1, vol.size(), med, platform.processor() or platform.machine(),
                                 os.cpu_count() or 0),
+    }
+
+
+def twistnet_flops_per_sample():
+    """Multiply-adds x 2 of one forward pass (3x3 stride-2 convolutions, two FC layers)."""
+    ch, h, w, fl = [2, 16, 32, 64, 64, 128], 106, 128, 0
+    for l in range(5):
+        h, w = (h - 1) // 2 + 1, (w - 1) // 2 + 1
+        fl += 2 * h * w * ch[l + 1] * ch[l] * 9
+    return fl + 2 * (2048 * 256 + 256 * 6)
+
+
+def learned_section(mf, ctx, cam, args):
+    """Learned vs plain initial pose on a fast trajectory (steps U[2,10] cm and
+    U[2,15] deg): the network is trained here, on this GPU, by the library's own
+    forward/backward kernels (seeded, bounded budget), then both pipelines run
+    the same device-resident frames."""
+    scenes = [mf.make_scene(mf.SCENE_ROOM, 100 + k) for k in range(64)]
+    t0 = time.perf_counter()
+    pairs = mf.PairSet(ctx, cam, scenes, args.train_pairs, seed=7)
+    ctx.synchronize()
+    t_gen = time.perf_counter() - t0
+    net = mf.TwistNet(ctx, 1)
+    spe = args.train_pairs // 256
+    losses = []
+    t0 = time.perf_counter()
+    for ep in range(args.train_epochs):
+        lr = 1e-3 * 0.5 * (1.0 + np.cos(np.pi * ep / args.train_epochs))
+        losses.append(net.train(pairs, spe, 256, lr, seed=100 + ep))
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    pairs.close()
+    hold = mf.PairSet(ctx, cam, [mf.make_scene(mf.SCENE_ROOM, 500 + k) for k in range(8)], 1024, seed=2)
+    _, _, truth, valid = hold.download(inputs=False)
+    pred = net.predict(hold)
+    hold.close()
+    rot_err = np.degrees(np.linalg.norm(pred[valid, :3] - truth[valid, :3], axis=1))
+    tr_err = np.linalg.norm(pred[valid, 3:] - truth[valid, 3:], axis=1)
+    samples = spe * 256 * args.train_epochs
+    train_tflops = 3.0 * twistnet_flops_per_sample() * samples / t_train / 1e12
+
+    nf = args.fast_frames
+    scene = mf.make_scene(mf.SCENE_ROOM, SEED)
+    poses = [mf.trajectory_pose(mf.TRAJ_FAST, 3, f) for f in range(nf)]
+    fbytes = W * H * 2
+    buf = ctx.device_upload(np.zeros(nf * W * H, np.uint16))
+    for f in range(nf):
+        mf.render_device(ctx, scene, cam, poses[f], 5000 + f, out_ptr=buf.address(f * fbytes))
+    ctx.synchronize()
+
+    def run(storage, use_net, track):
+        prm = mf.IcpParams.default()
+        if use_net:
+            prm.fallback_rms = 0.005      # 5 mm (1 m-scaled residual)
+            prm.fallback_iters = 20       # inert with the 10/5/4 budget (19 iterations)
+        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=storage, params=prm)
+        if use_net:
+            pipe.set_twistnet(net)
+        errs = []
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for f in range(nf):
+            w2e = mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None
+            pipe.integrate_device(buf.address(f * fbytes), w2e)
+            if track:
+                errs.append(mf.pose_error(pipe.pose(), poses[f])[0])
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        st = pipe.stats()
+        pipe.close()
+        return dt, st, np.array(errs)
+
+    res = {}
+    for name, storage, use_net in (("plain_f32", mf.STORAGE_F32, False), ("plain_f16", mf.STORAGE_F16U8, False),
+                                   ("learned_f16", mf.STORAGE_F16U8, True)):
+        dt, st, _ = run(storage, use_net, False)
+        _, _, e = run(storage, use_net, True)
+        lost = int(np.argmax(e > 0.10)) if (e > 0.10).any() else None
+        res[name] = {
+            "frames_per_s": round((nf - 1) / dt, 2),
+            "icp_iters_per_frame": round(st["sum_iters"] / max(st["frames"], 1), 3),
+            "fallback_rate": round(st["fallbacks"] / max(st["frames"], 1), 4),
+            "ate_m": round(float(np.sqrt(np.mean(e ** 2))), 4),
+            "first_frame_error_over_10cm": lost,
+        }
+    buf.free()
+    net.close()
+    base = res["plain_f32"]
+    return {
+        "workload": "fast trajectory (steps U[2,10] cm, U[2,15] deg), %d frames of the room scene, 512x424, "
+                    "4 mm voxels, 3-level point-to-plane ICP 10/5/4; learned: CNN twist as ICP start, plain ICP "
+                    "from the previous pose as fallback when the 1 m-scaled residual > 5 mm or 20 iterations "
+                    "are reached (the 10/5/4 budget is 19)" % nf,
+        "training": {
+            "pairs": args.train_pairs, "epochs": args.train_epochs, "batch": 256, "optimizer": "adam, cosine lr 1e-3",
+            "pair_generation_s": round(t_gen, 2), "train_s": round(t_train, 2),
+            "samples_per_s": round(samples / t_train, 1), "tflops": round(train_tflops, 3),
+            "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": round(train_tflops / 2500.0, 6),
+            "loss_first_last": [round(float(losses[0][:20].mean()), 4), round(float(losses[-1][-20:].mean()), 4)],
+            "heldout_rot_err_deg_mean": round(float(rot_err.mean()), 3),
+            "heldout_trans_err_cm_mean": round(float(tr_err.mean() * 100), 3),
+            "heldout_prev_pose_rot_err_deg_mean": round(float(np.degrees(np.linalg.norm(truth[valid, :3],
+                                                                                          axis=1)).mean()), 3),
+            "heldout_prev_pose_trans_err_cm_mean": round(float(np.linalg.norm(truth[valid, 3:],
+                                                                              axis=1).mean() * 100), 3),
+        },
+        "results": res,
+        "fps_ratio_learned_f16_vs_plain_f32": round(res["learned_f16"]["frames_per_s"] / base["frames_per_s"], 3),
+        "icp_iters_reduction_vs_plain": round(1.0 - res["learned_f16"]["icp_iters_per_frame"] /
+                                              max(base["icp_iters_per_frame"], 1e-9), 4),
     }
 
 
@@ -192,6 +309,23 @@ def main():
             traffic_note = "measured traffic/algorithmic ratio %.3f (%s, %s) x this run's algorithmic bytes" % (
                 pmc["traffic_over_alg"], pmc.get("launches"), pmc.get("source"))
 
+    vol_final = int(pipe.volume.size())
+    pipe.close()
+
+    # the same frames handed over from host memory (PCIe copy inside each step)
+    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
+    for f in range(args.warmup):
+        pipe.integrate(frames[f], mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for f in range(args.warmup, n):
+        pipe.integrate(frames[f])
+    ctx.synchronize()
+    fps_h2d = args.steps / (time.perf_counter() - t0)
+    pipe.close()
+
+    learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
+
     icp_frames = st1["frames"] - st0["frames"]
     icp_iters = (st1["sum_iters"] - st0["sum_iters"]) / max(icp_frames, 1)
     out = {
@@ -233,10 +367,11 @@ def main():
         "icp_fallback_rate": round((st1["fallbacks"] - st0["fallbacks"]) / max(icp_frames, 1), 4),
         "pose_error_m_deg": [[f, round(e[0], 5), round(e[1], 4)] for f, e in pose_err] +
                             [[n - 1, round(final_err[0], 5), round(final_err[1], 4)]],
-        "volume_voxels_final": int(pipe.volume.size()),
+        "volume_voxels_final": vol_final,
         "mesh_vertices": mesh_counts,
         "stage_ms_per_frame": {k: round(v[0] / max(args.steps, 1), 4) for k, v in stage.items()},
-        "fps_with_h2d": None,
+        "fps_with_h2d": round(fps_h2d, 3),
+        "learned_vs_plain": learned,
     }
     print(json.dumps(out))
 

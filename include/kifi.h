@@ -130,12 +130,17 @@ typedef struct {
     int   mode;            /* KIFI_ICP_POINT_TO_PLANE or KIFI_ICP_HORN                 */
     int   levels;          /* pyramid levels, 1..3 (level 0 = full resolution)         */
     int   iters[3];        /* maximum iterations per level, index = level              */
-    float dist_thresh;     /* correspondence gate, metres                              */
+    float dist_thresh;     /* correspondence gate at full resolution, metres; level l  */
+                           /* of the pyramid gates at dist_thresh x 2^l                */
     float eps_rot;         /* convergence: |rotation increment| < eps_rot (rad) ...    */
     float eps_trans;       /* ... and |translation increment| < eps_trans (m)          */
     int   min_inliers;     /* fewer correspondences: stop (tracking failure)           */
     float fallback_rms;    /* > 0: re-run plain ICP from the previous pose when the    */
-    int   fallback_iters;  /*       final RMS exceeds it or iterations reach the cap   */
+    int   fallback_iters;  /*       first pass failed, its rms_1m exceeds fallback_rms */
+                           /*       or it used fallback_iters iterations (> 0); the    */
+                           /*       fallback result replaces the first only if that    */
+                           /*       failed or broke fallback_rms and the fallback's    */
+                           /*       rms_1m is lower                                    */
 } kifi_icp_params;
 
 typedef struct {
@@ -148,6 +153,8 @@ typedef struct {
     long long sum_iters;   /* running totals over tracked frames                       */
     long long frames;
     long long fallbacks;
+    float rms_1m;          /* RMS of the point-to-plane residual scaled to 1 m depth   */
+                           /* by the sensor noise model (r / d^2 x 1 m^2), metres      */
 } kifi_icp_stats;
 
 kifi_status kifi_icp_params_default(kifi_icp_params *p);
@@ -200,6 +207,64 @@ kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double eyeToWorld[16
 kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double eyeToWorld[16]);
 kifi_status kifi_pipeline_pose(kifi_pipeline *p, double eyeToWorld[16]);     /* EyeToWorld(), sync */
 kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats);     /* sync */
+
+/* ---- learned initial pose (no reference counterpart: the reference starts ICP
+ * from the previous pose, Pipeline.cpp:37-39) ----
+ * A small CNN predicts the rigid twist between the previous and the current
+ * frame; its pose (previous * exp(twist)) starts ICP, and plain ICP from the
+ * previous pose is the fallback when the final rms_1m exceeds fallback_rms or the
+ * iterations reach fallback_iters (kifi_icp_params); ICP judges the two starts.  Input: both frames as
+ * 128 x 106 4x4-block mean depth / 4 m (bfloat16); five 3x3 stride-2
+ * convolutions (16-32-64-64-128, ReLU), FC 2048-256 (ReLU)-6.  BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, FP32 master weights, Adam.
+ * Twists are (omega [rad], v [m]) with T = [R(omega) | v], eye(t) -> eye(t-1);
+ * the network's raw outputs are twist / (0.15, 0.15, 0.15, 0.06, 0.06, 0.06). */
+typedef struct kifi_twistnet kifi_twistnet;
+typedef struct kifi_pairset kifi_pairset;
+struct kifi_scene_s;
+
+size_t      kifi_twistnet_param_count(void);
+/* seeded He-normal initialisation */
+kifi_status kifi_twistnet_create(kifi_ctx *ctx, uint64_t seed, kifi_twistnet **out);
+void        kifi_twistnet_destroy(kifi_twistnet *net);
+/* FP32 master weights in layer order (conv1 W [16][2][3][3], conv1 b, ..., fc2 W [6][256], fc2 b).
+ * set also resets the Adam state. */
+kifi_status kifi_twistnet_get_params(kifi_twistnet *net, float *out);
+kifi_status kifi_twistnet_set_params(kifi_twistnet *net, const float *in);
+/* B <= 256 host inputs (B x 2 x 106 x 128, rounded to bfloat16) -> B x 6 raw outputs */
+kifi_status kifi_twistnet_forward(kifi_twistnet *net, const float *x, int B, float *y);
+/* loss 0.5/B sum (y - t)^2 against raw targets t, and its gradient (param order) */
+kifi_status kifi_twistnet_loss_grad(kifi_twistnet *net, const float *x, const float *t, int B, float *loss,
+                                    float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `steps` batches drawn without
+ * replacement from the valid pairs of `set` (seeded); losses[steps] may be NULL */
+kifi_status kifi_twistnet_train(kifi_twistnet *net, kifi_pairset *set, int steps, int batch, float lr,
+                                uint64_t seed, float *losses);
+/* twists (physical units) predicted for pairs [first, first + n) of a set */
+kifi_status kifi_twistnet_predict_set(kifi_twistnet *net, kifi_pairset *set, int first, int n, float *twists);
+
+/* Training pairs generated on the device: per pair a random pose in the room
+ * (within 1 m of its centre), a relative motion of random axis and direction,
+ * angle U[0, max_rot_deg], length U[0, max_trans]; both frames rendered with
+ * the sensor noise model; the target is the twist that point-to-plane ICP of
+ * the second frame against the first frame's surface converges to from the
+ * true pose.  A pair is valid when ICP kept >2000 inliers and stayed within
+ * 2 deg / 2 cm of the truth.  scenes: kifi_scene[n_scenes] (kifi_synth.h),
+ * pair i uses scenes[i % n_scenes]. */
+kifi_status kifi_pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, const struct kifi_scene_s *scenes,
+                                  int n_scenes, int n, uint64_t seed, float max_trans, float max_rot_deg,
+                                  kifi_pairset **out);
+void        kifi_pairset_destroy(kifi_pairset *set);
+int         kifi_pairset_size(const kifi_pairset *set);
+/* inputs n x 2 x 106 x 128, raw targets n x 6, true twists n x 6, valid n; any may be NULL */
+kifi_status kifi_pairset_download(kifi_pairset *set, float *inputs, float *targets, float *truth, int *valid);
+/* kifi_scene_render (kifi_synth.h) on the device; out is a host or (out_on_device) device buffer */
+kifi_status kifi_render(kifi_ctx *ctx, const struct kifi_scene_s *scene, const kifi_intrinsics *cam,
+                        const double eyeToWorld[16], uint64_t noise_seed, double noise_k, double invalid_frac,
+                        uint16_t *out, int out_on_device);
+/* Attach a network to a pipeline (NULL detaches); tracked frames then start ICP
+ * from its prediction, with the fallback of the pipeline's kifi_icp_params. */
+kifi_status kifi_pipeline_set_twistnet(kifi_pipeline *p, kifi_twistnet *net);
 
 /* ---- measurement and staging (no reference counterpart; the reference times
  * stages with a CPU stop_watch, util/stop_watch.cpp:23-58) ----

@@ -32,7 +32,7 @@ enum {
     KIFI_TRAJ_FAST = 2    /* random walk, steps U[2,10] cm and U[2,15] deg         */
 };
 
-typedef struct {
+typedef struct kifi_scene_s {
     int    kind;
     int    n_spheres, n_boxes;
     double spheres[8][4];   /* centre xyz, radius                    */

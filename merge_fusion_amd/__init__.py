@@ -36,6 +36,10 @@ HIP_EXPORTS = [
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_memcpy_to_device",
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
+    "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
+    "kifi_twistnet_set_params", "kifi_twistnet_forward", "kifi_twistnet_loss_grad", "kifi_twistnet_train",
+    "kifi_twistnet_predict_set", "kifi_pairset_generate", "kifi_pairset_destroy", "kifi_pairset_size",
+    "kifi_pairset_download", "kifi_render", "kifi_pipeline_set_twistnet",
 ]
 STAGE_MERGE_UPDATE, STAGE_SORT, STAGE_ICP, STAGE_MODEL, STAGE_MESH = range(5)
 ICP_POINT_TO_PLANE, ICP_HORN = 0, 1
@@ -76,7 +80,8 @@ class IcpParams(ctypes.Structure):
 class IcpStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("iters", ctypes.c_int * 3), ("rms", ctypes.c_float),
                 ("inliers", ctypes.c_int), ("failed", ctypes.c_int), ("fallback", ctypes.c_int),
-                ("sum_iters", ctypes.c_longlong), ("frames", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong)]
+                ("sum_iters", ctypes.c_longlong), ("frames", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong),
+                ("rms_1m", ctypes.c_float)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k == "iters" else getattr(self, k)) for k, _ in self._fields_}
@@ -156,6 +161,23 @@ def load_hip():
             "kifi_integrate_totals": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
             "kifi_icp_align_points": ([vp, vp, ctypes.POINTER(Intrinsics), f32p, f32p, i32, i32, f32p, i32, f32p,
                                        ctypes.POINTER(IcpStats)], i32),
+            "kifi_twistnet_param_count": ([], sz),
+            "kifi_twistnet_create": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i32),
+            "kifi_twistnet_destroy": ([vp], None),
+            "kifi_twistnet_get_params": ([vp, f32p], i32),
+            "kifi_twistnet_set_params": ([vp, f32p], i32),
+            "kifi_twistnet_forward": ([vp, f32p, i32, f32p], i32),
+            "kifi_twistnet_loss_grad": ([vp, f32p, f32p, i32, f32p, f32p], i32),
+            "kifi_twistnet_train": ([vp, vp, i32, i32, ctypes.c_float, ctypes.c_uint64, f32p], i32),
+            "kifi_twistnet_predict_set": ([vp, vp, i32, i32, f32p], i32),
+            "kifi_pairset_generate": ([vp, ctypes.POINTER(Intrinsics), vp, i32, i32, ctypes.c_uint64,
+                                       ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp)], i32),
+            "kifi_pairset_destroy": ([vp], None),
+            "kifi_pairset_size": ([vp], i32),
+            "kifi_pairset_download": ([vp, f32p, f32p, f32p, ctypes.POINTER(ctypes.c_int)], i32),
+            "kifi_render": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, ctypes.c_uint64, ctypes.c_double,
+                             ctypes.c_double, vp, i32], i32),
+            "kifi_pipeline_set_twistnet": ([vp, vp], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(lib, name)
@@ -573,6 +595,12 @@ class Pipeline:
         self.ctx.check(self.ctx._lib.kifi_pipeline_integrate(self._h, ctypes.c_void_p(depth_ptr), 1, m),
                        "kifi_pipeline_integrate")
 
+    def set_twistnet(self, net):
+        """Learned initial pose for tracked frames (None detaches)."""
+        self.ctx.check(self.ctx._lib.kifi_pipeline_set_twistnet(self._h, net._h if net is not None else None),
+                       "kifi_pipeline_set_twistnet")
+        self._net = net
+
     def set_guess(self, e2w):
         g = np.ascontiguousarray(e2w, np.float64).reshape(16)
         self.ctx.check(self.ctx._lib.kifi_pipeline_set_guess(self._h, _f64p(g)), "kifi_pipeline_set_guess")
@@ -600,3 +628,120 @@ def pose_error(a, b):
     R = A[:3, :3].T @ B[:3, :3]
     ang = np.degrees(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1)))
     return dt, ang
+
+
+# ------------------------------------------------------- learned initial pose
+TWIST_IN_SHAPE = (2, 106, 128)
+TWIST_SCALE = np.array([0.15, 0.15, 0.15, 0.06, 0.06, 0.06], np.float32)
+
+
+class TwistNet:
+    """BF16 CNN twist predictor (kifi_twistnet): 2 x 106 x 128 -> 6 raw outputs."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.kifi_twistnet_create(ctx.handle, seed, ctypes.byref(h)), "kifi_twistnet_create")
+        self._h = h
+        self.n_params = ctx._lib.kifi_twistnet_param_count()
+        ctx._adopt(self)
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.kifi_twistnet_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_get_params(self._h, _f32p(out)), "kifi_twistnet_get_params")
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.n_params:
+            raise KifiError("expected %d parameters" % self.n_params)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_set_params(self._h, _f32p(p)), "kifi_twistnet_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape((-1,) + TWIST_IN_SHAPE)
+        y = np.zeros((x.shape[0], 6), np.float32)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_forward(self._h, _f32p(x), x.shape[0], _f32p(y)),
+                       "kifi_twistnet_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape((-1,) + TWIST_IN_SHAPE)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 6)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_loss_grad(self._h, _f32p(x), _f32p(t), x.shape[0], _f32p(loss),
+                                                             _f32p(g)), "kifi_twistnet_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, pairs, steps, batch=256, lr=1e-3, seed=1):
+        losses = np.zeros(max(steps, 1), np.float32)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_train(self._h, pairs._h, steps, batch, lr, seed, _f32p(losses)),
+                       "kifi_twistnet_train")
+        return losses[:steps]
+
+    def predict(self, pairs, first=0, n=None):
+        n = pairs.size - first if n is None else n
+        out = np.zeros((n, 6), np.float32)
+        self.ctx.check(self.ctx._lib.kifi_twistnet_predict_set(self._h, pairs._h, first, n, _f32p(out)),
+                       "kifi_twistnet_predict_set")
+        return out
+
+
+class PairSet:
+    """Device-resident training pairs (kifi_pairset) with ICP-converged targets."""
+
+    def __init__(self, ctx, cam, scenes, n, seed=1, max_trans=0.10, max_rot_deg=15.0):
+        self.ctx = ctx
+        arr = (Scene * len(scenes))(*scenes)
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.kifi_pairset_generate(ctx.handle, ctypes.byref(cam), ctypes.cast(arr, ctypes.c_void_p),
+                                                 len(scenes), n, seed, max_trans, max_rot_deg, ctypes.byref(h)),
+                  "kifi_pairset_generate")
+        self._h = h
+        self.size = n
+        ctx._adopt(self)
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.kifi_pairset_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self, inputs=True):
+        x = np.zeros((self.size,) + TWIST_IN_SHAPE, np.float32) if inputs else None
+        t = np.zeros((self.size, 6), np.float32)
+        g = np.zeros((self.size, 6), np.float32)
+        v = np.zeros(self.size, np.int32)
+        self.ctx.check(self.ctx._lib.kifi_pairset_download(
+            self._h, _f32p(x) if inputs else None, _f32p(t), _f32p(g), v.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+            "kifi_pairset_download")
+        return x, t, g, v.astype(bool)
+
+
+def render_device(ctx, scene, cam, eye_to_world, noise_seed, noise_k=1.5e-3, invalid_frac=0.02, out_ptr=None):
+    """kifi_render: the renderer on the GPU; host (H, W) uint16, or into a device pointer."""
+    e2w = np.ascontiguousarray(eye_to_world, np.float64)
+    if out_ptr is not None:
+        ctx.check(ctx._lib.kifi_render(ctx.handle, ctypes.byref(scene), ctypes.byref(cam), _f64p(e2w), noise_seed,
+                                       noise_k, invalid_frac, ctypes.c_void_p(out_ptr), 1), "kifi_render")
+        return None
+    out = np.zeros((cam.height, cam.width), np.uint16)
+    ctx.check(ctx._lib.kifi_render(ctx.handle, ctypes.byref(scene), ctypes.byref(cam), _f64p(e2w), noise_seed,
+                                   noise_k, invalid_frac, out.ctypes.data, 0), "kifi_render")
+    return out

@@ -25,7 +25,8 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["kifi_api.hip", "kifi_integrate.hip", "kifi_mesh.hip", "kifi_icp.hip", "kifi_pipeline.hip"]
+HIP_SOURCES = ["kifi_api.hip", "kifi_integrate.hip", "kifi_mesh.hip", "kifi_icp.hip", "kifi_pipeline.hip",
+               "kifi_learn.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -288,7 +288,7 @@ __device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds
 //      17 for Horn (sum p (3), sum q (3), sum p q^T (9), count, r^2) about origin o.
 template <int MODE>
 struct IcpNV;
-template <> struct IcpNV<KIFI_ICP_POINT_TO_PLANE> { static constexpr int nv = 29; };
+template <> struct IcpNV<KIFI_ICP_POINT_TO_PLANE> { static constexpr int nv = 30; };
 template <> struct IcpNV<KIFI_ICP_HORN> { static constexpr int nv = 17; };
 
 template <int MODE>
@@ -380,6 +380,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
                                     cadd(acc[21 + i], wgt * J[i] * r, 0.0f);
                                 cadd(acc[27], 1.0f, 0.0f);
                                 cadd(acc[28], r * r, 0.0f);
+                                cadd(acc[29], wgt * (r * r), 0.0f);
                             } else {
                                 const float a[3] = { p[0] - mpose->cam[0], p[1] - mpose->cam[1], p[2] - mpose->cam[2] };
                                 const float b[3] = { Q.x - mpose->cam[0], Q.y - mpose->cam[1], Q.z - mpose->cam[2] };
@@ -666,8 +667,12 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
     double v[NV];
     for (int j = 0; j < NV; ++j)
         v[j] = (double)lds.result[j].s + (double)lds.result[j].c;
-    const double cnt = v[NV - 2], e2 = v[NV - 1];
+    // count, sum r^2 and (point-to-plane) sum r^2 / d^4: the residual scaled to
+    // 1 m depth by the sensor noise model sigma ~ z^2
+    constexpr int IC = MODE == KIFI_ICP_POINT_TO_PLANE ? 27 : 15;
+    const double cnt = v[IC], e2 = v[IC + 1], ew2 = MODE == KIFI_ICP_POINT_TO_PLANE ? v[IC + 2] : v[IC + 1];
     st->rms = cnt > 0.0 ? (float)sqrt(e2 / cnt) : 0.0f;
+    st->rms_1m = cnt > 0.0 ? (float)sqrt(ew2 / cnt) : 0.0f;
     st->inliers = (int)cnt;
     if (cnt < (double)sp.min_inliers) {
         st->level_done = 1;
@@ -746,6 +751,7 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     st->active = 1;
     st->fallback = 0;
     st->rms = 0.0f;
+    st->rms_1m = 0.0f;
     st->inliers = 0;
     st->ticket = 0;
 }
@@ -757,24 +763,49 @@ __global__ void k_icp_level_begin(IcpState *__restrict__ st)
 }
 
 // Fallback gate (no host round trip): when the first pass started from an
-// external guess and its residual or iteration count exceeds the limits, the
-// second pass re-runs plain ICP from the previous frame's pose.
+// external guess and its 1 m-scaled residual or its iteration count exceeds the
+// limits, the second pass re-runs plain ICP from the previous frame's pose; the
+// first pass's result is kept aside for k_icp_fallback_pick.
 __global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, int max_iters)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
-    const bool bad = st->failed || (max_rms > 0.0f && st->rms > max_rms) ||
+    const bool bad = st->failed || (max_rms > 0.0f && st->rms_1m > max_rms) ||
                      (max_iters > 0 && st->total_iters >= max_iters);
     if (bad) {
         st->fallback = 1;
         st->active = 1;
+        st->alt_rms = st->failed ? FLT_MAX : st->rms_1m;
+        st->alt_inliers = st->inliers;
         st->failed = 0;
         for (int i = 0; i < 16; ++i) {
+            st->alt_pose[i] = st->pose[i];
             st->pose[i] = st->prev[i];
             st->pose_f.m[i] = (float)st->prev[i];
         }
     } else {
         st->active = 0;
+    }
+}
+
+// ICP judges the two starts: after a fallback pass, the first result stands
+// unless it failed or broke the residual limit and the fallback's residual is
+// lower (a converged pass with an acceptable residual is never replaced: a
+// far-off start can reach a wrong minimum with a smaller residual).
+__global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !st->fallback)
+        return;
+    const float second = st->failed ? FLT_MAX : st->rms_1m;
+    const bool first_bad = st->alt_rms == FLT_MAX || (max_rms > 0.0f && st->alt_rms > max_rms);
+    if (!first_bad || st->alt_rms <= second) {
+        for (int i = 0; i < 16; ++i) {
+            st->pose[i] = st->alt_pose[i];
+            st->pose_f.m[i] = (float)st->alt_pose[i];
+        }
+        st->rms_1m = st->alt_rms;
+        st->inliers = st->alt_inliers;
+        st->failed = 0;
     }
 }
 
@@ -907,6 +938,46 @@ kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host
     return KIFI_OK;
 }
 
+// Frame-to-frame model: a depth frame back-projected at its pose (binary32,
+// p = e2w * ((u + 0.5 - cx)/fx d, (cy - v - 0.5)/fy d, -d, 1)), then normals.
+__global__ __launch_bounds__(kIT) void k_model_backproject(const uint16_t *__restrict__ mm, ModelParams mp, Mat4 e2w,
+                                                           float4 *__restrict__ pts)
+{
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q >= mp.W * mp.H)
+        return;
+    const int v = q / mp.W, u = q - v * mp.W;
+    const float d = (float)mm[q] * 0.001f;
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (d > 0.0f) {
+        const float pe[4] = { ((float)u + 0.5f - mp.cx) / mp.fx * d, (mp.cy - (float)v - 0.5f) / mp.fy * d, -d, 1.0f };
+        float p[4];
+        mat_vec(e2w, pe, p);
+        o = make_float4(p[0], p[1], p[2], 1.0f);
+    }
+    pts[q] = o;
+}
+
+kifi_status model_from_depth(kifi_model *m, const uint16_t *d_mm, const float e2w[16], const float w2e[16])
+{
+    kifi_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    Mat4 wv, ev, clip;
+    for (int i = 0; i < 16; ++i) {
+        wv.m[i] = w2e[i];
+        ev.m[i] = e2w[i];
+    }
+    eye_to_clip_rh(m->cam.width, m->cam.height, m->cam.fx, m->cam.fy, m->cam.cx, m->cam.cy, m->cam.znear,
+                   m->cam.zfar, clip.m);
+    const int P = m->mp.W * m->mp.H;
+    hipLaunchKernelGGL(k_model_pose, dim3(1), dim3(64), 0, st, wv, (const Mat4 *)nullptr, clip, m->d_pose);
+    hipLaunchKernelGGL(k_model_backproject, dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, d_mm, m->mp, ev, m->d_pts);
+    hipLaunchKernelGGL(k_model_normals, dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, m->d_pts, m->mp.W, m->mp.H,
+                       m->d_pose, m->disc, m->d_nrm);
+    KIFI_HIP(ctx, hipGetLastError());
+    return KIFI_OK;
+}
+
 // A caller's point buffer as the model (ICP::Align's synthDepthBuffer and
 // synthEyeToWorld, ICP.cpp:38-48): W*H float3 world points, all-zero = empty
 // (ICP.cpp:112-113).  The association projection is EyeToClipRH of the raw
@@ -958,7 +1029,7 @@ kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H)
     const size_t P = (size_t)W * H;
     size_t bytes = al256(P * 4) + al256(P / 4 * 4 + 4) + al256(P / 16 * 4 + 4);
     const size_t nb = (P + kIT - 1) / kIT;
-    bytes += al256(nb * 29 * sizeof(CSum));
+    bytes += al256(nb * IcpNV<KIFI_ICP_POINT_TO_PLANE>::nv * sizeof(CSum));
     KIFI_HIP(ctx, hipMalloc(&w->buf, bytes));
     char *p = (char *)w->buf;
     w->lv[0] = (float *)p; p += al256(P * 4);
@@ -1006,6 +1077,7 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const
         hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
                            ip->fallback_iters);
         icp_pass(m, w, cam, ip, levels);
+        hipLaunchKernelGGL(k_icp_fallback_pick, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms);
     }
     hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state);
     timer_end(ctx, KIFI_STAGE_ICP, t_icp);
@@ -1060,7 +1132,7 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
                                    w->d_state, (CSum *)w->partials, l, sp);
             } else {
                 hipLaunchKernelGGL(k_icp_assoc<KIFI_ICP_POINT_TO_PLANE>, dim3(nb), dim3(kIT), 0, st, lv, m->mp,
-                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, ip->dist_thresh,
+                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, ip->dist_thresh * (float)(1 << l),
                                    (CSum *)w->partials, l, sp);
             }
         }

@@ -49,7 +49,12 @@ struct IcpState {
     int    active;          // gate of the current pass
     int    fallback;        // this frame re-ran plain ICP
     float  rms;
+    float  rms_1m;          // RMS of r / d^2 x 1 m^2 (noise-normalised residual)
     int    inliers;
+    // first pass of a frame that fell back (kept if it beats the second)
+    double alt_pose[16];
+    float  alt_rms;
+    int    alt_inliers;
     unsigned ticket;        // workgroups done in the current association launch
     // running totals over frames
     long long sum_iters;
@@ -86,6 +91,7 @@ void        model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, 
 SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev);
 void        model_end(kifi_model *m, kifi_volume *vol, bool splat);
 kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, const kifi_intrinsics *proj_cam);
+kifi_status model_from_depth(kifi_model *m, const uint16_t *d_mm, const float e2w[16], const float w2e[16]);
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 void        icp_free(IcpWork *w);
 kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,

@@ -10,6 +10,10 @@ namespace kifi {
 kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st);
+bool twistnet_guess(kifi_twistnet *n, const uint16_t *d_mm, int W, int H, const double *prev_pose_dev,
+                    double *guess_dev, hipStream_t st);
+void twistnet_observe(kifi_twistnet *n, const uint16_t *d_mm, int W, int H, hipStream_t st);
+void twistnet_reset_frames(kifi_twistnet *n);
 } // namespace kifi
 
 using namespace kifi;
@@ -25,6 +29,7 @@ struct kifi_pipeline {
     uint16_t        *d_depth = nullptr; // staging for host frames
     double          *d_guess = nullptr;
     bool             have_guess = false;
+    kifi_twistnet   *net = nullptr;     // learned initial guess (optional, not owned)
     double           guess[16];
     long long        frame = 0;
 };
@@ -129,6 +134,7 @@ static void fill_stats(const IcpState &s, kifi_icp_stats *st)
     for (int l = 0; l < 3; ++l)
         st->iters[l] = s.iters[l];
     st->rms = s.rms;
+    st->rms_1m = s.rms_1m;
     st->inliers = s.inliers;
     st->failed = s.failed;
     st->fallback = s.fallback;
@@ -347,6 +353,16 @@ kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double e2w[16])
     return KIFI_OK;
 }
 
+kifi_status kifi_pipeline_set_twistnet(kifi_pipeline *p, kifi_twistnet *net)
+{
+    if (!p)
+        return KIFI_ERR_ARG;
+    if (net)
+        twistnet_reset_frames(net);
+    p->net = net;
+    return KIFI_OK;
+}
+
 kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double e2w[16])
 {
     if (!p || !e2w)
@@ -372,18 +388,30 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
     if (w2e) {
         // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
+        if (p->net)
+            twistnet_observe(p->net, d_depth, p->cam.width, p->cam.height, st);
     } else if (p->frame > 0) {
-        // Pipeline.cpp:37-39: track against the model of the previous frame
+        // Pipeline.cpp:37-39: track against the model of the previous frame,
+        // from a caller's guess, the network's prediction, or the previous pose
         kifi_icp_params ip = p->params;
-        if (!p->have_guess) {
+        const double *guess_dev = prev_pose_dev(&p->work);
+        bool guessed = p->have_guess;
+        if (!guessed && p->net &&
+            twistnet_guess(p->net, d_depth, p->cam.width, p->cam.height, prev_pose_dev(&p->work), p->d_guess, st)) {
+            guess_dev = p->d_guess;
+            guessed = true;
+        }
+        if (!guessed) {
             ip.fallback_rms = 0.0f;   // the guess is the previous pose: nothing to fall back to
             ip.fallback_iters = 0;
         }
         s = icp_run(p->model, &p->work, d_depth, &p->cam, p->have_guess ? p->guess : nullptr,
-                    p->have_guess ? nullptr : prev_pose_dev(&p->work), &ip);
+                    p->have_guess ? nullptr : guess_dev, &ip);
         p->have_guess = false;
         if (s != KIFI_OK)
             return s;
+    } else if (p->net) {
+        twistnet_observe(p->net, d_depth, p->cam.width, p->cam.height, st);
     }
     // Pipeline.cpp:41-46: worldToEye = invert(eyeToWorld), integrate; the
     // model for the next frame (Pipeline.cpp:48-52, seen from this pose) is
