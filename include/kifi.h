@@ -130,8 +130,7 @@ typedef struct {
     int   mode;            /* KIFI_ICP_POINT_TO_PLANE or KIFI_ICP_HORN                 */
     int   levels;          /* pyramid levels, 1..3 (level 0 = full resolution)         */
     int   iters[3];        /* maximum iterations per level, index = level              */
-    float dist_thresh;     /* correspondence gate at full resolution, metres; level l  */
-                           /* of the pyramid gates at dist_thresh x 2^l                */
+    float dist_thresh;     /* correspondence gate at full resolution, metres           */
     float eps_rot;         /* convergence: |rotation increment| < eps_rot (rad) ...    */
     float eps_trans;       /* ... and |translation increment| < eps_trans (m)          */
     int   min_inliers;     /* fewer correspondences: stop (tracking failure)           */
@@ -141,6 +140,9 @@ typedef struct {
                            /*       fallback result replaces the first only if that    */
                            /*       failed or broke fallback_rms and the fallback's    */
                            /*       rms_1m is lower                                    */
+    float level_gate_growth; /* level l gates at dist_thresh x growth^l (1: the same    */
+                             /* gate on every level; 2 widens the coarse levels for    */
+                             /* large initial errors)                                  */
 } kifi_icp_params;
 
 typedef struct {
@@ -269,7 +271,9 @@ kifi_status kifi_pipeline_set_twistnet(kifi_pipeline *p, kifi_twistnet *net);
 /* ---- measurement and staging (no reference counterpart; the reference times
  * stages with a CPU stop_watch, util/stop_watch.cpp:23-58) ----
  * Stage timers bracket the named kernels with HIP events on the context's
- * stream while enabled; kifi_timing_read synchronises and returns the summed
+ * stream for the stages whose bit is set in stage_mask (1 << KIFI_STAGE_*;
+ * 0 disables; each event pair costs some microseconds of stream time, so time
+ * only what is read); kifi_timing_read synchronises and returns the summed
  * elapsed milliseconds and the number of timed launches, then resets. */
 enum {
     KIFI_STAGE_MERGE_UPDATE = 0, /* fused merge + TSDF update kernel            */
@@ -277,9 +281,10 @@ enum {
     KIFI_STAGE_ICP = 2,          /* depth pyramid + all ICP iterations          */
     KIFI_STAGE_MODEL = 3,        /* model splat + resolve + normals             */
     KIFI_STAGE_MESH = 4,         /* marching cubes                              */
-    KIFI_STAGE_COUNT = 5
+    KIFI_STAGE_COUNT = 5,
+    KIFI_TIMING_ALL = (1 << 5) - 1
 };
-kifi_status kifi_timing_enable(kifi_ctx *ctx, int enable);
+kifi_status kifi_timing_enable(kifi_ctx *ctx, int stage_mask);
 kifi_status kifi_timing_read(kifi_ctx *ctx, int stage, double *total_ms, long long *count);
 /* Device buffers for callers without their own allocator (frames resident in HBM). */
 kifi_status kifi_device_alloc(kifi_ctx *ctx, size_t bytes, void **out);

@@ -68,7 +68,8 @@ class VolumeDesc(ctypes.Structure):
 class IcpParams(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("levels", ctypes.c_int), ("iters", ctypes.c_int * 3),
                 ("dist_thresh", ctypes.c_float), ("eps_rot", ctypes.c_float), ("eps_trans", ctypes.c_float),
-                ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int)]
+                ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int),
+                ("level_gate_growth", ctypes.c_float)]
 
     @classmethod
     def default(cls):
@@ -299,8 +300,12 @@ class Context:
         except Exception:
             pass
 
-    def timing(self, enable):
-        self.check(self._lib.kifi_timing_enable(self._h, 1 if enable else 0), "kifi_timing_enable")
+    def timing(self, enable, stages=None):
+        """Stage timers on (all stages, or the STAGE_* listed) or off."""
+        mask = 0
+        if enable:
+            mask = (1 << 5) - 1 if stages is None else sum(1 << s for s in stages)
+        self.check(self._lib.kifi_timing_enable(self._h, mask), "kifi_timing_enable")
 
     def timing_read(self, stage):
         """(total ms, launches) of a stage since the last read."""

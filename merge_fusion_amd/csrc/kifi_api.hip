@@ -113,7 +113,7 @@ static hipEvent_t pool_event(kifi_ctx *ctx)
 
 int timer_begin(kifi_ctx *ctx, int stage)
 {
-    if (!ctx->timing || stage < 0 || stage >= KIFI_STAGE_COUNT)
+    if (stage < 0 || stage >= KIFI_STAGE_COUNT || !(ctx->timing_mask >> stage & 1))
         return -1;
     hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
     if (!a || !b)
@@ -136,11 +136,11 @@ using namespace kifi;
 
 extern "C" {
 
-kifi_status kifi_timing_enable(kifi_ctx *ctx, int enable)
+kifi_status kifi_timing_enable(kifi_ctx *ctx, int stage_mask)
 {
     if (!ctx)
         return KIFI_ERR_ARG;
-    ctx->timing = enable != 0;
+    ctx->timing_mask = (unsigned)stage_mask & ((1u << KIFI_STAGE_COUNT) - 1u);
     return KIFI_OK;
 }
 

@@ -1124,6 +1124,9 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
         // reduction covers few partials
         const int cap = std::max(1, m->ctx->num_cus);
         const int nb = std::min((lv.W * lv.H + kIT - 1) / kIT, cap);
+        float gate = ip->dist_thresh;
+        for (int k = 0; k < l; ++k)
+            gate *= ip->level_gate_growth > 0.0f ? ip->level_gate_growth : 1.0f;
         hipLaunchKernelGGL(k_icp_level_begin, dim3(1), dim3(64), 0, st, w->d_state);
         for (int it = 0; it < ip->iters[l]; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
@@ -1132,7 +1135,7 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
                                    w->d_state, (CSum *)w->partials, l, sp);
             } else {
                 hipLaunchKernelGGL(k_icp_assoc<KIFI_ICP_POINT_TO_PLANE>, dim3(nb), dim3(kIT), 0, st, lv, m->mp,
-                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, ip->dist_thresh * (float)(1 << l),
+                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, gate,
                                    (CSum *)w->partials, l, sp);
             }
         }

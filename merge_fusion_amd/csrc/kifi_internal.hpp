@@ -189,7 +189,7 @@ struct kifi_ctx {
     size_t       depth_pixels = 0;
     int          num_cus = 0;
     // stage timers (HIP event pairs recorded on `stream` while enabled)
-    bool         timing = false;
+    unsigned     timing_mask = 0;   // bit s: time stage s
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timers[KIFI_STAGE_COUNT];
     std::vector<hipEvent_t> event_pool;
 };

@@ -63,6 +63,7 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->eps_trans = 2e-4f;
     p->min_inliers = 64;
     p->fallback_rms = 0.0f;
+    p->level_gate_growth = 1.0f;
     p->fallback_iters = 0;
     return KIFI_OK;
 }

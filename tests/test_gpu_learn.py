@@ -115,6 +115,7 @@ def test_pipeline_with_learned_init_needs_fewer_iterations(ctx, trained):
     out = {}
     for name, use in (("plain", False), ("learned", True)):
         prm = mf.IcpParams.default()
+        prm.level_gate_growth = 2.0
         if use:
             prm.fallback_rms = 0.005
             prm.fallback_iters = 20
