@@ -309,7 +309,9 @@ kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *d, kifi_vo
     v->consts.minOverNegVl = v->consts.vlInv * -minx;
     v->consts.trunc = d->trunc;
     hipError_t e = hipMalloc(&v->d_counters, sizeof(FrameCounters));
-    if (e == hipSuccess) e = hipHostMalloc(&v->h_counters, sizeof(FrameCounters), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&v->h_counters, sizeof(FrameCounters),
+                                                 hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&v->h_counters_dev, v->h_counters, 0);
     if (e == hipSuccess) e = hipMalloc(&v->d_size, 8);
     if (e == hipSuccess) e = hipMalloc(&v->d_mats, sizeof(FrameMats));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&v->size_event, hipEventDisableTiming);

@@ -168,12 +168,6 @@ __global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict_
     nrm[q] = N;
 }
 
-__global__ void k_fill_zbuf(unsigned long long *__restrict__ p, int n)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = ~0ull;
-}
-
 // model pose from a world->eye matrix (host value or device pointer)
 __global__ void k_model_pose(Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev, Mat4 clip,
                              ModelPose *__restrict__ out)
@@ -191,14 +185,64 @@ __global__ void k_model_pose(Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev, Mat
     *out = p;
 }
 
-// ------------------------------------------------------------ depth pyramid
-__global__ void k_depth_level0(const uint16_t *__restrict__ mm, int n, float *__restrict__ out)
+// k_model_pose and the z-buffer clear (a fill kernel, never a memset node) in one launch
+__global__ void k_model_begin(Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev, Mat4 clip, ModelPose *__restrict__ out,
+                              unsigned long long *__restrict__ zbuf, int n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = (float)mm[i] * 0.001f;                // DepthStream.cpp:89-90
+        zbuf[i] = ~0ull;
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    ModelPose p;
+    p.w2e = w2e_dev ? *w2e_dev : w2e_val;
+    mat_mul(clip.m, p.w2e.m, p.proj.m);
+    Mat4 e2w = p.w2e;
+    invert_transform(e2w.m);
+    p.cam[0] = e2w.m[12];
+    p.cam[1] = e2w.m[13];
+    p.cam[2] = e2w.m[14];
+    *out = p;
 }
 
+// ------------------------------------------------------------ depth pyramid
 // 2x2 mean of valid depths in the order (0,0),(1,0),(0,1),(1,1)
+// Levels 1 and 2 in one launch: the thread of a level-2 pixel computes its four
+// level-1 pixels (each exactly as k_depth_down does) and then itself from them.
+__device__ __forceinline__ float mean_valid4(float a, float b, float c, float d)
+{
+    float s = 0.0f, n = 0.0f;
+    const float v[4] = { a, b, c, d };
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (v[k] > 0.0f) {
+            s += v[k];
+            n += 1.0f;
+        }
+    return n > 0.0f ? s / n : 0.0f;
+}
+
+__global__ void k_depth_down2(const float *__restrict__ l0, int W, int H, float *__restrict__ l1,
+                              float *__restrict__ l2)
+{
+    const int W1 = W / 2, W2 = W / 4, H2 = H / 4;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= W2 * H2)
+        return;
+    const int y = q / W2, x = q - y * W2;
+    float c1[4];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int x1 = 2 * x + i, y1 = 2 * y + j;
+            const float *r0 = l0 + (size_t)(2 * y1) * W + 2 * x1, *r1 = r0 + W;
+            const float v = mean_valid4(r0[0], r0[1], r1[0], r1[1]);
+            l1[(size_t)y1 * W1 + x1] = v;
+            c1[2 * j + i] = v;
+        }
+    l2[q] = mean_valid4(c1[0], c1[1], c1[2], c1[3]);
+}
+
 __global__ void k_depth_down(const float *__restrict__ src, int W, int H, float *__restrict__ dst)
 {
     const int w = W / 2, h = H / 2;
@@ -335,7 +379,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
     constexpr int NV = IcpNV<MODE>::nv;
     constexpr int MODE_OF_KERNEL = MODE;
     __shared__ CsumLds<NV> lds;
-    if (st->level_done)
+    if (st->level_done[level])
         return;
     CSum acc[NV];
 #pragma unroll
@@ -419,7 +463,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
     constexpr int NV = IcpNV<KIFI_ICP_HORN>::nv;
     constexpr int MODE_OF_KERNEL = KIFI_ICP_HORN;
     __shared__ CsumLds<NV> lds;
-    if (st->level_done)
+    if (st->level_done[level])
         return;
     CSum acc[NV];
 #pragma unroll
@@ -675,7 +719,7 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
     st->rms_1m = cnt > 0.0 ? (float)sqrt(ew2 / cnt) : 0.0f;
     st->inliers = (int)cnt;
     if (cnt < (double)sp.min_inliers) {
-        st->level_done = 1;
+        st->level_done[level] = 1;
         st->failed = 1;
         return;
     }
@@ -717,7 +761,7 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
         ok = true;
     }
     if (!ok) {
-        st->level_done = 1;
+        st->level_done[level] = 1;
         st->failed = 1;
         return;
     }
@@ -729,12 +773,17 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
     const double rot = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
     const double tr = sqrt(xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
     if (rot < sp.eps_rot && tr < sp.eps_trans)
-        st->level_done = 1;
+        st->level_done[level] = 1;
 }
 
 // state control kernels
-__global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val)
+// Frame start: the state from the guess (one thread) and the level-0 depth map
+// in metres (DepthStream.cpp:89-90), in one launch.
+__global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val,
+                            const uint16_t *__restrict__ mm, int n, float *__restrict__ lv0)
 {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        lv0[i] = (float)mm[i] * 0.001f;
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     const double *g = guess_dev ? guess_dev : guess_val.m;
@@ -746,7 +795,8 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     for (int l = 0; l < 4; ++l)
         st->iters[l] = 0;
     st->total_iters = 0;
-    st->level_done = 0;
+    for (int l = 0; l < 4; ++l)
+        st->level_done[l] = 0;
     st->failed = 0;
     st->active = 1;
     st->fallback = 0;
@@ -756,11 +806,6 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     st->ticket = 0;
 }
 
-__global__ void k_icp_level_begin(IcpState *__restrict__ st)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        st->level_done = st->active ? 0 : 1;
-}
 
 // Fallback gate (no host round trip): when the first pass started from an
 // external guess and its 1 m-scaled residual or its iteration count exceeds the
@@ -772,6 +817,8 @@ __global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, i
         return;
     const bool bad = st->failed || (max_rms > 0.0f && st->rms_1m > max_rms) ||
                      (max_iters > 0 && st->total_iters >= max_iters);
+    for (int l = 0; l < 4; ++l)
+        st->level_done[l] = bad ? 0 : 1;     // the second pass runs only after a bad first one
     if (bad) {
         st->fallback = 1;
         st->active = 1;
@@ -809,7 +856,9 @@ __global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
     }
 }
 
-__global__ void k_icp_frame_end(IcpState *__restrict__ st)
+// Frame end; with w2e set, also the binary32 world->eye of the tracked pose
+// (as k_icp_w2e writes it).
+__global__ void k_icp_frame_end(IcpState *__restrict__ st, Mat4 *__restrict__ w2e)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
@@ -818,6 +867,11 @@ __global__ void k_icp_frame_end(IcpState *__restrict__ st)
     st->sum_iters += st->total_iters;
     st->frames_tracked += 1;
     st->fallbacks += st->fallback;
+    if (w2e) {
+        Mat4 m = st->pose_f;
+        invert_transform(m.m);
+        *w2e = m;
+    }
 }
 
 // Sets the pose without tracking (first frame or a given pose).
@@ -891,8 +945,8 @@ void model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const M
     m->mp.c0 = vol->consts.vlHalfPlusMin;
     m->mp.dmax = vol->consts.vl;                 // near-surface band: |D| < one voxel
     const int P = m->mp.W * m->mp.H;
-    hipLaunchKernelGGL(k_model_pose, dim3(1), dim3(64), 0, st, wv, w2e_dev, clip, m->d_pose);
-    hipLaunchKernelGGL(k_fill_zbuf, dim3((P + 255) / 256), dim3(256), 0, st, m->d_zbuf, P);
+    hipLaunchKernelGGL(k_model_begin, dim3((P + 255) / 256), dim3(256), 0, st, wv, w2e_dev, clip, m->d_pose,
+                       m->d_zbuf, P);
 }
 
 // Splat target for the fused voxel pass of an integration.
@@ -1055,7 +1109,7 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
 
 // Runs the whole pyramid schedule asynchronously on the context's stream.
 kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
-                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip)
+                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip, Mat4 *w2e_out)
 {
     kifi_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -1065,13 +1119,15 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const
     for (int i = 0; i < 16; ++i)
         gv.m[i] = guess_host ? guess_host[i] : 0.0;
     const int t_icp = timer_begin(ctx, KIFI_STAGE_ICP);
-    hipLaunchKernelGGL(k_icp_begin, dim3(1), dim3(64), 0, st, w->d_state, guess_dev, gv);
-    hipLaunchKernelGGL(k_depth_level0, dim3((P + 255) / 256), dim3(256), 0, st, d_depth_mm, P, w->lv[0]);
+    hipLaunchKernelGGL(k_icp_begin, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv, d_depth_mm,
+                       P, w->lv[0]);
     const int levels = ip->levels < 1 ? 1 : (ip->levels > 3 ? 3 : ip->levels);
-    int Wl[3] = { W, W / 2, W / 4 }, Hl[3] = { H, H / 2, H / 4 };
-    for (int l = 1; l < levels; ++l)
-        hipLaunchKernelGGL(k_depth_down, dim3((Wl[l] * Hl[l] + 255) / 256), dim3(256), 0, st, w->lv[l - 1],
-                           Wl[l - 1], Hl[l - 1], w->lv[l]);
+    if (levels == 3)
+        hipLaunchKernelGGL(k_depth_down2, dim3((W / 4 * (H / 4) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
+                           w->lv[1], w->lv[2]);
+    else if (levels == 2)
+        hipLaunchKernelGGL(k_depth_down, dim3((W / 2 * (H / 2) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
+                           w->lv[1]);
     icp_pass(m, w, cam, ip, levels);
     if (ip->fallback_rms > 0.0f || ip->fallback_iters > 0) {
         hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
@@ -1079,7 +1135,7 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const
         icp_pass(m, w, cam, ip, levels);
         hipLaunchKernelGGL(k_icp_fallback_pick, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms);
     }
-    hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state);
+    hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state, w2e_out);
     timer_end(ctx, KIFI_STAGE_ICP, t_icp);
     KIFI_HIP(ctx, hipGetLastError());
     return KIFI_OK;
@@ -1127,7 +1183,6 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
         float gate = ip->dist_thresh;
         for (int k = 0; k < l; ++k)
             gate *= ip->level_gate_growth > 0.0f ? ip->level_gate_growth : 1.0f;
-        hipLaunchKernelGGL(k_icp_level_begin, dim3(1), dim3(64), 0, st, w->d_state);
         for (int it = 0; it < ip->iters[l]; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
                 const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, cap);

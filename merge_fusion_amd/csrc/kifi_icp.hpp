@@ -44,7 +44,7 @@ struct IcpState {
     Mat4   pose_f;          // binary32 copy used by the association kernels
     int    iters[4];
     int    total_iters;
-    int    level_done;
+    int    level_done[4];   // per pyramid level: converged, failed or not run
     int    failed;
     int    active;          // gate of the current pass
     int    fallback;        // this frame re-ran plain ICP
@@ -95,7 +95,8 @@ kifi_status model_from_depth(kifi_model *m, const uint16_t *d_mm, const float e2
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 void        icp_free(IcpWork *w);
 kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
-                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip);
+                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
+                    Mat4 *w2e_out = nullptr);
 void        icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st);
 void        store_w2e(const float *w2e, Mat4 *d_out, hipStream_t st);
 kifi_status horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16]);

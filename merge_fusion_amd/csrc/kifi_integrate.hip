@@ -21,14 +21,23 @@ constexpr int kRsTile = kThreads * kRsItems;   // 4096 keys per radix tile
 constexpr int kMgTile = 4096;                  // merged positions per merge tile
 
 // ===================================================================== reset
-__global__ void k_reset(FrameCounters *c, uint32_t *totals, int n_totals)
+// Frame start: per-frame counters and radix totals cleared (a fill kernel, never
+// a memset node), and the frame's matrices from worldToEye (Integrator.cpp:41-42).
+__global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, Mat4 clip, Mat4 w2e_val,
+                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
-        c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
-    }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
         totals[i] = 0;
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
+    c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
+    FrameMats f;
+    f.w2e = w2e_dev ? *w2e_dev : w2e_val;
+    mat_mul(clip.m, f.w2e.m, f.w2c.m);
+    f.e2w = f.w2e;
+    invert_transform(f.e2w.m);
+    *out = f;
 }
 
 // ==================================================================== keygen
@@ -42,18 +51,7 @@ struct KeygenParams {
 // Integrator.cpp:41-42: worldToClip = EyeToClipRH() * worldToEye and
 // eyeToWorld = invert_transform(worldToEye), in binary32 on the device so that a
 // device-resident pose (tracking) and a host pose give the same bits.
-__global__ void k_frame_setup(Mat4 clip, Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev,
-                              FrameMats *__restrict__ out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    FrameMats f;
-    f.w2e = w2e_dev ? *w2e_dev : w2e_val;
-    mat_mul(clip.m, f.w2e.m, f.w2c.m);
-    f.e2w = f.w2e;
-    invert_transform(f.e2w.m);
-    *out = f;
-}
+
 
 // Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
 // the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
@@ -444,11 +442,14 @@ __global__ __launch_bounds__(kThreads) void k_merge_dups(const K *__restrict__ A
 
 __global__ __launch_bounds__(1024) void k_merge_scan(uint32_t *__restrict__ tile_dups,
                                                      FrameCounters *__restrict__ cnt,
-                                                     uint64_t *__restrict__ d_size, uint64_t cap)
+                                                     uint64_t *__restrict__ d_size, uint64_t cap,
+                                                     FrameCounters *__restrict__ host_cnt)
 {
     __shared__ uint64_t total;
     const uint32_t T = cnt->n_tiles;
     if (cnt->overflow) {
+        if (threadIdx.x == 0)
+            *host_cnt = *cnt;                // host-mapped: read after the frame's event
         return;
     }
     block_scan_array<1024>(tile_dups, (size_t)T, &total);
@@ -466,6 +467,7 @@ __global__ __launch_bounds__(1024) void k_merge_scan(uint32_t *__restrict__ tile
             cnt->tot_v_new += v1;
             cnt->tot_keys += cnt->n_keys;
         }
+        *host_cnt = *cnt;
     }
 }
 
@@ -479,26 +481,28 @@ struct UpdateParams {
 // Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356).  Returns true and
 // the clamped signed distance when the voxel receives an update.
 template <typename K>
-__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, float &sd_out)
+__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const Mat4 &w2c, float &sd_out)
 {
     uint32_t x, y, z;
     unpack_key<K>(key, x, y, z);
     float k[4] = { (float)x * p.vl + p.c0, (float)y * p.vl + p.c0, (float)z * p.vl + p.c0,
                    1.0f };                                         // Volume.cpp:75-78
-    mat_vec(p.mats->w2c, k, k);                                    // :290-293
+    mat_vec(w2c, k, k);                                            // :290-293
     const float kw = k[3];
     // |x/w| or |y/w| above 1.01 lands outside [0,W) x [0,H) whatever the rounding,
-    // so such voxels (most of the volume) skip the four divisions; NaN passes on
+    // so such voxels (most of the volume) skip the divisions; NaN passes on
     const float lim = 1.01f * fabsf(kw);
     if (fabsf(k[0]) > lim || fabsf(k[1]) > lim)
         return false;
-    float u = k[0] / kw, v = k[1] / kw, zz = k[2] / kw, ww = k[3] / kw;  // :305-308
+    // :305-315 divides all four lanes by w and keeps the voxel only if
+    // (z/w)*0+0 and (w/w)*0+0 are in range, i.e. z/w is finite and w is finite
+    // and nonzero; that needs no division unless |w| is tiny or |z| huge
+    const bool wok = kw != 0.0f && fabsf(kw) <= FLT_MAX;
+    const bool zok = (fabsf(kw) >= 0x1p-32f && fabsf(k[2]) <= 0x1p32f) || fabsf(k[2] / kw) <= FLT_MAX;
+    float u = k[0] / kw, v = k[1] / kw;
     u = u * p.ndcU + p.ndcU;                                       // :310-313
     v = v * p.ndcV + p.ndcV;
-    zz = zz * 0.0f + 0.0f;
-    ww = ww * 0.0f + 0.0f;
-    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zz >= -1.0f && zz < 1.0f &&
-        ww >= -FLT_MAX && ww < FLT_MAX) {                          // :315
+    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zok && wok) {
         const float d = (float)p.depth[(size_t)(p.H - (unsigned)v - 1) * p.W + (unsigned)u] * 0.001f;
         const float sd = d - kw;                                   // :319-320
         if (d > 0.0f && sd >= -p.trunc) {                          // :322
@@ -548,13 +552,26 @@ __device__ __forceinline__ uint32_t orderable_z(float z)
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// |D| < dmax of the stored voxel.  Binary32 sum/weight: a product test decides
+// all but the cases within 2^-20 of the bound (the rounded quotient can only
+// land on dmax inside that margin), which take the exact division.
+__device__ __forceinline__ bool near_surface(const VoxF32 &v, float dmax)
+{
+    const float a = fabsf(v.sum), t = dmax * v.weight;
+    if (a >= t * (1.0f + 0x1p-20f))
+        return false;
+    if (a <= t * (1.0f - 0x1p-20f))
+        return true;
+    return fabsf(v.sum / v.weight) < dmax;
+}
+__device__ __forceinline__ bool near_surface(uint32_t v, float dmax) { return fabsf(vox_dist(v)) < dmax; }
+
 template <typename K, typename V>
 __device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
 {
     if (!(vox_weight(val) > 0.0f))
         return;
-    const float D = vox_dist(val);
-    if (!(fabsf(D) < sp.dmax))
+    if (!near_surface(val, sp.dmax))
         return;
     uint32_t x, y, zz;
     unpack_key<K>(key, x, y, zz);
@@ -597,11 +614,17 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     SplatParams sp)
 {
     __shared__ K sA[kMgTile];
-    __shared__ K sB[kMgTile];
+    __shared__ K sB[kMgTile];          // the tile's new keys, later the A-side shift counts
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
         return;
+    // frame matrices into registers once (the output stores would otherwise force
+    // a reload per voxel)
+    const Mat4 w2c = up.mats->w2c;
+    Mat4 w2e;
+    if (SPLAT)
+        w2e = *sp.w2e;
     const uint64_t tot = cnt->v_old + cnt->n_new;
     const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
@@ -611,21 +634,24 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads)
         sB[j] = B[mt.b0 + j];
     __syncthreads();
-    // drop the new keys already in the volume (A wins ties; the A element just
-    // before the tile can match the first B element), compacting sB in place
+    // new keys already in the volume are dropped (A wins ties; the A element just
+    // before the tile can match the first B element); lb = #A keys below the key
     const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
     constexpr int PER = kMgTile / kThreads;
     K kept[PER];
+    uint32_t lbs[PER];
     uint32_t fl = 0, c = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t j = threadIdx.x * PER + q;
         bool keep = false;
+        lbs[q] = 0;
         if (j < mt.nb) {
             const K b = sB[j];
             const uint32_t lb = lds_lower_bound(sA, mt.na, b);
             keep = !((lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b));
             kept[q] = b;
+            lbs[q] = lb;
         }
         fl |= (uint32_t)keep << q;
         c += keep;
@@ -637,29 +663,56 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
         if (fl >> q & 1u)
             sB[pos++] = kept[q];
     __syncthreads();
-    Mat4 w2e;
-    if (SPLAT)
-        w2e = *sp.w2e;
-    // old voxels: keep their value (first range wins), then update
-    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads) {
-        const K k = sA[i];
-        const uint64_t o = obase + i + lds_lower_bound(sB, nbk, k);
-        V val = Av[mt.a0 + i];
+    // new voxels (spread over all threads): Voxel() default, then update
+    for (uint32_t j = threadIdx.x; j < nbk; j += kThreads) {
+        const K k = sB[j];
+        const uint64_t o = obase + j + lds_lower_bound(sA, mt.na, k);
+        V val{};
         float s;
-        if (o < upd_end && voxel_sample<K>(k, up, s))
+        if (o < upd_end && voxel_sample<K>(k, up, w2c, s))
             voxel_apply(val, s);
         O[o] = k;
         Ov[o] = val;
         if (SPLAT)
             splat_voxel<K, V>(k, val, o, sp, w2e);
     }
-    // new voxels: Voxel() default, then update
-    for (uint32_t j = threadIdx.x; j < nbk; j += kThreads) {
-        const K k = sB[j];
-        const uint64_t o = obase + j + lds_lower_bound(sA, mt.na, k);
-        V val{};
+    __syncthreads();
+    // old voxels: shift of A[i] = #kept B keys below it = #kept with lb <= i, by an
+    // LDS histogram of the lbs and an inclusive scan over the tile's A positions
+    uint32_t *h = reinterpret_cast<uint32_t *>(sB);
+    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads)
+        h[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+        if ((fl >> q & 1u) && lbs[q] < mt.na)
+            atomicAdd(&h[lbs[q]], 1u);
+    __syncthreads();
+    {
+        uint32_t loc[PER], sum = 0;
+        const uint32_t base = threadIdx.x * PER;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            loc[q] = base + q < mt.na ? h[base + q] : 0u;
+            sum += loc[q];
+        }
+        uint32_t total;
+        uint32_t run = block_exclusive_scan<kThreads>(sum, tmp, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            run += loc[q];
+            if (base + q < mt.na)
+                h[base + q] = run;
+        }
+    }
+    __syncthreads();
+    // old voxels keep their value (first range wins), then update
+    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads) {
+        const K k = sA[i];
+        const uint64_t o = obase + i + h[i];
+        V val = Av[mt.a0 + i];
         float s;
-        if (o < upd_end && voxel_sample<K>(k, up, s))
+        if (o < upd_end && voxel_sample<K>(k, up, w2c, s))
             voxel_apply(val, s);
         O[o] = k;
         Ov[o] = val;
@@ -740,12 +793,12 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     FrameCounters *cnt = vol->d_counters;
     hipStream_t stream = ctx->stream;
 
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256);
     Mat4 clip, wval;
     eye_to_clip_rh(cam->width, cam->height, cam->fx, cam->fy, cam->cx, cam->cy, cam->znear, cam->zfar, clip.m);
     for (int i = 0; i < 16; ++i)
         wval.m[i] = w2e ? w2e[i] : 0.0f;
-    hipLaunchKernelGGL(k_frame_setup, dim3(1), dim3(64), 0, stream, clip, wval, w2e_dev, vol->d_mats);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256, clip, wval,
+                       w2e_dev, vol->d_mats);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;
@@ -790,7 +843,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                        (const K *)vol->keys[cur], Ekeys, part_a, cnt, tile_dups);
     hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
-                       (uint64_t)vol->cap);
+                       (uint64_t)vol->cap, vol->h_counters_dev);
     UpdateParams up;
     up.mats = vol->d_mats;
     up.vl = vol->consts.vl;
@@ -815,8 +868,8 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
 
-    // async read-back of the counters to refresh the host size bound later
-    KIFI_HIP(ctx, hipMemcpyAsync(vol->h_counters, cnt, sizeof(FrameCounters), hipMemcpyDeviceToHost, stream));
+    // k_merge_scan wrote the counters into host-mapped memory; the event marks
+    // when they may be read to refresh the host size bound
     KIFI_HIP(ctx, hipEventRecord(vol->size_event, stream));
     vol->size_event_pending = true;
     vol->size_bound += 8 * P;

@@ -211,7 +211,8 @@ struct kifi_volume {
     size_t           cap = 0;         // capacity of each buffer (voxels)
     size_t           size_bound = 0;  // host upper bound of the device-side size
     kifi::FrameCounters *d_counters = nullptr;   // device
-    kifi::FrameCounters *h_counters = nullptr;   // pinned host mirror
+    kifi::FrameCounters *h_counters = nullptr;   // pinned, device-mapped host mirror
+    kifi::FrameCounters *h_counters_dev = nullptr;   // its device address
     uint64_t        *d_size = nullptr;           // device-resident volume size
     kifi::FrameMats *d_mats = nullptr;           // matrices of the last integration
     hipEvent_t       size_event = nullptr;       // marks the last async size read-back

@@ -386,6 +386,7 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
         d_depth = p->d_depth;
     }
     kifi_status s = KIFI_OK;
+    bool tracked = false;     // icp_run wrote this frame's worldToEye
     if (w2e) {
         // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
@@ -407,7 +408,8 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
             ip.fallback_iters = 0;
         }
         s = icp_run(p->model, &p->work, d_depth, &p->cam, p->have_guess ? p->guess : nullptr,
-                    p->have_guess ? nullptr : guess_dev, &ip);
+                    p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e);
+        tracked = true;
         p->have_guess = false;
         if (s != KIFI_OK)
             return s;
@@ -419,7 +421,7 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
     // splatted inside the integration's voxel pass, then resolved
     if (w2e)
         store_w2e(w2e, p->d_w2e, st);         // the given worldToEye as is (Pipeline.cpp:35)
-    else
+    else if (!tracked)
         icp_w2e(&p->work, p->d_w2e, st);
     int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
     model_begin(p->model, p->vol, nullptr, p->d_w2e);
