@@ -16,8 +16,8 @@
 namespace kifi {
 
 constexpr int kThreads = 256;
-constexpr int kRsItems = 16;
-constexpr int kRsTile = kThreads * kRsItems;   // 4096 keys per radix tile
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kThreads * kRsItems;   // 2048 keys per radix tile
 constexpr int kMgTile = 4096;                  // merged positions per merge tile
 
 // ===================================================================== reset
@@ -141,8 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict_
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ keys,
                                                          const uint32_t *__restrict__ d_n, int shift,
-                                                         uint32_t *__restrict__ counts, int Tb,
-                                                         uint32_t *__restrict__ totals)
+                                                         uint32_t *__restrict__ counts, int Tb)
 {
     __shared__ uint32_t hist[256];
     const uint32_t n = *d_n;
@@ -162,21 +161,19 @@ __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ k
         }
     }
     __syncthreads();
-    const uint32_t c = hist[threadIdx.x];
-    counts[(size_t)threadIdx.x * Tb + blockIdx.x] = c;
-    if (c)
-        atomicAdd(&totals[threadIdx.x], c);
+    counts[(size_t)threadIdx.x * Tb + blockIdx.x] = hist[threadIdx.x];
 }
 
 // One workgroup per digit: exclusive scan of that digit's tile counts, offset
 // by the number of keys with smaller digits.
+// Per digit (one workgroup each): exclusive scan of the digit's tile counts,
+// in place, and the digit's total (no atomics; the downsweep scans the totals).
 __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ counts, int Tb,
-                                                      const uint32_t *__restrict__ totals)
+                                                      uint32_t *__restrict__ totals)
 {
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const int d = blockIdx.x;
-    const uint32_t t = threadIdx.x < (unsigned)d ? totals[threadIdx.x] : 0u;
-    uint32_t carry = block_reduce_sum<kThreads>(t, tmp);
+    uint32_t carry = 0;
     uint32_t *row = counts + (size_t)d * Tb;
     constexpr int IT = 4;
     for (int start = 0; start < Tb; start += kThreads * IT) {
@@ -197,13 +194,16 @@ __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ cou
         }
         carry += tot;
     }
+    if (threadIdx.x == 0)
+        totals[d] = carry;
 }
 
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__ in,
                                                            K *__restrict__ out,
                                                            const uint32_t *__restrict__ d_n, int shift,
-                                                           const uint32_t *__restrict__ offsets, int Tb)
+                                                           const uint32_t *__restrict__ offsets, int Tb,
+                                                           const uint32_t *__restrict__ totals)
 {
     __shared__ uint32_t wcnt[kThreads / kWave][256];
     __shared__ uint32_t dstart[256];
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     const int w = threadIdx.x / kWave, lane = __lane_id();
     for (int q = lane; q < 256; q += kWave)
         wcnt[w][q] = 0;
-    // Each wave ranks its own contiguous 1024-key sub-tile, item by item, so the
+    // Each wave ranks its own contiguous 64 x kRsItems-key sub-tile, item by item, so the
     // per-wave counters need no barrier (a wave's LDS operations are in order).
     K keys[kRsItems];
     uint32_t rank[kRsItems];
@@ -252,7 +252,9 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
         uint32_t tot;
         const uint32_t ex = block_exclusive_scan<kThreads>(acc, tmp, tot);
         dstart[d] = ex;
-        gbase[d] = offsets[(size_t)d * Tb + blockIdx.x] - ex;
+        // digit base: exclusive scan of the 256 digit totals
+        const uint32_t dbase = block_exclusive_scan<kThreads>(totals[d], tmp, tot);
+        gbase[d] = dbase + offsets[(size_t)d * Tb + blockIdx.x] - ex;
     }
     __syncthreads();
 #pragma unroll
@@ -820,10 +822,10 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
         const int shift = 8 * pass;
         uint32_t *tot = s.rs_totals + pass * 256;
         hipLaunchKernelGGL(k_rs_upsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
-                           &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
+                           &cnt->n_cand, shift, s.rs_counts, (int)Tb);
         hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kThreads), 0, stream, s.rs_counts, (int)Tb, tot);
         hipLaunchKernelGGL(k_rs_downsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, dst,
-                           &cnt->n_cand, shift, s.rs_counts, (int)Tb);
+                           &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
         K *t = src; src = dst; dst = t;
     }
     // unique (minus the maximum) -> E keys in candB
