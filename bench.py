@@ -20,6 +20,8 @@ trajectory, timed on this host.  "fps_with_h2d" repeats the timed frames handed
 over from host memory.  "learned_vs_plain" trains the twist-prediction network
 on this GPU (seeded pairs, bounded budget) and compares plain and learned
 initial poses on a 1000-frame fast trajectory rendered on the device.
+"precision_variant_2mm" times integration into a 2 mm, 64-bit-key volume of
+about 50 M active voxels (binary16 + u8 and binary32 storage).
 """
 import argparse
 import json
@@ -53,6 +55,10 @@ def parse():
     ap.add_argument("--fast-frames", type=int, default=1000, help="frames of the fast trajectory")
     ap.add_argument("--train-pairs", type=int, default=60000)
     ap.add_argument("--train-epochs", type=int, default=16)
+    ap.add_argument("--no-precision", action="store_true", help="skip the 2 mm / 64-bit-key variant at scale")
+    ap.add_argument("--precision-voxels", type=int, default=50_000_000,
+                    help="active voxels the 2 mm volume is grown to before its timed frames")
+    ap.add_argument("--precision-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain copy kernel of this size 20 times (HBM counter calibration)")
     return ap.parse_args()
@@ -210,6 +216,65 @@ def learned_section(mf, ctx, cam, args):
     }
 
 
+def precision_section(mf, ctx, cam, args):
+    """The precision variant at scale: the plane + spheres scene at 2 mm voxels
+    (res 4096, side 8.192 m, truncation 4 cm, 64-bit keys), binary16 + u8 and
+    binary32 storage.  Each volume is first grown to args.precision_voxels
+    active voxels by integrating seeded views about the identity pose (rendered
+    on the GPU); the timed steps then integrate the identity view, fresh noise
+    each step, into it: merge + TSDF update over tens of millions of voxels.
+    Frames are resident in HBM before timing."""
+    sc = mf.make_scene(mf.SCENE_PLANE_SPHERES, 1)
+    res, side, trunc, sk = 4096, 8.192, 0.04, 8
+    fbytes = W * H * 2
+    K = args.precision_steps
+    ident = np.eye(4, dtype=np.float32).reshape(16)
+    timed = ctx.device_upload(np.zeros((K, H, W), np.uint16))
+    grow = ctx.device_upload(np.zeros((H, W), np.uint16))
+    for k in range(K):
+        mf.render_device(ctx, sc, cam, np.eye(4), 7000 + k, out_ptr=timed.address(k * fbytes))
+    out = {"workload": "plane + 3 spheres, 512x424, 2 mm voxels (res 4096, side 8.192 m, trunc 4 cm), 64-bit keys; "
+                       "volume grown by seeded views about the identity, then %d identity views integrated" % K}
+    for name, storage, sv in (("f16_u8", mf.STORAGE_F16U8, 4), ("f32", mf.STORAGE_F32, 8)):
+        vol = mf.Volume(ctx, res, side, trunc, 64, storage)
+        f = 0
+        while f < 400:
+            pose = mf.trajectory_pose(mf.TRAJ_SWAY, 1, f)
+            mf.render_device(ctx, sc, cam, pose, 100 + f, out_ptr=grow.address(0))
+            vol.integrate_device(grow.address(0), cam, mf.invert_rigid(pose).astype(np.float32))
+            f += 1
+            if f % 8 == 0 and vol.size() >= args.precision_voxels:
+                break
+        ctx.synchronize()
+        v_start = int(vol.size())
+        tot0 = vol.totals()
+        ctx.timing_read(mf.STAGE_MERGE_UPDATE)
+        ctx.timing(True, [mf.STAGE_MERGE_UPDATE])
+        t0 = time.perf_counter()
+        for k in range(K):
+            vol.integrate_device(timed.address(k * fbytes), cam, ident)
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        ctx.timing(False)
+        tot1 = vol.totals()
+        mu_ms, mu_n = ctx.timing_read(mf.STAGE_MERGE_UPDATE)
+        launches = tot1["frames"] - tot0["frames"]
+        dv0, de, dv1 = (tot1[k] - tot0[k] for k in ("v_old", "new", "v_new"))
+        alg = ((sk + sv) * (dv0 + dv1) + sk * de + 2 * W * H * launches) / max(launches, 1)
+        gbs = alg / (mu_ms / mu_n * 1e-3) / 1e9 if mu_n else 0.0
+        out[name] = {
+            "voxels_before": v_start, "voxels_after": int(vol.size()), "growth_frames": f,
+            "ms_per_frame": round(1000.0 * elapsed / K, 4),
+            "merge_update_ms": round(mu_ms / mu_n, 4) if mu_n else None,
+            "alg_bytes_per_launch": int(alg),
+            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+        }
+        vol.close()
+    timed.free()
+    grow.free()
+    return out
+
+
 def calibrate_copy(mf, mib):
     ctx = mf.Context(0)
     nbytes = mib << 20
@@ -331,6 +396,7 @@ def main():
     pipe.close()
 
     learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
+    precision = None if args.no_precision else precision_section(mf, ctx, cam, args)
 
     icp_frames = st1["frames"] - st0["frames"]
     icp_iters = (st1["sum_iters"] - st0["sum_iters"]) / max(icp_frames, 1)
@@ -381,6 +447,7 @@ def main():
                              % (args.steps / diag["elapsed"]),
         "fps_with_h2d": round(fps_h2d, 3),
         "learned_vs_plain": learned,
+        "precision_variant_2mm": precision,
     }
     print(json.dumps(out))
 

@@ -143,6 +143,8 @@ typedef struct {
     float level_gate_growth; /* level l gates at dist_thresh x growth^l (1: the same    */
                              /* gate on every level; 2 widens the coarse levels for    */
                              /* large initial errors)                                  */
+    int   naive_sums;      /* 1: plain binary32 sums without compensation (precision   */
+                           /*    study only; default 0)                                */
 } kifi_icp_params;
 
 typedef struct {
@@ -184,6 +186,18 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
                                   const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
                                   const float synthEyeToWorld[16], int iterations, float out[16],
                                   kifi_icp_stats *stats);
+/* Precision study: the reduced point-to-plane normal equations of one projective
+ * association of the frame (host, mm) at eyeToWorld against the model, on
+ * pyramid level `level` (0..2) with gate dist_thresh (m), weights w = 1/d^4:
+ * out[0..20] = sum w J_i J_j for i >= j row by row (J = [p x n, n]),
+ * out[21..26] = sum w J_i r, out[27] = count, out[28] = sum r^2,
+ * out[29] = sum w r^2; r = n . (q - p).  The sums are the ones the tracker
+ * solves: binary32, compensated unless naive_sums, widened to FP64.  No
+ * counterpart in the reference (its ICP is point-to-point, ICP.cpp:35-265).
+ * Synchronous. */
+kifi_status kifi_icp_normal_equations(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                                      const double eyeToWorld[16], int level, float dist_thresh, int naive_sums,
+                                      double out[30]);
 /* Horn's closed-form rigid alignment of n correspondences (ICP.cpp:166-264):
  * compensated centroid and cross-covariance reductions, FP64 4x4 eigen solve.
  * out = the transform mapping src onto dst.  Synchronous. */

@@ -29,7 +29,9 @@ enum {
 enum {
     KIFI_TRAJ_STATIC = 0, /* identity pose                                         */
     KIFI_TRAJ_CIRCLE = 1, /* r = 1.2 m circle, 1.5 cm and 1 deg per frame, jitter  */
-    KIFI_TRAJ_FAST = 2    /* random walk, steps U[2,10] cm and U[2,15] deg         */
+    KIFI_TRAJ_FAST = 2,   /* random walk, steps U[2,10] cm and U[2,15] deg         */
+    KIFI_TRAJ_SWAY = 3    /* independent views about the identity: U[-5,5] cm per  */
+                          /* axis, yaw/pitch/roll U[-2,2] deg; frame 0 = identity  */
 };
 
 typedef struct kifi_scene_s {

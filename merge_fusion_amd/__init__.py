@@ -23,7 +23,7 @@ SYNTH_LIB_PATH = os.path.join(_PKG, "libkifi_synth.so")
 KIFI_OK, KIFI_ERR_ARG, KIFI_ERR_HIP, KIFI_ERR_NOMEM, KIFI_ERR_CAPACITY, KIFI_ERR_STATE = range(6)
 STORAGE_F32, STORAGE_F16U8 = 0, 1
 SCENE_PLANE_SPHERES, SCENE_ROOM = 0, 1
-TRAJ_STATIC, TRAJ_CIRCLE, TRAJ_FAST = 0, 1, 2
+TRAJ_STATIC, TRAJ_CIRCLE, TRAJ_FAST, TRAJ_SWAY = 0, 1, 2, 3
 
 HIP_EXPORTS = [
     "kifi_ctx_create", "kifi_ctx_destroy", "kifi_last_error", "kifi_ctx_synchronize", "kifi_ctx_stream",
@@ -31,7 +31,7 @@ HIP_EXPORTS = [
     "kifi_volume_download", "kifi_volume_upload", "kifi_integrate", "kifi_integrate_device",
     "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_bin",
     "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
-    "kifi_model_download", "kifi_icp_align", "kifi_horn_align", "kifi_pipeline_create",
+    "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_memcpy_to_device",
@@ -69,7 +69,7 @@ class IcpParams(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("levels", ctypes.c_int), ("iters", ctypes.c_int * 3),
                 ("dist_thresh", ctypes.c_float), ("eps_rot", ctypes.c_float), ("eps_trans", ctypes.c_float),
                 ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int),
-                ("level_gate_growth", ctypes.c_float)]
+                ("level_gate_growth", ctypes.c_float), ("naive_sums", ctypes.c_int)]
 
     @classmethod
     def default(cls):
@@ -142,6 +142,8 @@ def load_hip():
             "kifi_model_download": ([vp, f32p, f32p], i32),
             "kifi_icp_align": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, ctypes.POINTER(IcpParams), f64p,
                                 ctypes.POINTER(IcpStats)], i32),
+            "kifi_icp_normal_equations": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, i32, ctypes.c_float, i32,
+                                           f64p], i32),
             "kifi_horn_align": ([vp, f32p, f32p, sz, f32p], i32),
             "kifi_pipeline_create": ([vp, ctypes.POINTER(VolumeDesc), ctypes.POINTER(Intrinsics),
                                       ctypes.POINTER(IcpParams), ctypes.POINTER(vp)], i32),
@@ -522,6 +524,18 @@ class Model:
                                                     ctypes.byref(p), _f64p(out), ctypes.byref(st)),
                        "kifi_icp_align")
         return out, st.as_dict()
+
+    def normal_equations(self, depth_mm, e2w, level=0, dist_thresh=0.1, naive=False):
+        """One association at the eye->world pose e2w on `level`: the 30 reduced sums
+        (21 sum w J_i J_j for i >= j row by row, 6 sum w J_i r, count, sum r^2,
+        sum w r^2) as the tracker sums them, in float64."""
+        d = np.ascontiguousarray(depth_mm, np.uint16)
+        g = np.ascontiguousarray(e2w, np.float64).reshape(16)
+        out = np.zeros(30, np.float64)
+        self.ctx.check(self.ctx._lib.kifi_icp_normal_equations(self._h, d.ctypes.data, ctypes.byref(self.cam), _f64p(g),
+                                                               int(level), float(dist_thresh), int(bool(naive)),
+                                                               _f64p(out)), "kifi_icp_normal_equations")
+        return out
 
 
 def horn_align(ctx, src, dst):

@@ -276,12 +276,19 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
     e = (a - (s - bp)) + (b - bp);
 }
 
+// COMP = false: plain binary32 addition (the naive reduction of the precision
+// study); the compensation term stays zero.
+template <bool COMP = true>
 __device__ __forceinline__ void cadd(CSum &x, float s2, float c2)
 {
-    float t, e;
-    two_sum(x.s, s2, t, e);
-    x.s = t;
-    x.c = x.c + c2 + e;
+    if constexpr (COMP) {
+        float t, e;
+        two_sum(x.s, s2, t, e);
+        x.s = t;
+        x.c = x.c + c2 + e;
+    } else {
+        x.s = x.s + s2;
+    }
 }
 
 // Compensated block reduction through an LDS transpose: every thread stores its
@@ -298,7 +305,7 @@ struct CsumLds {
     CSum result[NV];
 };
 
-template <int NV>
+template <int NV, bool COMP = true>
 __device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds)
 {
     static_assert(kSlices * NV <= kIT, "one thread per (slice, value)");
@@ -312,7 +319,7 @@ __device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds
         CSum a = lds.rows[(sl * kRows) * NV + j];
         for (int r = 1; r < kRows; ++r) {
             const CSum b = lds.rows[(sl * kRows + r) * NV + j];
-            cadd(a, b.s, b.c);
+            cadd<COMP>(a, b.s, b.c);
         }
         lds.slices[sl * NV + j] = a;
     }
@@ -321,7 +328,7 @@ __device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds
         CSum a = lds.slices[threadIdx.x];
 #pragma unroll
         for (int sl = 1; sl < kSlices; ++sl)
-            cadd(a, lds.slices[sl * NV + threadIdx.x].s, lds.slices[sl * NV + threadIdx.x].c);
+            cadd<COMP>(a, lds.slices[sl * NV + threadIdx.x].s, lds.slices[sl * NV + threadIdx.x].c);
         lds.result[threadIdx.x] = a;
     }
     __syncthreads();
@@ -335,7 +342,7 @@ struct IcpNV;
 template <> struct IcpNV<KIFI_ICP_POINT_TO_PLANE> { static constexpr int nv = 30; };
 template <> struct IcpNV<KIFI_ICP_HORN> { static constexpr int nv = 17; };
 
-template <int MODE>
+template <int MODE, bool COMP>
 __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
                            const ModelPose *mpose, CsumLds<IcpNV<MODE>::nv> &lds);
 
@@ -346,12 +353,12 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 // workgroup acquires (one L2 invalidate per launch).  A full __threadfence()
 // here would invalidate the XCD's L2 once per workgroup and make every other
 // workgroup re-fetch the model maps from HBM.
-template <int MODE, int NV>
+template <int MODE, bool COMP, int NV>
 __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, CSum *partials, IcpState *st, int level,
                                             const IcpSolveParams &sp, const ModelPose *mpose)
 {
     __shared__ bool last;
-    block_csum<NV>(acc, lds);
+    block_csum<NV, COMP>(acc, lds);
     if (threadIdx.x < NV)
         partials[(size_t)blockIdx.x * NV + threadIdx.x] = lds.result[threadIdx.x];
     __syncthreads();     // the stores of threads 0..NV-1 precede thread 0's release
@@ -364,12 +371,12 @@ __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, C
     if (!last)
         return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // see every workgroup's partial
-    icp_finish<MODE>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
+    icp_finish<MODE, COMP>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
     if (threadIdx.x == 0)
         st->ticket = 0;
 }
 
-template <int MODE>
+template <int MODE, bool COMP>
 __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, const ModelPose *__restrict__ mpose,
                                                    const float4 *__restrict__ mpts,
                                                    const float4 *__restrict__ mnrm,
@@ -416,28 +423,28 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
 #pragma unroll
                                     for (int j = 0; j <= i; ++j)
                                     {
-                                        cadd(acc[t], wgt * J[i] * J[j], 0.0f);
+                                        cadd<COMP>(acc[t], wgt * J[i] * J[j], 0.0f);
                                         ++t;
                                     }
 #pragma unroll
                                 for (int i = 0; i < 6; ++i)
-                                    cadd(acc[21 + i], wgt * J[i] * r, 0.0f);
-                                cadd(acc[27], 1.0f, 0.0f);
-                                cadd(acc[28], r * r, 0.0f);
-                                cadd(acc[29], wgt * (r * r), 0.0f);
+                                    cadd<COMP>(acc[21 + i], wgt * J[i] * r, 0.0f);
+                                cadd<COMP>(acc[27], 1.0f, 0.0f);
+                                cadd<COMP>(acc[28], r * r, 0.0f);
+                                cadd<COMP>(acc[29], wgt * (r * r), 0.0f);
                             } else {
                                 const float a[3] = { p[0] - mpose->cam[0], p[1] - mpose->cam[1], p[2] - mpose->cam[2] };
                                 const float b[3] = { Q.x - mpose->cam[0], Q.y - mpose->cam[1], Q.z - mpose->cam[2] };
 #pragma unroll
                                 for (int i = 0; i < 3; ++i) {
-                                    cadd(acc[i], a[i], 0.0f);
-                                    cadd(acc[3 + i], b[i], 0.0f);
+                                    cadd<COMP>(acc[i], a[i], 0.0f);
+                                    cadd<COMP>(acc[3 + i], b[i], 0.0f);
 #pragma unroll
                                     for (int j = 0; j < 3; ++j)
-                                        cadd(acc[6 + 3 * i + j], a[i] * b[j], 0.0f);
+                                        cadd<COMP>(acc[6 + 3 * i + j], a[i] * b[j], 0.0f);
                                 }
-                                cadd(acc[15], 1.0f, 0.0f);
-                                cadd(acc[16], (dx * dx + dy * dy) + dz * dz, 0.0f);
+                                cadd<COMP>(acc[15], 1.0f, 0.0f);
+                                cadd<COMP>(acc[16], (dx * dx + dy * dy) + dz * dz, 0.0f);
                             }
                         }
                     }
@@ -445,7 +452,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
             }
         }
     }
-    icp_publish<MODE_OF_KERNEL>(acc, lds, partials, st, level, sp, mpose);
+    icp_publish<MODE_OF_KERNEL, COMP>(acc, lds, partials, st, level, sp, mpose);
 }
 
 // ICP::AlignStep association (ICP.cpp:66-131): every 4th column, back-projected
@@ -526,7 +533,7 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
             }
         }
     }
-    icp_publish<MODE_OF_KERNEL>(acc, lds, partials, st, level, sp, mpose);
+    icp_publish<MODE_OF_KERNEL, true>(acc, lds, partials, st, level, sp, mpose);
 }
 
 // ---------------------------------------------------------------- FP64 solve
@@ -688,7 +695,7 @@ __device__ void horn_increment(const double S[9], const double ms[3], const doub
 // Reduce the block partials (compensated), solve in FP64, compose the pose.
 // Run by the whole last workgroup of an association kernel; the partials of the
 // other workgroups are read with device-scope loads after the ticket.
-template <int MODE>
+template <int MODE, bool COMP>
 __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int level, const IcpSolveParams &sp,
                            const ModelPose *mpose, CsumLds<IcpNV<MODE>::nv> &lds)
 {
@@ -703,14 +710,16 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             const CSum p = partials[(size_t)b * NV + j];
-            cadd(acc[j], p.s, p.c);
+            cadd<COMP>(acc[j], p.s, p.c);
         }
-    block_csum<NV>(acc, lds);
+    block_csum<NV, COMP>(acc, lds);
     if (threadIdx.x != 0)
         return;
     double v[NV];
-    for (int j = 0; j < NV; ++j)
+    for (int j = 0; j < NV; ++j) {
         v[j] = (double)lds.result[j].s + (double)lds.result[j].c;
+        st->sums[j] = v[j];
+    }
     // count, sum r^2 and (point-to-plane) sum r^2 / d^4: the residual scaled to
     // 1 m depth by the sensor noise model sigma ~ z^2
     constexpr int IC = MODE == KIFI_ICP_POINT_TO_PLANE ? 27 : 15;
@@ -1189,9 +1198,14 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
                                    w->d_state, (CSum *)w->partials, l, sp);
             } else {
-                hipLaunchKernelGGL(k_icp_assoc<KIFI_ICP_POINT_TO_PLANE>, dim3(nb), dim3(kIT), 0, st, lv, m->mp,
-                                   m->d_pose, m->d_pts, m->d_nrm, w->d_state, gate,
-                                   (CSum *)w->partials, l, sp);
+                if (ip->naive_sums)
+                    hipLaunchKernelGGL((k_icp_assoc<KIFI_ICP_POINT_TO_PLANE, false>), dim3(nb), dim3(kIT), 0, st,
+                                       lv, m->mp, m->d_pose, m->d_pts, m->d_nrm, w->d_state, gate,
+                                       (CSum *)w->partials, l, sp);
+                else
+                    hipLaunchKernelGGL((k_icp_assoc<KIFI_ICP_POINT_TO_PLANE, true>), dim3(nb), dim3(kIT), 0, st,
+                                       lv, m->mp, m->d_pose, m->d_pts, m->d_nrm, w->d_state, gate,
+                                       (CSum *)w->partials, l, sp);
             }
         }
     }

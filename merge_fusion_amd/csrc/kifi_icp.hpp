@@ -56,6 +56,7 @@ struct IcpState {
     float  alt_rms;
     int    alt_inliers;
     unsigned ticket;        // workgroups done in the current association launch
+    double sums[30];        // reduced sums of the last association (icp_finish layout)
     // running totals over frames
     long long sum_iters;
     long long frames_tracked;

@@ -480,22 +480,29 @@ struct UpdateParams {
     const uint16_t *depth;
 };
 
-// Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356).  Returns true and
-// the clamped signed distance when the voxel receives an update.
+// Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356), in two halves so
+// that a lane can issue the depth gathers of all its voxels before it uses one.
+// voxel_pixel: the depth pixel the voxel reads (:285-319), or -1 when it is
+// outside the image.
 template <typename K>
-__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const Mat4 &w2c, float &sd_out)
+__device__ __forceinline__ float voxel_centre(uint32_t c, const UpdateParams &p)
+{
+    return (float)c * p.vl + p.c0;                                 // Volume.cpp:75-78
+}
+
+template <typename K>
+__device__ __forceinline__ int voxel_pixel(K key, const UpdateParams &p, const Mat4 &w2c)
 {
     uint32_t x, y, z;
     unpack_key<K>(key, x, y, z);
-    float k[4] = { (float)x * p.vl + p.c0, (float)y * p.vl + p.c0, (float)z * p.vl + p.c0,
-                   1.0f };                                         // Volume.cpp:75-78
+    float k[4] = { voxel_centre<K>(x, p), voxel_centre<K>(y, p), voxel_centre<K>(z, p), 1.0f };
     mat_vec(w2c, k, k);                                            // :290-293
     const float kw = k[3];
     // |x/w| or |y/w| above 1.01 lands outside [0,W) x [0,H) whatever the rounding,
     // so such voxels (most of the volume) skip the divisions; NaN passes on
     const float lim = 1.01f * fabsf(kw);
     if (fabsf(k[0]) > lim || fabsf(k[1]) > lim)
-        return false;
+        return -1;
     // :305-315 divides all four lanes by w and keeps the voxel only if
     // (z/w)*0+0 and (w/w)*0+0 are in range, i.e. z/w is finite and w is finite
     // and nonzero; that needs no division unless |w| is tiny or |z| huge
@@ -504,15 +511,36 @@ __device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const
     float u = k[0] / kw, v = k[1] / kw;
     u = u * p.ndcU + p.ndcU;                                       // :310-313
     v = v * p.ndcV + p.ndcV;
-    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zok && wok) {
-        const float d = (float)p.depth[(size_t)(p.H - (unsigned)v - 1) * p.W + (unsigned)u] * 0.001f;
-        const float sd = d - kw;                                   // :319-320
-        if (d > 0.0f && sd >= -p.trunc) {                          // :322
-            sd_out = p.trunc < sd ? p.trunc : sd;                  // std::min, :323
-            return true;
-        }
+    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zok && wok)
+        return (int)((p.H - (unsigned)v - 1) * (unsigned)p.W + (unsigned)u);   // :319
+    return -1;
+}
+
+// voxel_update: with the gathered depth (mm) of the voxel's pixel, true and the
+// clamped signed distance when the voxel receives an update (:319-323).  w is
+// recomputed: the same products and sums in the same order as mat_vec's row 3.
+template <typename K>
+__device__ __forceinline__ bool voxel_update(K key, uint32_t dmm, const UpdateParams &p, const Mat4 &w2c,
+                                             float &sd_out)
+{
+    uint32_t x, y, z;
+    unpack_key<K>(key, x, y, z);
+    const float cx = voxel_centre<K>(x, p), cy = voxel_centre<K>(y, p), cz = voxel_centre<K>(z, p);
+    const float kw = (w2c.m[3] * cx + w2c.m[7] * cy) + (w2c.m[11] * cz + w2c.m[15] * 1.0f);
+    const float d = (float)dmm * 0.001f;
+    const float sd = d - kw;                                       // :319-320
+    if (d > 0.0f && sd >= -p.trunc) {                              // :322
+        sd_out = p.trunc < sd ? p.trunc : sd;                      // std::min, :323
+        return true;
     }
     return false;
+}
+
+template <typename K>
+__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const Mat4 &w2c, float &sd_out)
+{
+    const int pix = voxel_pixel<K>(key, p, w2c);
+    return pix >= 0 && voxel_update<K>(key, p.depth[pix], p, w2c, sd_out);
 }
 
 struct VoxF32 { float sum, weight; };
@@ -608,6 +636,12 @@ __device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, con
         }
 }
 
+// LDS index of the shift histogram: one pad word per 32 keeps a lane's 16
+// consecutive entries (the scan) and a wave's consecutive entries (the old-voxel
+// pass) on distinct banks
+__device__ __forceinline__ uint32_t hpad(uint32_t i) { return i + (i >> 5); }
+constexpr int kHistWords = kMgTile + kMgTile / 32;
+
 template <typename K, typename V, bool SPLAT>
 __global__ __launch_bounds__(kThreads) void k_merge_update(
     const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B,
@@ -616,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     SplatParams sp)
 {
     __shared__ K sA[kMgTile];
-    __shared__ K sB[kMgTile];          // the tile's new keys, later the A-side shift counts
+    __shared__ uint32_t h[kHistWords];   // per A position: kept new keys sorting just below it
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
@@ -631,71 +665,91 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
     const uint64_t obase = (uint64_t)blockIdx.x * kMgTile - tile_base_dups[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads)
-        sA[i] = A[mt.a0 + i];
-    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads)
-        sB[j] = B[mt.b0 + j];
-    __syncthreads();
-    // new keys already in the volume are dropped (A wins ties; the A element just
-    // before the tile can match the first B element); lb = #A keys below the key
-    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
+    // Every global load of the tile is issued up front: old keys, new keys, then
+    // old values, which stay in registers until the old-voxel pass.  A lane has
+    // up to 3 x PER loads in flight, where a loop of load-use iterations would
+    // hold one.  Element i of either range belongs to lane i % kThreads, so the
+    // loads coalesce and a tile's few new keys spread over many lanes.
     constexpr int PER = kMgTile / kThreads;
-    K kept[PER];
-    uint32_t lbs[PER];
-    uint32_t fl = 0, c = 0;
+    K ak[PER], bk[PER];
+    V av[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t j = threadIdx.x * PER + q;
-        bool keep = false;
-        lbs[q] = 0;
-        if (j < mt.nb) {
-            const K b = sB[j];
-            const uint32_t lb = lds_lower_bound(sA, mt.na, b);
-            keep = !((lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b));
-            kept[q] = b;
-            lbs[q] = lb;
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na)
+            ak[q] = A[mt.a0 + i];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.nb)
+            bk[q] = B[mt.b0 + i];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na)
+            av[q] = Av[mt.a0 + i];
+    }
+    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na)
+            sA[i] = ak[q];
+    }
+    __syncthreads();
+    // New keys, kThreads per round: a key already in the volume is dropped (A wins
+    // ties; the A element just before the tile can match the first new key).  A
+    // kept key goes to obase + (kept keys before it) + (A keys below it).
+    uint32_t lbq[PER], keepm = 0, nbk = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        lbq[q] = 0;
+        if ((uint32_t)q * kThreads < mt.nb) {                     // uniform over the block
+            const uint32_t j = q * kThreads + threadIdx.x;
+            bool keep = false;
+            if (j < mt.nb) {
+                const K b = bk[q];
+                const uint32_t lb = lds_lower_bound(sA, mt.na, b);
+                keep = !((lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b));
+                lbq[q] = lb;
+            }
+            uint32_t kept;
+            const uint32_t pos = nbk + block_exclusive_scan<kThreads>((uint32_t)keep, tmp, kept);
+            nbk += kept;
+            keepm |= (uint32_t)keep << q;
+            if (keep) {
+                // new voxel: Voxel() default, then update
+                const uint64_t o = obase + pos + lbq[q];
+                V val{};
+                float s;
+                if (o < upd_end && voxel_sample<K>(bk[q], up, w2c, s))
+                    voxel_apply(val, s);
+                O[o] = bk[q];
+                Ov[o] = val;
+                if (SPLAT)
+                    splat_voxel<K, V>(bk[q], val, o, sp, w2e);
+            }
         }
-        fl |= (uint32_t)keep << q;
-        c += keep;
     }
-    uint32_t nbk;
-    uint32_t pos = block_exclusive_scan<kThreads>(c, tmp, nbk);   // ends with a barrier
+    // shift of A[i] = kept new keys below it = kept keys with lb <= i: an LDS
+    // histogram of the lbs and an inclusive scan over the tile's A positions.
+    // nbk is uniform; a tile that inserts nothing copies its old voxels in place.
+    if (nbk > 0) {
+        for (uint32_t i = threadIdx.x; i < kHistWords; i += kThreads)
+            h[i] = 0;
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PER; ++q)
-        if (fl >> q & 1u)
-            sB[pos++] = kept[q];
-    __syncthreads();
-    // new voxels (spread over all threads): Voxel() default, then update
-    for (uint32_t j = threadIdx.x; j < nbk; j += kThreads) {
-        const K k = sB[j];
-        const uint64_t o = obase + j + lds_lower_bound(sA, mt.na, k);
-        V val{};
-        float s;
-        if (o < upd_end && voxel_sample<K>(k, up, w2c, s))
-            voxel_apply(val, s);
-        O[o] = k;
-        Ov[o] = val;
-        if (SPLAT)
-            splat_voxel<K, V>(k, val, o, sp, w2e);
-    }
-    __syncthreads();
-    // old voxels: shift of A[i] = #kept B keys below it = #kept with lb <= i, by an
-    // LDS histogram of the lbs and an inclusive scan over the tile's A positions
-    uint32_t *h = reinterpret_cast<uint32_t *>(sB);
-    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads)
-        h[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; ++q)
-        if ((fl >> q & 1u) && lbs[q] < mt.na)
-            atomicAdd(&h[lbs[q]], 1u);
-    __syncthreads();
-    {
+        for (int q = 0; q < PER; ++q)
+            if ((keepm >> q & 1u) && lbq[q] < mt.na)
+                atomicAdd(&h[hpad(lbq[q])], 1u);
+        __syncthreads();
         uint32_t loc[PER], sum = 0;
         const uint32_t base = threadIdx.x * PER;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            loc[q] = base + q < mt.na ? h[base + q] : 0u;
+            loc[q] = h[hpad(base + q)];
             sum += loc[q];
         }
         uint32_t total;
@@ -703,23 +757,41 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             run += loc[q];
-            if (base + q < mt.na)
-                h[base + q] = run;
+            h[hpad(base + q)] = run;
         }
+        __syncthreads();
     }
-    __syncthreads();
-    // old voxels keep their value (first range wins), then update
-    for (uint32_t i = threadIdx.x; i < mt.na; i += kThreads) {
-        const K k = sA[i];
-        const uint64_t o = obase + i + h[i];
-        V val = Av[mt.a0 + i];
-        float s;
-        if (o < upd_end && voxel_sample<K>(k, up, w2c, s))
-            voxel_apply(val, s);
-        O[o] = k;
-        Ov[o] = val;
-        if (SPLAT)
-            splat_voxel<K, V>(k, val, o, sp, w2e);
+    // old voxels keep their value (first range wins), then update: project all
+    // of the lane's voxels, gather their depths together, then finish each
+    int pix[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        pix[q] = -1;
+        if (i < mt.na && obase + i + (nbk ? h[hpad(i)] : 0u) < upd_end)
+            pix[q] = voxel_pixel<K>(sA[i], up, w2c);
+    }
+    // unconditional loads (pixel 0 stands in for "none"): a load under a branch
+    // would be waited for before its value merges, one gather at a time
+    uint32_t dmm[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+        dmm[q] = up.depth[pix[q] >= 0 ? pix[q] : 0];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na) {
+            const K k = sA[i];
+            const uint64_t o = obase + i + (nbk ? h[hpad(i)] : 0u);
+            V val = av[q];
+            float s;
+            if (pix[q] >= 0 && voxel_update<K>(k, dmm[q], up, w2c, s))
+                voxel_apply(val, s);
+            O[o] = k;
+            Ov[o] = val;
+            if (SPLAT)
+                splat_voxel<K, V>(k, val, o, sp, w2e);
+        }
     }
 }
 

@@ -65,6 +65,7 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->fallback_rms = 0.0f;
     p->level_gate_growth = 1.0f;
     p->fallback_iters = 0;
+    p->naive_sums = 0;
     return KIFI_OK;
 }
 
@@ -144,12 +145,10 @@ static void fill_stats(const IcpState &s, kifi_icp_stats *st)
     st->fallbacks = s.fallbacks;
 }
 
-kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
-                           const double guess[16], const kifi_icp_params *params, double out[16],
-                           kifi_icp_stats *stats)
+// Host frame -> one ICP run from `guess`; the final device state in *hs.
+static kifi_status align_host(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                              const double guess[16], const kifi_icp_params *params, IcpState &hs)
 {
-    if (!m || !depth_mm || !guess || !params || !out)
-        return KIFI_ERR_ARG;
     kifi_ctx *ctx = m->ctx;
     kifi_status s = bad_cam(ctx, cam);
     if (s != KIFI_OK)
@@ -174,7 +173,6 @@ kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_i
     }
     icp_set_pose(&w, guess, nullptr, ctx->stream);   // previous pose for a fallback = the guess
     s = icp_run(m, &w, d_depth, cam, guess, nullptr, params);
-    IcpState hs;
     if (s == KIFI_OK) {
         e = hipMemcpyAsync(&hs, w.d_state, sizeof hs, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess)
@@ -182,14 +180,56 @@ kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_i
         if (e != hipSuccess)
             s = hip_error(ctx, e, "kifi_icp_align");
     }
+    (void)hipFree(d_depth);
+    icp_free(&w);
+    return s;
+}
+
+kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const double guess[16], const kifi_icp_params *params, double out[16],
+                           kifi_icp_stats *stats)
+{
+    if (!m || !depth_mm || !guess || !params || !out)
+        return KIFI_ERR_ARG;
+    IcpState hs;
+    const kifi_status s = align_host(m, depth_mm, cam, guess, params, hs);
     if (s == KIFI_OK) {
         for (int i = 0; i < 16; ++i)
             out[i] = hs.pose[i];
         if (stats)
             fill_stats(hs, stats);
     }
-    (void)hipFree(d_depth);
-    icp_free(&w);
+    return s;
+}
+
+kifi_status kifi_icp_normal_equations(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                                      const double eyeToWorld[16], int level, float dist_thresh, int naive_sums,
+                                      double out[30])
+{
+    if (!m || !depth_mm || !eyeToWorld || !out)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = m->ctx;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    if (cam->width != m->cam.width || cam->height != m->cam.height)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_icp_normal_equations: frame size differs from the model's");
+    if (level < 0 || level > 2 || !(dist_thresh > 0.0f))
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_icp_normal_equations: level must be 0..2 and the gate positive");
+    // one association on `level`: the schedule runs that level once and no other
+    kifi_icp_params ip;
+    kifi_icp_params_default(&ip);
+    ip.levels = level + 1;
+    ip.iters[0] = ip.iters[1] = ip.iters[2] = 0;
+    ip.iters[level] = 1;
+    ip.dist_thresh = dist_thresh;
+    ip.min_inliers = 0;
+    ip.naive_sums = naive_sums ? 1 : 0;
+    IcpState hs;
+    s = align_host(m, depth_mm, cam, eyeToWorld, &ip, hs);
+    if (s == KIFI_OK)
+        for (int j = 0; j < 30; ++j)
+            out[j] = hs.sums[j];
     return s;
 }
 

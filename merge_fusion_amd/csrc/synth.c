@@ -261,5 +261,21 @@ int kifi_trajectory_pose(int traj, uint64_t seed, int frame, double *m)
         pose_from(yaw, pitch, 0.0, p, m);
         return 0;
     }
+    if (traj == KIFI_TRAJ_SWAY) {
+        /* independent seeded views about the identity pose; frame 0 is the identity */
+        const double deg = PI / 180.0;
+        double p[3] = { 0, 0, 0 };
+        if (frame == 0) {
+            pose_from(0, 0, 0, p, m);
+            return 0;
+        }
+        rng_t g = { kifi_splitmix64(seed * 0x9e3779b97f4a7c15ull + (uint64_t)frame) };
+        for (int i = 0; i < 3; ++i)
+            p[i] = rng_range(&g, -0.05, 0.05);
+        const double yaw = rng_range(&g, -2.0, 2.0) * deg, pitch = rng_range(&g, -2.0, 2.0) * deg,
+                     roll = rng_range(&g, -2.0, 2.0) * deg;
+        pose_from(yaw, pitch, roll, p, m);
+        return 0;
+    }
     return -1;
 }

@@ -12,7 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-STORE_F32, STORE_F16U8 = 0, 1
+STORE_F32, STORE_F16U8, STORE_F64 = 0, 1, 2
 
 
 class Camera(ctypes.Structure):
@@ -159,13 +159,14 @@ class Volume:
     def read(self):
         n = self.size()
         keys = np.zeros(n, np.uint64)
-        vals = np.zeros((n, 2), np.float32) if self.storage == STORE_F32 else np.zeros(n, np.uint32)
+        vals = (np.zeros((n, 2), np.float32) if self.storage == STORE_F32 else
+                np.zeros((n, 2), np.float64) if self.storage == STORE_F64 else np.zeros(n, np.uint32))
         lib().orc_volume_read(self.h, u64p(keys), vals.ctypes.data, n)
         return keys, vals
 
     def write(self, keys, vals):
         k = np.ascontiguousarray(keys, np.uint64)
-        v = np.ascontiguousarray(vals, np.float32 if self.storage == STORE_F32 else np.uint32)
+        v = np.ascontiguousarray(vals, {STORE_F32: np.float32, STORE_F64: np.float64}.get(self.storage, np.uint32))
         if lib().orc_volume_write(self.h, u64p(k), v.ctypes.data, k.shape[0]):
             raise MemoryError
 
@@ -283,6 +284,12 @@ def _icp_sigs(L):
     L.orc_icp_level_f64.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P, F64P,
                                     i32, d, d, d, F64P]
     L.orc_icp_level_f64.restype = i32
+    L.orc_icp_sums_f64.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P, F64P, d,
+                                   F64P]
+    L.orc_icp_sums_f64.restype = None
+    L.orc_icp_sums_f32terms.argtypes = [F32P, i32, F32P, F32P, ctypes.POINTER(Camera), F32P, F32P, ctypes.c_float,
+                                        F64P]
+    L.orc_icp_sums_f32terms.restype = None
 
 
 def model_predict(vol, world_to_eye, cam, dmax=None, znear=0.1, disc=0.15):
@@ -347,6 +354,44 @@ def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dis
                                 st.ctypes.data_as(F64P))
         total += n
     return pose, total, st[0]
+
+
+def icp_sums_f64(depth_mm, cam, pts, nrm, model_w2e, e2w, level=0, dist_thresh=0.1):
+    """FP64 normal equations of one association of pyramid level `level` at the
+    eye->world pose e2w: 30 sums in the layout of kifi_icp_normal_equations."""
+    L = lib()
+    if not hasattr(L, "_icp_ready"):
+        _icp_sigs(L)
+        L._icp_ready = True
+    d = depth_pyramid(depth_mm, level + 1)[level]
+    s = 1.0 / (1 << level)
+    W2E = np.ascontiguousarray(model_w2e, np.float64).reshape(16)
+    pose = np.ascontiguousarray(e2w, np.float64).reshape(16)
+    out = np.zeros(30, np.float64)
+    c = camera(cam)
+    L.orc_icp_sums_f64(f32p(np.ascontiguousarray(d)), d.shape[1], d.shape[0], cam.fx * s, cam.fy * s, cam.cx * s,
+                       cam.cy * s, f32p(np.ascontiguousarray(pts, np.float32)),
+                       f32p(np.ascontiguousarray(nrm, np.float32)), ctypes.byref(c), W2E.ctypes.data_as(F64P),
+                       pose.ctypes.data_as(F64P), dist_thresh, out.ctypes.data_as(F64P))
+    return out
+
+
+def icp_sums_f32terms(depth_mm, cam, pts, nrm, model_w2e, e2w, level=0, dist_thresh=0.1):
+    """The binary32 terms of the GPU's association (pose rounded to binary32 as the
+    GPU does), summed in double: the reference for the GPU's summation error."""
+    L = lib()
+    if not hasattr(L, "_icp_ready"):
+        _icp_sigs(L)
+        L._icp_ready = True
+    d = depth_pyramid(depth_mm, level + 1)[level]
+    w2e = np.ascontiguousarray(model_w2e, np.float32).reshape(16)
+    pose_f = np.ascontiguousarray(np.asarray(e2w, np.float64).reshape(16).astype(np.float32))
+    out = np.zeros(30, np.float64)
+    c = camera(cam)
+    L.orc_icp_sums_f32terms(f32p(np.ascontiguousarray(d)), level, f32p(np.ascontiguousarray(pts, np.float32)),
+                            f32p(np.ascontiguousarray(nrm, np.float32)), ctypes.byref(c), f32p(w2e), f32p(pose_f),
+                            dist_thresh, out.ctypes.data_as(F64P))
+    return out
 
 
 def icp_align_ref(depth_mm, cam, model_pts4, model_e2w, guess_e2w, iters=4):

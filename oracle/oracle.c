@@ -181,13 +181,15 @@ float orc_voxel_distance_f16u8(uint32_t p)
 
 static size_t value_bytes(int storage)
 {
-    return storage == ORC_STORE_F32 ? 8 : 4;
+    return storage == ORC_STORE_F32 ? 8 : (storage == ORC_STORE_F64 ? 16 : 4);
 }
 
 static float vox_weight(const void *values, size_t i, int storage)
 {
     if (storage == ORC_STORE_F32)
         return ((const float *)values)[2 * i + 1];
+    if (storage == ORC_STORE_F64)
+        return (float)((const double *)values)[2 * i + 1];
     return (float)((((const uint32_t *)values)[i] >> 16) & 0xffu);
 }
 
@@ -195,6 +197,10 @@ static float vox_distance(const void *values, size_t i, int storage)
 {
     if (storage == ORC_STORE_F32)
         return orc_voxel_distance_f32((const float *)values + 2 * i);
+    if (storage == ORC_STORE_F64) {
+        const double *sw = (const double *)values + 2 * i;
+        return (float)(sw[0] / sw[1]);
+    }
     return orc_voxel_distance_f16u8(((const uint32_t *)values)[i]);
 }
 
@@ -520,7 +526,7 @@ size_t orc_integrate(orc_volume *v, const uint16_t *mm, const orc_camera *cam,
     size_t cap = v->size + E;
     uint64_t *nk = (uint64_t *)malloc((cap ? cap : 1) * sizeof *nk);
     void *nv = malloc((cap ? cap : 1) * vb);
-    uint8_t vdef[8] = { 0 };                              /* Voxel() = {0, 0} */
+    uint8_t vdef[16] = { 0 };                             /* Voxel() = {0, 0} */
     size_t V1 = orc_set_union(v->keys, v->size, keys, E, v->values, vdef, vb, nk, nv);
     free(v->keys);
     free(v->values);
@@ -529,7 +535,29 @@ size_t orc_integrate(orc_volume *v, const uint16_t *mm, const orc_camera *cam,
     v->size = V1;
     v->cap = cap;
 
-    orc_update(v->keys, v->values, v->size, v->storage, depth, cam, w2c, &v->desc); /* :67 */
+    if (v->storage == ORC_STORE_F64) {
+        /* the full-FP64 run: same keys, projection and update in double from
+         * the same binary32 inputs (intrinsics, worldToEye, depth) */
+        double clip[16] = { 0 }, w2c64[16];
+        const double W = cam->width, H = cam->height;
+        clip[0] = 2.0 * cam->fx / W;
+        clip[8] = W - 2.0 * cam->cx;
+        clip[5] = 2.0 * cam->fy / H;
+        clip[9] = H - 2.0 * cam->cy;
+        clip[10] = -((double)cam->zfar + cam->znear) / ((double)cam->zfar - cam->znear);
+        clip[14] = -2.0 * cam->zfar * (double)cam->znear / ((double)cam->zfar - cam->znear);
+        clip[11] = -1.0;
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; ++k)
+                    acc += clip[r + 4 * k] * (double)worldToEye[k + 4 * c];
+                w2c64[r + 4 * c] = acc;
+            }
+        orc_update_f64(v->keys, (double *)v->values, v->size, depth, cam, w2c64, &v->desc);
+    } else {
+        orc_update(v->keys, v->values, v->size, v->storage, depth, cam, w2c, &v->desc); /* :67 */
+    }
     if (stats) {
         stats[0] = K; stats[1] = U; stats[2] = E; stats[3] = V1;
     }

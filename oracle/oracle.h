@@ -50,7 +50,8 @@ typedef struct {
 } orc_volume_desc;
 
 /* storage layouts of a voxel */
-enum { ORC_STORE_F32 = 0, ORC_STORE_F16U8 = 1 };
+enum { ORC_STORE_F32 = 0, ORC_STORE_F16U8 = 1, ORC_STORE_F64 = 2 /* {double sum, double weight}:
+                                                                     the full-FP64 precision oracle */ };
 
 /* ---- util/math.h restatements ---- */
 void orc_mat_mul(const float m[16], const float n[16], float out[16]);      /* math.h:367-377 */
@@ -159,6 +160,14 @@ int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl
                       const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
                       double pose[16], int max_iters, double dist_thresh, double eps_rot,
                       double eps_trans, double stats[2]);
+/* the FP64 normal equations of one association (layout of kifi_icp_normal_equations) */
+void orc_icp_sums_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                      const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                      const double pose[16], double dist_thresh, double out[30]);
+/* the same sums from the binary32 terms of the HIP association, summed in double */
+void orc_icp_sums_f32terms(const float *depth, int level, const float *pts, const float *nrm,
+                           const orc_camera *cam, const float w2e[16], const float pose_f[16], float gate,
+                           double out[30]);
 
 #ifdef __cplusplus
 }

@@ -265,6 +265,127 @@ static void mat_mul_d(const double *A, const double *B, double *C)
     memcpy(C, R, sizeof R);
 }
 
+/* One projective association in FP64: the point-to-plane normal equations in
+ * the layout of the GPU's reduction, out[0..20] = sum w J_i J_j (i >= j, row by
+ * row), out[21..26] = sum w J_i r, out[27] = count, out[28] = sum r^2,
+ * out[29] = sum w r^2, with w = 1/d^4 and r = n . (q - p). */
+static void icp_sums_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                         const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                         const double pose[16], double dist_thresh, double out[30])
+{
+    memset(out, 0, 30 * sizeof(double));
+    for (int v = 0; v < Hl; ++v)
+        for (int u = 0; u < Wl; ++u) {
+            double d = depth[(size_t)v * Wl + u];
+            if (!(d > 0.0))
+                continue;
+            double pe[3] = { (u + 0.5 - cxl) / fxl * d, (cyl - v - 0.5) / fyl * d, -d };
+            double p[3];
+            for (int r = 0; r < 3; ++r)
+                p[r] = pose[r] * pe[0] + pose[r + 4] * pe[1] + pose[r + 8] * pe[2] + pose[r + 12];
+            double e[3];
+            for (int r = 0; r < 3; ++r)
+                e[r] = W2E[r] * p[0] + W2E[r + 4] * p[1] + W2E[r + 8] * p[2] + W2E[r + 12];
+            double z = -e[2];
+            if (!(z > 0.0))
+                continue;
+            double fu = cam->fx * e[0] / z + cam->cx, fv = cam->cy - cam->fy * e[1] / z;
+            if (!(fu >= 0.0 && fu < cam->width && fv >= 0.0 && fv < cam->height))
+                continue;
+            size_t q = (size_t)(int)fv * cam->width + (int)fu;
+            const float *Q = pts + 4 * q, *N = nrm + 4 * q;
+            if (Q[3] == 0.0f || N[3] == 0.0f)
+                continue;
+            double df[3] = { Q[0] - p[0], Q[1] - p[1], Q[2] - p[2] };
+            if (df[0] * df[0] + df[1] * df[1] + df[2] * df[2] > dist_thresh * dist_thresh)
+                continue;
+            double n[3] = { N[0], N[1], N[2] };
+            double r = n[0] * df[0] + n[1] * df[1] + n[2] * df[2];
+            double J[6] = { p[1] * n[2] - p[2] * n[1], p[2] * n[0] - p[0] * n[2],
+                            p[0] * n[1] - p[1] * n[0], n[0], n[1], n[2] };
+            /* inverse variance of the sensor noise model sigma ~ z^2 */
+            double iz = 1.0 / d, wgt = iz * iz * iz * iz;
+            int t = 0;
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j <= i; ++j)
+                    out[t++] += wgt * J[i] * J[j];
+            for (int i = 0; i < 6; ++i)
+                out[21 + i] += wgt * J[i] * r;
+            out[27] += 1.0;
+            out[28] += r * r;
+            out[29] += wgt * r * r;
+        }
+}
+
+void orc_icp_sums_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                      const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                      const double pose[16], double dist_thresh, double out[30])
+{
+    icp_sums_f64(depth, Wl, Hl, fxl, fyl, cxl, cyl, pts, nrm, cam, W2E, pose, dist_thresh, out);
+}
+
+/* The same association in binary32, operation for operation as the HIP
+ * association kernel (k_icp_assoc): level intrinsics = full-resolution values
+ * x 2^-level, back-projection, pose and model transforms in the order of
+ * mat_vec, projection, gate, point-to-plane residual and Jacobian, w = 1/d^4.
+ * Each binary32 term is then summed in double, so out[] is the (practically
+ * exact) sum of the very terms the GPU reduces: the GPU's difference from it
+ * is its summation error alone.  Layout as orc_icp_sums_f64. */
+static void mat_vec_f32(const float *m, const float v[4], float out[4])
+{
+    float r[4];
+    for (int i = 0; i < 4; ++i)
+        r[i] = (m[i] * v[0] + m[i + 4] * v[1]) + (m[i + 8] * v[2] + m[i + 12] * v[3]);
+    memcpy(out, r, sizeof r);
+}
+
+void orc_icp_sums_f32terms(const float *depth, int level, const float *pts, const float *nrm,
+                           const orc_camera *cam, const float w2e[16], const float pose_f[16], float gate,
+                           double out[30])
+{
+    const float sc = 1.0f / (float)(1 << level);
+    const int Wl = cam->width >> level, Hl = cam->height >> level;
+    const float fxl = cam->fx * sc, fyl = cam->fy * sc, cxl = cam->cx * sc, cyl = cam->cy * sc;
+    const float Wm = (float)cam->width, Hm = (float)cam->height;
+    memset(out, 0, 30 * sizeof(double));
+    for (int v = 0; v < Hl; ++v)
+        for (int u = 0; u < Wl; ++u) {
+            const float d = depth[(size_t)v * Wl + u];
+            if (!(d > 0.0f))
+                continue;
+            const float pe[4] = { ((float)u + 0.5f - cxl) / fxl * d, (cyl - (float)v - 0.5f) / fyl * d, -d, 1.0f };
+            float p[4], e[4];
+            mat_vec_f32(pose_f, pe, p);
+            mat_vec_f32(w2e, p, e);
+            const float z = -e[2];
+            if (!(z > 0.0f))
+                continue;
+            const float fu = cam->fx * e[0] / z + cam->cx, fv = cam->cy - cam->fy * e[1] / z;
+            if (!(fu >= 0.0f && fu < Wm && fv >= 0.0f && fv < Hm))
+                continue;
+            const size_t q = (size_t)(int)fv * cam->width + (int)fu;
+            const float *Q = pts + 4 * q, *N = nrm + 4 * q;
+            if (Q[3] == 0.0f || N[3] == 0.0f)
+                continue;
+            const float dx = Q[0] - p[0], dy = Q[1] - p[1], dz = Q[2] - p[2];
+            if (!((dx * dx + dy * dy) + dz * dz <= gate * gate))
+                continue;
+            const float r = (N[0] * dx + N[1] * dy) + N[2] * dz;
+            const float J[6] = { p[1] * N[2] - p[2] * N[1], p[2] * N[0] - p[0] * N[2], p[0] * N[1] - p[1] * N[0],
+                                 N[0], N[1], N[2] };
+            const float iz = 1.0f / d, wgt = iz * iz * iz * iz;
+            int t = 0;
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j <= i; ++j)
+                    out[t++] += (double)(wgt * J[i] * J[j]);
+            for (int i = 0; i < 6; ++i)
+                out[21 + i] += (double)(wgt * J[i] * r);
+            out[27] += 1.0;
+            out[28] += (double)(r * r);
+            out[29] += (double)(wgt * (r * r));
+        }
+}
+
 /* One pyramid level of projective point-to-plane ICP in FP64.
  * Live level: depth (metres, binary32 as the GPU sees it), Wl x Hl, intrinsics
  * (fxl, fyl, cxl, cyl).  Model: pts/nrm W*H*4 at the full-resolution camera `cam`
@@ -280,53 +401,22 @@ int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl
     stats[0] = 0.0;
     stats[1] = 0.0;
     for (; it < max_iters; ++it) {
-        double A[36] = { 0 }, b[6] = { 0 }, cnt = 0, e2 = 0;
-        for (int v = 0; v < Hl; ++v)
-            for (int u = 0; u < Wl; ++u) {
-                double d = depth[(size_t)v * Wl + u];
-                if (!(d > 0.0))
-                    continue;
-                double pe[3] = { (u + 0.5 - cxl) / fxl * d, (cyl - v - 0.5) / fyl * d, -d };
-                double p[3];
-                for (int r = 0; r < 3; ++r)
-                    p[r] = pose[r] * pe[0] + pose[r + 4] * pe[1] + pose[r + 8] * pe[2] + pose[r + 12];
-                double e[3];
-                for (int r = 0; r < 3; ++r)
-                    e[r] = W2E[r] * p[0] + W2E[r + 4] * p[1] + W2E[r + 8] * p[2] + W2E[r + 12];
-                double z = -e[2];
-                if (!(z > 0.0))
-                    continue;
-                double fu = cam->fx * e[0] / z + cam->cx, fv = cam->cy - cam->fy * e[1] / z;
-                if (!(fu >= 0.0 && fu < cam->width && fv >= 0.0 && fv < cam->height))
-                    continue;
-                size_t q = (size_t)(int)fv * cam->width + (int)fu;
-                const float *Q = pts + 4 * q, *N = nrm + 4 * q;
-                if (Q[3] == 0.0f || N[3] == 0.0f)
-                    continue;
-                double df[3] = { Q[0] - p[0], Q[1] - p[1], Q[2] - p[2] };
-                if (df[0] * df[0] + df[1] * df[1] + df[2] * df[2] > dist_thresh * dist_thresh)
-                    continue;
-                double n[3] = { N[0], N[1], N[2] };
-                double r = n[0] * df[0] + n[1] * df[1] + n[2] * df[2];
-                double J[6] = { p[1] * n[2] - p[2] * n[1], p[2] * n[0] - p[0] * n[2],
-                                p[0] * n[1] - p[1] * n[0], n[0], n[1], n[2] };
-                /* inverse variance of the sensor noise model sigma ~ z^2 */
-                double iz = 1.0 / d, wgt = iz * iz * iz * iz;
-                for (int i = 0; i < 6; ++i) {
-                    for (int j = 0; j <= i; ++j)
-                        A[6 * i + j] += wgt * J[i] * J[j];
-                    b[i] += wgt * J[i] * r;
-                }
-                cnt += 1.0;
-                e2 += r * r;
+        double sums[30];
+        icp_sums_f64(depth, Wl, Hl, fxl, fyl, cxl, cyl, pts, nrm, cam, W2E, pose, dist_thresh, sums);
+        double A[36], b[6], cnt = sums[27], e2 = sums[28];
+        int t = 0;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j <= i; ++j) {
+                A[6 * i + j] = sums[t];
+                A[6 * j + i] = sums[t];
+                ++t;
             }
+        for (int i = 0; i < 6; ++i)
+            b[i] = sums[21 + i];
         stats[0] = cnt > 0 ? sqrt(e2 / cnt) : 0.0;
         stats[1] = cnt;
         if (cnt < 12.0)
             break;
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < i; ++j)
-                A[6 * j + i] = A[6 * i + j];
         double xi[6];
         if (cholesky6(A, b, xi))
             break;

@@ -172,3 +172,91 @@ def test_reference_align_exact_model_fixed_point_and_convergence():
     o12, _ = orc.icp_align_ref(d, CAM, model, me, g.astype(np.float32), iters=12)
     r, t = _rot_err(o12, p0)
     assert r < 1e-4 and t < 1e-4
+
+
+# ------------------------------------------------------------------ normal equations
+def _plane_maps(z=2.5):
+    """Model maps of the plane z = -z seen head-on: each pixel's centre ray hits the plane; normal +z."""
+    H, W = CAM.height, CAM.width
+    u, v = np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5)
+    pts = np.zeros((H, W, 4), np.float32)
+    pts[..., 0] = (u - CAM.cx) / CAM.fx * z
+    pts[..., 1] = (CAM.cy - v) / CAM.fy * z
+    pts[..., 2] = -z
+    pts[..., 3] = 1.0
+    nrm = np.zeros_like(pts)
+    nrm[..., 2] = 1.0
+    nrm[..., 3] = 1.0
+    return pts, nrm
+
+
+def _tri(i, j):
+    return i * (i + 1) // 2 + j
+
+
+@pytest.mark.parametrize("dz", [0.0, 0.01])
+def test_icp_sums_plane_closed_form(dz):
+    """A flat frame at 2.5 m against the plane's own model, camera moved by dz
+    along its axis: every residual is r = -dz, J = (p_y, -p_x, 0, 0, 0, 1), and
+    the sums follow from the pixels' back-projections (w = 1/2.5^4)."""
+    pts, nrm = _plane_maps()
+    d = np.full((CAM.height, CAM.width), 2500, np.uint16)
+    pose = np.eye(4)
+    pose[2, 3] = dz
+    e2w = pose.T.reshape(16)
+    I = np.eye(4).reshape(16)
+    s64 = orc.icp_sums_f64(d, CAM, pts, nrm, I, e2w)
+    s32 = orc.icp_sums_f32terms(d, CAM, pts, nrm, I, e2w)
+    # brute force over the pixels: back-project, keep those projecting into the image
+    u, v = np.meshgrid(np.arange(CAM.width) + 0.5, np.arange(CAM.height) + 0.5)
+    px = (u - CAM.cx) / CAM.fx * 2.5
+    py = (CAM.cy - v) / CAM.fy * 2.5
+    fu = CAM.fx * px / (2.5 - dz) + CAM.cx
+    fv = CAM.cy - CAM.fy * py / (2.5 - dz)
+    ok = (fu >= 0) & (fu < CAM.width) & (fv >= 0) & (fv < CAM.height)
+    n, w = ok.sum(), 1.0 / 2.5 ** 4
+    want = np.zeros(30)
+    want[_tri(0, 0)] = w * (py[ok] ** 2).sum()
+    want[_tri(1, 0)] = -w * (px[ok] * py[ok]).sum()
+    want[_tri(1, 1)] = w * (px[ok] ** 2).sum()
+    want[_tri(5, 0)] = w * py[ok].sum()
+    want[_tri(5, 1)] = -w * px[ok].sum()
+    want[_tri(5, 5)] = w * n
+    want[21 + 0] = -dz * w * py[ok].sum()
+    want[21 + 1] = dz * w * px[ok].sum()
+    want[21 + 5] = -dz * w * n
+    want[27], want[28], want[29] = n, n * dz * dz, w * n * dz * dz
+    scale = np.abs(want).max()
+    assert s64[27] == n == s32[27]
+    assert np.abs(s64 - want).max() <= 1e-9 * scale, np.abs(s64 - want)
+    # binary32 terms: the same sums to binary32 rounding of each term
+    assert np.abs(s32 - want).max() <= 1e-6 * scale, np.abs(s32 - want)
+
+
+def test_icp_sums_f32terms_agree_with_f64_on_a_room_frame(room_model):
+    sc, p0, w2e, pts, nrm = room_model
+    pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 3)
+    d = mf.render(sc, CAM, pt, noise_seed=8)
+    for level in (0, 2):
+        s64 = orc.icp_sums_f64(d, CAM, pts, nrm, w2e.astype(np.float64), pt, level=level)
+        s32 = orc.icp_sums_f32terms(d, CAM, pts, nrm, w2e, pt, level=level)
+        assert s64[27] > 1000 and abs(s32[27] - s64[27]) <= 1e-4 * s64[27]
+        diag = np.array([s64[_tri(i, i)] for i in range(6)])
+        for i in range(6):
+            for j in range(i + 1):
+                assert abs(s32[_tri(i, j)] - s64[_tri(i, j)]) <= 1e-3 * np.sqrt(diag[i] * diag[j])
+        # one iteration of the FP64 solver from the sums is the oracle's own step
+        A = np.zeros((6, 6))
+        for i in range(6):
+            for j in range(i + 1):
+                A[i, j] = A[j, i] = s64[_tri(i, j)]
+        xi = np.linalg.solve(A, s64[21:27])
+        if level == 0:
+            pose1, it, _ = orc.icp_f64(d, CAM, pts, nrm, w2e.astype(np.float64), pt, iters=(1,))
+            P = np.asarray(pt, np.float64).reshape(4, 4).T
+            th = np.linalg.norm(xi[:3])
+            K = np.array([[0, -xi[2], xi[1]], [xi[2], 0, -xi[0]], [-xi[1], xi[0], 0]])
+            R = np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
+            D = np.eye(4)
+            D[:3, :3], D[:3, 3] = R, xi[3:]
+            assert it == 1 and np.abs((D @ P).T.reshape(16) - pose1).max() < 1e-9
