@@ -505,9 +505,12 @@ __device__ __forceinline__ int voxel_pixel(K key, const UpdateParams &p, const M
         return -1;
     // :305-315 divides all four lanes by w and keeps the voxel only if
     // (z/w)*0+0 and (w/w)*0+0 are in range, i.e. z/w is finite and w is finite
-    // and nonzero; that needs no division unless |w| is tiny or |z| huge
+    // and nonzero.  With w finite and nonzero, RN(z/w) is finite exactly when
+    // |z| < |w| (2^128 - 2^103) (the rounding boundary above FLT_MAX; a tie
+    // rounds to the even infinity); the product is exact in double (24 x 25
+    // bits), so the test needs no division.  NaN z fails the comparison.
     const bool wok = kw != 0.0f && fabsf(kw) <= FLT_MAX;
-    const bool zok = (fabsf(kw) >= 0x1p-32f && fabsf(k[2]) <= 0x1p32f) || fabsf(k[2] / kw) <= FLT_MAX;
+    const bool zok = (double)fabsf(k[2]) < (double)fabsf(kw) * 0x1.ffffffp127;
     float u = k[0] / kw, v = k[1] / kw;
     u = u * p.ndcU + p.ndcU;                                       // :310-313
     v = v * p.ndcV + p.ndcV;
@@ -596,13 +599,17 @@ __device__ __forceinline__ bool near_surface(const VoxF32 &v, float dmax)
 }
 __device__ __forceinline__ bool near_surface(uint32_t v, float dmax) { return fabsf(vox_dist(v)) < dmax; }
 
-template <typename K, typename V>
-__device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+// The voxel enters the model: weighted and within dmax of the surface.
+template <typename V>
+__device__ __forceinline__ bool splat_candidate(const V &val, const SplatParams &sp)
 {
-    if (!(vox_weight(val) > 0.0f))
-        return;
-    if (!near_surface(val, sp.dmax))
-        return;
+    return vox_weight(val) > 0.0f && near_surface(val, sp.dmax);
+}
+
+// Splat of a candidate voxel with merged index o into the model z-buffer.
+template <typename K>
+__device__ __forceinline__ void splat_point(K key, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+{
     uint32_t x, y, zz;
     unpack_key<K>(key, x, y, zz);
     const float p[4] = { (float)x * sp.vl + sp.c0, (float)y * sp.vl + sp.c0, (float)zz * sp.vl + sp.c0, 1.0f };
@@ -636,6 +643,13 @@ __device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, con
         }
 }
 
+template <typename K, typename V>
+__device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+{
+    if (splat_candidate(val, sp))
+        splat_point<K>(key, o, sp, w2e);
+}
+
 // LDS index of the shift histogram: one pad word per 32 keeps a lane's 16
 // consecutive entries (the scan) and a wave's consecutive entries (the old-voxel
 // pass) on distinct banks
@@ -655,16 +669,19 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
         return;
-    // frame matrices into registers once (the output stores would otherwise force
-    // a reload per voxel)
+    // the frame matrix into registers once (the output stores would otherwise
+    // force a reload per voxel); the splat's matrix is loaded where it is used,
+    // so the two are not live together
     const Mat4 w2c = up.mats->w2c;
-    Mat4 w2e;
-    if (SPLAT)
-        w2e = *sp.w2e;
     const uint64_t tot = cnt->v_old + cnt->n_new;
     const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
     const uint64_t obase = (uint64_t)blockIdx.x * kMgTile - tile_base_dups[blockIdx.x];
+    // outputs by 32-bit offset from the tile's base (offsets < 2^13); updates
+    // stop at offset ulim
+    K *const Ot = O + obase;
+    V *const Ovt = Ov + obase;
+    const uint32_t ulim = upd_end > obase ? (uint32_t)min(upd_end - obase, (uint64_t)0xffffffffu) : 0u;
     // Every global load of the tile is issued up front: old keys, new keys, then
     // old values, which stay in registers until the old-voxel pass.  A lane has
     // up to 3 x PER loads in flight, where a loop of load-use iterations would
@@ -721,15 +738,15 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             keepm |= (uint32_t)keep << q;
             if (keep) {
                 // new voxel: Voxel() default, then update
-                const uint64_t o = obase + pos + lbq[q];
+                const uint32_t ol = pos + lbq[q];
                 V val{};
                 float s;
-                if (o < upd_end && voxel_sample<K>(bk[q], up, w2c, s))
+                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, s))
                     voxel_apply(val, s);
-                O[o] = bk[q];
-                Ov[o] = val;
+                Ot[ol] = bk[q];
+                Ovt[ol] = val;
                 if (SPLAT)
-                    splat_voxel<K, V>(bk[q], val, o, sp, w2e);
+                    splat_voxel<K, V>(bk[q], val, obase + ol, sp, *sp.w2e);
             }
         }
     }
@@ -768,7 +785,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
         pix[q] = -1;
-        if (i < mt.na && obase + i + (nbk ? h[hpad(i)] : 0u) < upd_end)
+        if (i < mt.na && i + (nbk ? h[hpad(i)] : 0u) < ulim)
             pix[q] = voxel_pixel<K>(sA[i], up, w2c);
     }
     // unconditional loads (pixel 0 stands in for "none"): a load under a branch
@@ -777,20 +794,45 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
 #pragma unroll
     for (int q = 0; q < PER; ++q)
         dmm[q] = up.depth[pix[q] >= 0 ? pix[q] : 0];
+    uint32_t cand = 0;                 // splat candidates among the lane's voxels
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
         if (i < mt.na) {
             const K k = sA[i];
-            const uint64_t o = obase + i + (nbk ? h[hpad(i)] : 0u);
+            const uint32_t ol = i + (nbk ? h[hpad(i)] : 0u);
             V val = av[q];
             float s;
             if (pix[q] >= 0 && voxel_update<K>(k, dmm[q], up, w2c, s))
                 voxel_apply(val, s);
-            O[o] = k;
-            Ov[o] = val;
+            Ot[ol] = k;
+            Ovt[ol] = val;
             if (SPLAT)
-                splat_voxel<K, V>(k, val, o, sp, w2e);
+                cand |= (uint32_t)splat_candidate(val, sp) << q;
+        }
+    }
+    if (SPLAT) {
+        // The candidates are a minority spread over every wave: splatting them
+        // inline would run the projection for whole waves.  They are compacted
+        // into a list (tile index | output offset << 16, both < 2^16) in the
+        // histogram's space and splatted densely.
+        uint32_t ent[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t i = q * kThreads + threadIdx.x;
+            ent[q] = (cand >> q & 1u) ? i | ((i + (nbk ? h[hpad(i)] : 0u)) << 16) : 0u;
+        }
+        uint32_t ncand;
+        uint32_t pos = block_exclusive_scan<kThreads>((uint32_t)__builtin_popcount(cand), tmp, ncand);
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (cand >> q & 1u)
+                h[pos++] = ent[q];
+        __syncthreads();
+        const Mat4 w2e = *sp.w2e;
+        for (uint32_t t = threadIdx.x; t < ncand; t += kThreads) {
+            const uint32_t e = h[t];
+            splat_point<K>(sA[e & 0xffffu], obase + (e >> 16), sp, w2e);
         }
     }
 }
