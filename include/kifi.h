@@ -303,6 +303,10 @@ kifi_status kifi_timing_read(kifi_ctx *ctx, int stage, double *total_ms, long lo
 /* Device buffers for callers without their own allocator (frames resident in HBM). */
 kifi_status kifi_device_alloc(kifi_ctx *ctx, size_t bytes, void **out);
 kifi_status kifi_device_free(kifi_ctx *ctx, void *ptr);
+/* Page-locked host buffers: frames and meshes in them cross PCIe at full rate
+ * (a pageable buffer is staged through a bounce buffer by the runtime). */
+kifi_status kifi_host_alloc(kifi_ctx *ctx, size_t bytes, void **out);
+kifi_status kifi_host_free(kifi_ctx *ctx, void *ptr);
 kifi_status kifi_memcpy_to_device(kifi_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Plain streaming copy kernel (16-byte vectors), the calibration reference for
  * HBM byte counters.  bytes must be a multiple of 16.  Asynchronous. */

@@ -34,7 +34,7 @@ HIP_EXPORTS = [
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
-    "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_memcpy_to_device",
+    "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_host_alloc", "kifi_host_free", "kifi_memcpy_to_device",
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
     "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
     "kifi_twistnet_set_params", "kifi_twistnet_forward", "kifi_twistnet_loss_grad", "kifi_twistnet_train",
@@ -159,6 +159,8 @@ def load_hip():
             "kifi_timing_read": ([vp, i32, f64p, ctypes.POINTER(ctypes.c_longlong)], i32),
             "kifi_device_alloc": ([vp, sz, ctypes.POINTER(vp)], i32),
             "kifi_device_free": ([vp, vp], i32),
+            "kifi_host_alloc": ([vp, sz, ctypes.POINTER(vp)], i32),
+            "kifi_host_free": ([vp, vp], i32),
             "kifi_memcpy_to_device": ([vp, vp, vp, sz], i32),
             "kifi_device_copy": ([vp, vp, vp, sz], i32),
             "kifi_integrate_totals": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -315,6 +317,20 @@ class Context:
         self.check(self._lib.kifi_timing_read(self._h, stage, ctypes.byref(ms), ctypes.byref(n)), "kifi_timing_read")
         return ms.value, n.value
 
+    def host_empty(self, shape, dtype):
+        """A numpy array in page-locked host memory (kifi_host_alloc), freed with the
+        context or when the array and its views are gone."""
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        p = ctypes.c_void_p()
+        self.check(self._lib.kifi_host_alloc(self._h, nbytes, ctypes.byref(p)), "kifi_host_alloc")
+        buf = HostBuffer(self, p, nbytes)
+        self._adopt(buf)
+        raw = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
+        arr = np.frombuffer(raw, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+        buf.keep = arr
+        return arr
+
     def device_upload(self, array):
         """Copy a numpy array into a new device buffer; returns DeviceBuffer."""
         a = np.ascontiguousarray(array)
@@ -358,6 +374,18 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class HostBuffer:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx._lib.kifi_host_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    close = free
 
 
 class Volume:

@@ -183,6 +183,26 @@ kifi_status kifi_device_free(kifi_ctx *ctx, void *ptr)
     return KIFI_OK;
 }
 
+kifi_status kifi_host_alloc(kifi_ctx *ctx, size_t bytes, void **out)
+{
+    if (!ctx || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    KIFI_HIP(ctx, hipSetDevice(ctx->device));
+    KIFI_HIP(ctx, hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return KIFI_OK;
+}
+
+kifi_status kifi_host_free(kifi_ctx *ctx, void *ptr)
+{
+    if (!ctx)
+        return KIFI_ERR_ARG;
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ptr)
+        KIFI_HIP(ctx, hipHostFree(ptr));
+    return KIFI_OK;
+}
+
 kifi_status kifi_device_copy(kifi_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     if (!ctx || (bytes && (!dst || !src)) || bytes % 16 != 0)

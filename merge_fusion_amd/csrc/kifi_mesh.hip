@@ -108,20 +108,105 @@ __device__ __forceinline__ uint64_t upper_bound_g(const K *keys, uint64_t lo, ui
     return lo;
 }
 
-// min(V-2, upper_bound(t) - 1) for monotone targets across the wave's lanes:
-// lanes 0 and 63 search the whole array, the others their window.
-template <typename K>
-__device__ __forceinline__ int64_t wave_walk(const K *keys, uint64_t V, K t)
+// Neighbour search.  The reference walks pointers to the +y, +z (and +y+z)
+// neighbours of each voxel (Mesher.cpp:122-132, 174-175); here that is
+// min(V-2, upper_bound(keys, key + offset) - 1).  Voxels go in tiles of
+// kMeshTile; for a tile the upper bounds of all its targets lie between those
+// of its first voxel and of the next tile's first voxel, which k_mesh_windows
+// finds by one binary search per tile boundary.  Each tile loads that key
+// window [w_t - 1, w_{t+1}] into LDS and searches there; a window wider than
+// kMeshWin keys is searched in global memory instead.
+constexpr int kMeshTile = 1024;
+constexpr int kMeshWin = 2048;
+
+template <typename K, bool TRIS>
+struct MeshOffsets {
+    static constexpr int n = TRIS ? 3 : 2;
+    __device__ static K get(int o)
+    {
+        constexpr K d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
+        return o == 0 ? d2 : (o == 1 ? d4 : d4 + d2);
+    }
+};
+
+// w[o * (T + 1) + t] = upper_bound(keys, keys[min(t * kMeshTile, V - 1)] + offset_o)
+template <typename K, bool TRIS>
+__global__ __launch_bounds__(kMT) void k_mesh_windows(const K *__restrict__ keys, uint64_t n, uint64_t T,
+                                                      uint64_t *__restrict__ w)
 {
-    const int lane = __lane_id();
-    uint64_t u = 0;
-    if (lane == 0 || lane == kWave - 1)
-        u = upper_bound_g(keys, 0, V, t);
-    const uint64_t lo = __shfl(u, 0, kWave), hi = __shfl(u, kWave - 1, kWave);
-    u = upper_bound_g(keys, lo, hi, t);
+    constexpr int NO = MeshOffsets<K, TRIS>::n;
+    const uint64_t g = (uint64_t)blockIdx.x * kMT + threadIdx.x;
+    if (g >= (T + 1) * NO)
+        return;
+    const int o = (int)(g / (T + 1));
+    const uint64_t t = g % (T + 1);
+    const uint64_t i = min(t * (uint64_t)kMeshTile, n - 1);
+    w[g] = upper_bound_g(keys, 0, n, (K)(keys[i] + MeshOffsets<K, TRIS>::get(o)));
+}
+
+template <typename K, bool TRIS>
+struct MeshWin {
+    static constexpr int NO = MeshOffsets<K, TRIS>::n;
+    K s[NO][kMeshWin];
+};
+
+// The tile's windows: [lo, hi) per offset, in LDS when lds.  Uniform over the block.
+template <typename K, bool TRIS>
+struct WinRange {
+    uint64_t lo[MeshOffsets<K, TRIS>::n], hi[MeshOffsets<K, TRIS>::n];
+    bool lds[MeshOffsets<K, TRIS>::n];
+};
+
+template <typename K, bool TRIS>
+__device__ __forceinline__ WinRange<K, TRIS> mesh_load_windows(const K *keys, uint64_t n, const uint64_t *w,
+                                                               uint64_t T, uint64_t t, MeshWin<K, TRIS> &mw)
+{
+    WinRange<K, TRIS> r;
+#pragma unroll
+    for (int o = 0; o < MeshOffsets<K, TRIS>::n; ++o) {
+        const uint64_t a = min(w[o * (T + 1) + t], n - 1), b = w[o * (T + 1) + t + 1];
+        r.lo[o] = a >= 1 ? a - 1 : 0;
+        r.hi[o] = min(b + 1, n);
+        r.lds[o] = r.hi[o] - r.lo[o] <= (uint64_t)kMeshWin;
+        if (r.lds[o])
+            for (uint32_t j = threadIdx.x; j < r.hi[o] - r.lo[o]; j += kMT)
+                mw.s[o][j] = keys[r.lo[o] + j];
+    }
+    __syncthreads();
+    return r;
+}
+
+// min(V-2, upper_bound(target) - 1) and the keys at that index and the next
+template <typename K, bool TRIS>
+__device__ __forceinline__ int64_t win_walk(const K *keys, uint64_t n, const MeshWin<K, TRIS> &mw,
+                                            const WinRange<K, TRIS> &wr, int o, K target, K &k_at, K &k_next)
+{
+    uint64_t u;
+    if (wr.lds[o]) {
+        const K *s = mw.s[o];
+        uint32_t a = 0, b = (uint32_t)(wr.hi[o] - wr.lo[o]);
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (s[mid] <= target)
+                a = mid + 1;
+            else
+                b = mid;
+        }
+        u = wr.lo[o] + a;
+    } else {
+        u = upper_bound_g(keys, wr.lo[o], wr.hi[o], target);
+    }
+    const int64_t cl = (int64_t)n - 2;
     int64_t r = (int64_t)u - 1;
-    const int64_t cl = (int64_t)V - 2;
-    return r < cl ? r : cl;
+    r = r < cl ? r : cl;
+    if (wr.lds[o]) {
+        k_at = mw.s[o][r - (int64_t)wr.lo[o]];
+        k_next = mw.s[o][r + 1 - (int64_t)wr.lo[o]];
+    } else {
+        k_at = keys[r];
+        k_next = keys[r + 1];
+    }
+    return r;
 }
 
 struct MeshParams {
@@ -134,30 +219,27 @@ struct Corners {
 };
 
 // Neighbour search and vertex flags of voxel i (Mesher.cpp:117-168).
-// All lanes of the wave must call this with ascending i.
 template <typename K, typename V, bool TRIS>
-__device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals, uint64_t n, uint64_t i_raw,
+__device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals, uint64_t n, uint64_t i,
+                                                   const MeshWin<K, TRIS> &mw, const WinRange<K, TRIS> &wr,
                                                    Corners<K> &cn, int &lut)
 {
     constexpr K d1 = 1, d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
-    const bool active = i_raw + 2 <= n;                 // i <= V-2
-    const uint64_t i = active ? i_raw : (n >= 2 ? n - 2 : 0);
+    lut = -1;
     const K k = keys[i];
-    const int64_t iTop = wave_walk<K>(keys, n, k + d2);           // :122-123
-    const int64_t iFront = wave_walk<K>(keys, n, k + d4);         // :125-126
+    K kTop, kTop1, kFront, kFront1, kFT = 0, kFT1 = 0;
+    const int64_t iTop = win_walk<K, TRIS>(keys, n, mw, wr, 0, k + d2, kTop, kTop1);       // :122-123
+    const int64_t iFront = win_walk<K, TRIS>(keys, n, mw, wr, 1, k + d4, kFront, kFront1);  // :125-126
     int64_t iFrontTop = 0;
     if (TRIS)
-        iFrontTop = wave_walk<K>(keys, n, k + d4 + d2);           // :174-175
-    lut = -1;
-    if (!active)
-        return 0;
+        iFrontTop = win_walk<K, TRIS>(keys, n, mw, wr, 2, k + d4 + d2, kFT, kFT1);          // :174-175
     const V v0 = vals[i];
     if (vweight(v0) == 0.0f)                                       // :119-120
         return 0;
     cn.c[0] = (int64_t)i;
     cn.c[1] = keys[i + 1] == k + d1 ? (int64_t)i + 1 : -1;         // :132
-    cn.c[2] = keys[iTop] == k + d2 ? iTop : -1;                    // :129
-    cn.c[4] = keys[iFront] == k + d4 ? iFront : -1;                // :130
+    cn.c[2] = kTop == k + d2 ? iTop : -1;                          // :129
+    cn.c[4] = kFront == k + d4 ? iFront : -1;                      // :130
     const float dself = vdist(v0);
     uint32_t flags = 0;
     const int axis_corner[3] = { 1, 2, 4 };
@@ -171,10 +253,10 @@ __device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals,
         }
     }
     if (TRIS) {
-        cn.c[6] = keys[iFrontTop] == k + d4 + d2 ? iFrontTop : -1; // :177
-        cn.c[3] = keys[iTop + 1] == k + d2 + d1 ? iTop + 1 : -1;   // :179
-        cn.c[5] = keys[iFront + 1] == k + d4 + d1 ? iFront + 1 : -1;
-        cn.c[7] = keys[iFrontTop + 1] == k + d4 + d2 + d1 ? iFrontTop + 1 : -1;
+        cn.c[6] = kFT == k + d4 + d2 ? iFrontTop : -1;             // :177
+        cn.c[3] = kTop1 == k + d2 + d1 ? iTop + 1 : -1;            // :179
+        cn.c[5] = kFront1 == k + d4 + d1 ? iFront + 1 : -1;
+        cn.c[7] = kFT1 == k + d4 + d2 + d1 ? iFrontTop + 1 : -1;
         bool skip = false;
         float d[8];
         d[0] = dself;
@@ -198,22 +280,24 @@ __device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals,
     return flags;
 }
 
-// Pass 1: per-voxel vertex flags and counts.
+// Pass 1: per-voxel vertex flags and counts, a tile of kMeshTile voxels at a time.
 template <typename K, typename V, bool TRIS>
 __global__ __launch_bounds__(kMT) void k_mesh_count(const K *__restrict__ keys, const V *__restrict__ vals,
-                                                    uint64_t n, uint8_t *__restrict__ vflags,
+                                                    uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
+                                                    uint8_t *__restrict__ vflags,
                                                     uint32_t *__restrict__ cnt_v_fwd,
                                                     uint32_t *__restrict__ cnt_v_rev,
                                                     uint32_t *__restrict__ cnt_i_rev)
 {
-    const uint64_t m = n >= 2 ? n - 1 : 0;              // voxels 0 .. V-2
-    const uint64_t stride = (uint64_t)gridDim.x * kMT;
-    for (uint64_t base = (uint64_t)blockIdx.x * kMT; base < m; base += stride) {
-        const uint64_t i = base + threadIdx.x;
-        Corners<K> cn;
-        int lut;
-        const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, cn, lut);
-        if (i < m) {
+    __shared__ MeshWin<K, TRIS> mw;
+    const uint64_t m = n - 1;                           // voxels 0 .. V-2
+    for (uint64_t t = blockIdx.x; t < T; t += gridDim.x) {
+        const WinRange<K, TRIS> wr = mesh_load_windows<K, TRIS>(keys, n, win, T, t, mw);
+        const uint64_t i0 = t * kMeshTile, i1 = min(i0 + kMeshTile, m);
+        for (uint64_t i = i0 + threadIdx.x; i < i1; i += kMT) {
+            Corners<K> cn;
+            int lut;
+            const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, mw, wr, cn, lut);
             const uint32_t nv = __popc(f);
             vflags[i] = (uint8_t)f;
             if (cnt_v_fwd)
@@ -223,6 +307,7 @@ __global__ __launch_bounds__(kMT) void k_mesh_count(const K *__restrict__ keys, 
             if (TRIS)
                 cnt_i_rev[m - 1 - i] = lut >= 0 ? (uint32_t)(kTriOffsets[lut + 1] - kTriOffsets[lut]) : 0u;
         }
+        __syncthreads();                                // the windows are reloaded for the next tile
     }
 }
 
@@ -236,7 +321,8 @@ __constant__ int kCornerOff[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1
 // Pass 2: emit vertices and (optionally) indices at their scanned positions.
 template <typename K, typename V, bool TRIS>
 __global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, const V *__restrict__ vals,
-                                                   uint64_t n, MeshParams mp,
+                                                   uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
+                                                   MeshParams mp,
                                                    const uint8_t *__restrict__ vflags,
                                                    const uint32_t *__restrict__ voff,   // fwd (TRIS) or rev
                                                    const uint32_t *__restrict__ ioff_rev,
@@ -244,16 +330,16 @@ __global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, c
                                                    uint64_t *__restrict__ raw_ids,
                                                    unsigned long long *__restrict__ stall_id)
 {
-    const uint64_t m = n >= 2 ? n - 1 : 0;
-    const uint64_t stride = (uint64_t)gridDim.x * kMT;
+    __shared__ MeshWin<K, TRIS> mw;
+    const uint64_t m = n - 1;
     constexpr int kb = KeyTraits<K>::kb;
-    for (uint64_t base = (uint64_t)blockIdx.x * kMT; base < m; base += stride) {
-        const uint64_t i = base + threadIdx.x;
+    for (uint64_t t = blockIdx.x; t < T; t += gridDim.x) {
+      const WinRange<K, TRIS> wr = mesh_load_windows<K, TRIS>(keys, n, win, T, t, mw);
+      const uint64_t i0 = t * kMeshTile, i1 = min(i0 + kMeshTile, m);
+      for (uint64_t i = i0 + threadIdx.x; i < i1; i += kMT) {
         Corners<K> cn;
         int lut;
-        const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, cn, lut);
-        if (i >= m)
-            continue;
+        const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, mw, wr, cn, lut);
         if (f) {
             const K k = keys[i];
             uint32_t x0, y0, z0;
@@ -307,6 +393,8 @@ __global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, c
                 (void)kb;
             }
         }
+      }
+      __syncthreads();                                  // the windows are reloaded for the next tile
     }
 }
 
@@ -410,7 +498,9 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     }
     const uint64_t m = n - 1;
     const uint64_t nt = (m + kScanTile - 1) / kScanTile + 1;
-    size_t bytes = a256(m) + 3 * a256(m * 4) + a256(nt * 4) + 2 * a256(16);
+    const uint64_t T = (m + kMeshTile - 1) / kMeshTile;
+    const int NO = tris ? 3 : 2;
+    size_t bytes = a256(m) + 3 * a256(m * 4) + a256(nt * 4) + 2 * a256(16) + a256((T + 1) * NO * 8);
     kifi_status s = ensure_scratch(ctx, bytes);
     if (s != KIFI_OK)
         return s;
@@ -421,18 +511,23 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     uint32_t *ci_rev = (uint32_t *)p; p += a256(m * 4);
     uint32_t *tsum = (uint32_t *)p; p += a256(nt * 4);
     uint64_t *totals = (uint64_t *)p; p += a256(16);
-    unsigned long long *stall = (unsigned long long *)p;
+    unsigned long long *stall = (unsigned long long *)p; p += a256(16);
+    uint64_t *win = (uint64_t *)p;
 
     const K *keys = (const K *)vol->keys[vol->cur];
     const V *vals = (const V *)vol->vals[vol->cur];
-    const unsigned grid = (unsigned)std::min<uint64_t>((m + kMT - 1) / kMT, (uint64_t)ctx->num_cus * 16);
+    const unsigned grid = (unsigned)std::min<uint64_t>(T, (uint64_t)ctx->num_cus * 8);
+    const unsigned wgrid = (unsigned)(((T + 1) * NO + kMT - 1) / kMT);
     const int t_mesh = timer_begin(ctx, KIFI_STAGE_MESH);
-    if (tris)
-        hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, vflags,
-                           cv_fwd, (uint32_t *)nullptr, ci_rev);
-    else
-        hipLaunchKernelGGL((k_mesh_count<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, vflags,
-                           (uint32_t *)nullptr, cv_rev, (uint32_t *)nullptr);
+    if (tris) {
+        hipLaunchKernelGGL((k_mesh_windows<K, true>), dim3(wgrid), dim3(kMT), 0, st, keys, n, T, win);
+        hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T,
+                           vflags, cv_fwd, (uint32_t *)nullptr, ci_rev);
+    } else {
+        hipLaunchKernelGGL((k_mesh_windows<K, false>), dim3(wgrid), dim3(kMT), 0, st, keys, n, T, win);
+        hipLaunchKernelGGL((k_mesh_count<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T,
+                           vflags, (uint32_t *)nullptr, cv_rev, (uint32_t *)nullptr);
+    }
     uint32_t *voff = tris ? cv_fwd : cv_rev;
     device_scan(st, voff, voff, m, tsum, totals);
     if (tris)
@@ -461,8 +556,8 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     const int t_emit = timer_begin(ctx, KIFI_STAGE_MESH);
     if (tris) {
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, stall, (uint64_t)1, ~0ull);
-        hipLaunchKernelGGL((k_mesh_emit<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, mp, vflags,
-                           cv_fwd, ci_rev, d_verts, d_idx, d_raw, stall);
+        hipLaunchKernelGGL((k_mesh_emit<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T, mp,
+                           vflags, cv_fwd, ci_rev, d_verts, d_idx, d_raw, stall);
         unsigned long long h_stall = ~0ull;
         KIFI_HIP(ctx, hipMemcpyAsync(&h_stall, stall, 8, hipMemcpyDeviceToHost, st));
         KIFI_HIP(ctx, hipStreamSynchronize(st));
@@ -473,9 +568,9 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
                 *stalled = 1;
         }
     } else {
-        hipLaunchKernelGGL((k_mesh_emit<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, mp, vflags,
-                           cv_rev, (uint32_t *)nullptr, d_verts, (uint32_t *)nullptr, (uint64_t *)nullptr,
-                           stall);
+        hipLaunchKernelGGL((k_mesh_emit<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T, mp,
+                           vflags, cv_rev, (uint32_t *)nullptr, d_verts, (uint32_t *)nullptr,
+                           (uint64_t *)nullptr, stall);
     }
     timer_end(ctx, KIFI_STAGE_MESH, t_emit);
     KIFI_HIP(ctx, hipGetLastError());

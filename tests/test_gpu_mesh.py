@@ -93,3 +93,29 @@ def test_bin_matches_oracle(ctx):
     o = orc.bin_points(verts, w2c, CAM.width, CAM.height)
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     assert (np.abs(g).sum(-1) > 0).mean() > 0.2
+
+
+def test_mesh_into_page_locked_buffer(ctx):
+    """kifi_host_alloc buffers: a mesh copied into page-locked memory equals the
+    oracle's, a too-small buffer reports the count and writes nothing, and a
+    frame integrated from page-locked memory gives the same volume."""
+    gv, ov = _fused(ctx, mf.SCENE_ROOM, 2, mf.TRAJ_CIRCLE, [0, 3], 1024, 4.096, 0.02)
+    o = ov.mesh()
+    small = ctx.host_empty((16, 3), np.float32)
+    small[:] = 7.0
+    assert gv.mesh_into(small) == len(o) and (small == 7.0).all()
+    buf = ctx.host_empty((len(o) + 10, 3), np.float32)
+    n = gv.mesh_into(buf)
+    assert n == len(o) and np.array_equal(buf[:n].view(np.uint32), o.view(np.uint32))
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 6)
+    d = mf.render(sc, CAM, pose, noise_seed=506)
+    pinned = ctx.host_empty(d.shape, np.uint16)
+    pinned[:] = d
+    w2e = mf.invert_rigid(pose).astype(np.float32)
+    gv.integrate(pinned, CAM, w2e)
+    ov.integrate(d, CAM, w2e)
+    gk, gvals = gv.download()
+    ok, ovals = ov.read()
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(np.asarray(gvals).view(np.uint32).ravel(), np.asarray(ovals).view(np.uint32).ravel())
