@@ -413,40 +413,47 @@ __device__ __forceinline__ MergeTile<K> merge_tile(const uint64_t *part_a, uint6
     return mt;
 }
 
-// One wave per tile.  A new key of the tile can equal only an old key of the
-// tile or the one just before it (ties put A first; keys are unique), so each
-// new key is looked up by binary search in A[a0 - 1, a1) straight from global
-// memory: a tile's few new keys touch a few lines of its old range instead of
-// loading all of it.
+// Per tile: the new keys equal to an old key of the tile or the one just
+// before it (ties put A first; keys are unique).  The tile's old keys are
+// loaded into LDS with every load issued up front, then each new key (spread
+// over all lanes) is looked up there.
 template <typename K>
-__global__ __launch_bounds__(kWave) void k_merge_dups(const K *__restrict__ A,
-                                                      const K *__restrict__ B,
-                                                      const uint64_t *__restrict__ part_a,
-                                                      const FrameCounters *__restrict__ cnt,
-                                                      uint32_t *__restrict__ tile_dups)
+__global__ __launch_bounds__(kThreads) void k_merge_dups(const K *__restrict__ A,
+                                                         const K *__restrict__ B,
+                                                         const uint64_t *__restrict__ part_a,
+                                                         const FrameCounters *__restrict__ cnt,
+                                                         uint32_t *__restrict__ tile_dups)
 {
+    __shared__ K sA[kMgTile];
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
         return;
     const uint64_t tot = cnt->v_old + cnt->n_new;
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
-    const uint64_t lo0 = mt.a0 > 0 ? mt.a0 - 1 : 0;
-    const uint32_t n = (uint32_t)(mt.a1 - lo0);
-    const K *__restrict__ a = A + lo0;
-    uint32_t c = 0;
-    for (uint32_t j = threadIdx.x; j < mt.nb; j += kWave) {
-        const K b = B[mt.b0 + j];
-        uint32_t lo = 0, hi = n;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (a[mid] < b)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        c += lo < n && a[lo] == b;
+    constexpr int PER = kMgTile / kThreads;
+    K ak[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na)
+            ak[q] = A[mt.a0 + i];
     }
-    c = wave_reduce_sum(c);
+    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = q * kThreads + threadIdx.x;
+        if (i < mt.na)
+            sA[i] = ak[q];
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads) {
+        const K b = B[mt.b0 + j];
+        const uint32_t lb = lds_lower_bound(sA, mt.na, b);
+        c += (lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b);
+    }
+    c = block_reduce_sum<kThreads>(c, tmp);
     if (threadIdx.x == 0)
         tile_dups[blockIdx.x] = c;
 }
@@ -965,7 +972,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((Tm + 1 + kThreads - 1) / kThreads)),
                        dim3(kThreads), 0, stream, (const K *)vol->keys[cur], vol->d_size, Ekeys, cnt,
                        part_a, (int)Tm);
-    hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kWave), 0, stream,
+    hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                        (const K *)vol->keys[cur], Ekeys, part_a, cnt, tile_dups);
     hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
                        (uint64_t)vol->cap, vol->h_counters_dev);
