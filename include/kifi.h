@@ -91,6 +91,13 @@ kifi_status kifi_integrate(kifi_volume *vol, const uint16_t *depth_mm, const kif
                            const float worldToEye[16]);
 kifi_status kifi_integrate_device(kifi_volume *vol, const uint16_t *depth_mm_dev,
                                   const kifi_intrinsics *cam, const float worldToEye[16]);
+/* The same with depth in binary32 metres, used as it is: the Integrate of a
+ * vector2d<float> frame (Integrator.h:19-26), e.g. DepthStream's FLOAT texels
+ * (DepthStream.cpp:78-80).  A uint16 frame is the special case mm * 0.001f. */
+kifi_status kifi_integrate_f32(kifi_volume *vol, const float *depth_m, const kifi_intrinsics *cam,
+                               const float worldToEye[16]);
+kifi_status kifi_integrate_device_f32(kifi_volume *vol, const float *depth_m_dev,
+                                      const kifi_intrinsics *cam, const float worldToEye[16]);
 /* Counts of the last integration: [0] K keygen keys, [1] 8-neighbourhood candidates,
  * [2] E keys handed to the merge, [3] volume size after the merge.  Synchronous. */
 kifi_status kifi_integrate_stats(kifi_volume *vol, size_t stats[4]);
@@ -186,6 +193,11 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
                                   const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
                                   const float synthEyeToWorld[16], int iterations, float out[16],
                                   kifi_icp_stats *stats);
+/* The same with the raw frame in binary32 metres (ICP::Align's vector2d<float>). */
+kifi_status kifi_icp_align_points_f32(kifi_ctx *ctx, const float *depth_m, const kifi_intrinsics *cam,
+                                      const float guess[16], const float *synth_xyz, int synth_width,
+                                      int synth_height, const float synthEyeToWorld[16], int iterations,
+                                      float out[16], kifi_icp_stats *stats);
 /* Precision study: the reduced point-to-plane normal equations of one projective
  * association of the frame (host, mm) at eyeToWorld against the model, on
  * pyramid level `level` (0..2) with gate dist_thresh (m), weights w = 1/d^4:
@@ -216,6 +228,11 @@ kifi_volume *kifi_pipeline_volume(kifi_pipeline *p);
 kifi_model  *kifi_pipeline_model(kifi_pipeline *p);
 kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth_mm, int depth_on_device,
                                     const float worldToEye[16]);
+/* The same with depth in binary32 metres (Pipeline::Integrate of a
+ * vector2d<float>, Pipeline.h:30).  A metre frame does not feed the learned
+ * initial pose (it reads mm); tracking starts from the previous pose. */
+kifi_status kifi_pipeline_integrate_f32(kifi_pipeline *p, const float *depth_m, int depth_on_device,
+                                        const float worldToEye[16]);
 /* Initial guess (eye->world) for the next tracked frame instead of the previous
  * pose, e.g. a learned prediction; plain ICP from the previous pose remains the
  * fallback per kifi_icp_params. */

@@ -16,6 +16,8 @@ import weakref
 
 import numpy as np
 
+from .depthstream import DepthStream, write_depth  # noqa: E402,F401
+
 _PKG = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(_PKG, "libkifi_hip.so")
 SYNTH_LIB_PATH = os.path.join(_PKG, "libkifi_synth.so")
@@ -29,6 +31,7 @@ HIP_EXPORTS = [
     "kifi_ctx_create", "kifi_ctx_destroy", "kifi_last_error", "kifi_ctx_synchronize", "kifi_ctx_stream",
     "kifi_volume_create", "kifi_volume_destroy", "kifi_volume_clear", "kifi_volume_size",
     "kifi_volume_download", "kifi_volume_upload", "kifi_integrate", "kifi_integrate_device",
+    "kifi_integrate_f32", "kifi_integrate_device_f32", "kifi_pipeline_integrate_f32", "kifi_icp_align_points_f32",
     "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_bin",
     "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
@@ -130,6 +133,8 @@ def load_hip():
             "kifi_volume_upload": ([vp, vp, vp, sz], i32),
             "kifi_integrate": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_device": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
+            "kifi_integrate_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
+            "kifi_integrate_device_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_stats": ([vp, szp], i32),
             "kifi_mesh": ([vp, f32p, sz, szp], i32),
             "kifi_mesh_indexed": ([vp, f32p, sz, ctypes.POINTER(ctypes.c_uint32), sz, szp, szp,
@@ -151,6 +156,7 @@ def load_hip():
             "kifi_pipeline_volume": ([vp], vp),
             "kifi_pipeline_model": ([vp], vp),
             "kifi_pipeline_integrate": ([vp, vp, i32, f32p], i32),
+            "kifi_pipeline_integrate_f32": ([vp, vp, i32, f32p], i32),
             "kifi_pipeline_set_guess": ([vp, f64p], i32),
             "kifi_pipeline_set_pose": ([vp, f64p], i32),
             "kifi_pipeline_pose": ([vp, f64p], i32),
@@ -165,6 +171,8 @@ def load_hip():
             "kifi_device_copy": ([vp, vp, vp, sz], i32),
             "kifi_integrate_totals": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
             "kifi_icp_align_points": ([vp, vp, ctypes.POINTER(Intrinsics), f32p, f32p, i32, i32, f32p, i32, f32p,
+                                       ctypes.POINTER(IcpStats)], i32),
+            "kifi_icp_align_points_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p, f32p, i32, i32, f32p, i32, f32p,
                                        ctypes.POINTER(IcpStats)], i32),
             "kifi_twistnet_param_count": ([], sz),
             "kifi_twistnet_create": ([vp, ctypes.c_uint64, ctypes.POINTER(vp)], i32),
@@ -442,20 +450,21 @@ class Volume:
         self.ctx.check(self.ctx._lib.kifi_volume_upload(self._h, k.ctypes.data, v.ctypes.data, k.shape[0]),
                        "kifi_volume_upload")
 
-    def integrate(self, depth_mm, cam, world_to_eye):
-        """Integrator::Integrate (Integrator.cpp:26-71); depth in mm (H, W) uint16."""
-        d = np.ascontiguousarray(depth_mm, np.uint16)
+    def integrate(self, depth, cam, world_to_eye):
+        """Integrator::Integrate (Integrator.cpp:26-71): depth (H, W) uint16 mm, or
+        float32 metres used as they are (a vector2d<float> frame)."""
+        metres = np.asarray(depth).dtype == np.float32
+        d = np.ascontiguousarray(depth, np.float32 if metres else np.uint16)
         if d.shape != (cam.height, cam.width):
             raise KifiError("frame shape does not match the intrinsics")
         m = np.ascontiguousarray(world_to_eye, np.float32).reshape(16)
-        self.ctx.check(self.ctx._lib.kifi_integrate(self._h, d.ctypes.data, ctypes.byref(cam), _f32p(m)),
-                       "kifi_integrate")
+        fn = self.ctx._lib.kifi_integrate_f32 if metres else self.ctx._lib.kifi_integrate
+        self.ctx.check(fn(self._h, d.ctypes.data, ctypes.byref(cam), _f32p(m)), "kifi_integrate")
 
-    def integrate_device(self, depth_ptr, cam, world_to_eye):
+    def integrate_device(self, depth_ptr, cam, world_to_eye, metres=False):
         m = np.ascontiguousarray(world_to_eye, np.float32).reshape(16)
-        self.ctx.check(self.ctx._lib.kifi_integrate_device(self._h, ctypes.c_void_p(depth_ptr),
-                                                           ctypes.byref(cam), _f32p(m)),
-                       "kifi_integrate_device")
+        fn = self.ctx._lib.kifi_integrate_device_f32 if metres else self.ctx._lib.kifi_integrate_device
+        self.ctx.check(fn(self._h, ctypes.c_void_p(depth_ptr), ctypes.byref(cam), _f32p(m)), "kifi_integrate_device")
 
     def totals(self):
         """Running sums since creation: integrations, V0, E, V1, K."""
@@ -574,9 +583,11 @@ def horn_align(ctx, src, dst):
     return out
 
 
-def icp_align_points(ctx, depth_mm, cam, guess_e2w, synth_xyz, synth_e2w, iterations=4):
-    """ICP::Align (ICP.cpp:9-265) against a host point buffer: (pose float32[16], stats)."""
-    d = np.ascontiguousarray(depth_mm, np.uint16)
+def icp_align_points(ctx, depth, cam, guess_e2w, synth_xyz, synth_e2w, iterations=4):
+    """ICP::Align (ICP.cpp:9-265) against a host point buffer: (pose float32[16], stats).
+    depth: uint16 mm or float32 metres."""
+    metres = np.asarray(depth).dtype == np.float32
+    d = np.ascontiguousarray(depth, np.float32 if metres else np.uint16)
     if d.shape != (cam.height, cam.width):
         raise KifiError("depth frame must be (height, width) of the camera")
     syn = np.ascontiguousarray(np.asarray(synth_xyz, np.float32)[..., :3])
@@ -584,9 +595,9 @@ def icp_align_points(ctx, depth_mm, cam, guess_e2w, synth_xyz, synth_e2w, iterat
     se = np.ascontiguousarray(synth_e2w, np.float32).reshape(16)
     out = np.zeros(16, np.float32)
     st = IcpStats()
-    ctx.check(ctx._lib.kifi_icp_align_points(ctx.handle, d.ctypes.data, ctypes.byref(cam), _f32p(g), _f32p(syn),
-                                             syn.shape[1], syn.shape[0], _f32p(se), iterations, _f32p(out),
-                                             ctypes.byref(st)), "kifi_icp_align_points")
+    fn = ctx._lib.kifi_icp_align_points_f32 if metres else ctx._lib.kifi_icp_align_points
+    ctx.check(fn(ctx.handle, d.ctypes.data, ctypes.byref(cam), _f32p(g), _f32p(syn), syn.shape[1], syn.shape[0],
+                 _f32p(se), iterations, _f32p(out), ctypes.byref(st)), "kifi_icp_align_points")
     return out, st.as_dict()
 
 
@@ -631,16 +642,19 @@ class Pipeline:
         except Exception:
             pass
 
-    def integrate(self, depth_mm, world_to_eye=None):
-        """Integrate at world_to_eye, or track with ICP when it is None."""
-        d = np.ascontiguousarray(depth_mm, np.uint16)
+    def integrate(self, depth, world_to_eye=None):
+        """Integrate at world_to_eye, or track with ICP when it is None.  depth: uint16
+        mm or float32 metres."""
+        metres = np.asarray(depth).dtype == np.float32
+        d = np.ascontiguousarray(depth, np.float32 if metres else np.uint16)
         m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
-        self.ctx.check(self.ctx._lib.kifi_pipeline_integrate(self._h, d.ctypes.data, 0, m), "kifi_pipeline_integrate")
+        fn = self.ctx._lib.kifi_pipeline_integrate_f32 if metres else self.ctx._lib.kifi_pipeline_integrate
+        self.ctx.check(fn(self._h, d.ctypes.data, 0, m), "kifi_pipeline_integrate")
 
-    def integrate_device(self, depth_ptr, world_to_eye=None):
+    def integrate_device(self, depth_ptr, world_to_eye=None, metres=False):
         m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
-        self.ctx.check(self.ctx._lib.kifi_pipeline_integrate(self._h, ctypes.c_void_p(depth_ptr), 1, m),
-                       "kifi_pipeline_integrate")
+        fn = self.ctx._lib.kifi_pipeline_integrate_f32 if metres else self.ctx._lib.kifi_pipeline_integrate
+        self.ctx.check(fn(self._h, ctypes.c_void_p(depth_ptr), 1, m), "kifi_pipeline_integrate")
 
     def set_twistnet(self, net):
         """Learned initial pose for tracked frames (None detaches)."""

@@ -20,10 +20,10 @@ public:
                          const util::vector2d<util::float3> &synthDepthBuffer,
                          const util::float4x4 &synthDepthBufferEyeToWorld, const DepthSensorParams &cameraParams)
     {
-        detail::metres_to_mm(rawDepthMap, m_mm);
         const kifi_intrinsics c = detail::frame_camera(cameraParams, rawDepthMap.width(), rawDepthMap.height());
         util::float4x4 out;
-        detail::check(kifi_icp_align_points(detail::context(), m_mm.data(), &c, rawDepthMapEyeToWorldGuess.data(),
+        detail::check(kifi_icp_align_points_f32(detail::context(), rawDepthMap.data(), &c,
+                                                rawDepthMapEyeToWorldGuess.data(),
                                             &synthDepthBuffer.data()->x, (int)synthDepthBuffer.width(),
                                             (int)synthDepthBuffer.height(), synthDepthBufferEyeToWorld.data(), 4,
                                             out.data(), &m_stats),
@@ -34,7 +34,6 @@ public:
     const kifi_icp_stats &LastStats() const { return m_stats; }
 
 private:
-    std::vector<uint16_t> m_mm;
     kifi_icp_stats m_stats{};
 };
 

@@ -45,9 +45,9 @@ public:
 
     void Integrate(const util::vector2d<float> &rawDepthMap, const util::float4x4 &worldToEye)
     {
-        detail::metres_to_mm(rawDepthMap, m_mm);
         detail::frame_camera(m_camParams, rawDepthMap.width(), rawDepthMap.height());
-        detail::check(kifi_pipeline_integrate(m_h, m_mm.data(), 0, worldToEye.data()), "kifi_pipeline_integrate");
+        detail::check(kifi_pipeline_integrate_f32(m_h, rawDepthMap.data(), 0, worldToEye.data()),
+                      "kifi_pipeline_integrate");
         m_eyeToWorld = worldToEye;
         util::invert_transform(m_eyeToWorld);
     }
@@ -55,9 +55,8 @@ public:
     // tracking: the pose comes from ICP against the previous frame's model
     void Integrate(const util::vector2d<float> &rawDepthMap)
     {
-        detail::metres_to_mm(rawDepthMap, m_mm);
         detail::frame_camera(m_camParams, rawDepthMap.width(), rawDepthMap.height());
-        detail::check(kifi_pipeline_integrate(m_h, m_mm.data(), 0, nullptr), "kifi_pipeline_integrate");
+        detail::check(kifi_pipeline_integrate_f32(m_h, rawDepthMap.data(), 0, nullptr), "kifi_pipeline_integrate");
         m_eyeToWorld = Pose();
     }
 
@@ -101,7 +100,6 @@ private:
     kifi::Volume *m_volume = nullptr;
     Mesher m_mesher;
     util::float4x4 m_eyeToWorld;
-    std::vector<uint16_t> m_mm;
 };
 
 } // namespace kifi

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "kifi/DepthSensorParams.h"
+#include "kifi/DepthStream.h"
 #include "kifi/ICP.h"
 #include "kifi/Integrator.h"
 #include "kifi/Mesher.h"

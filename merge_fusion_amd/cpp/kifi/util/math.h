@@ -94,6 +94,18 @@ inline float4x4 operator*(const float4x4 &m, const float4x4 &n)
 
 inline float4 homogenize(float4 v) { return float4(v.x / v.w, v.y / v.w, v.z / v.w, 1.0f); }
 
+// util::transpose (math.h:627-630)
+inline void transpose(float4x4 &m)
+{
+    float *a = m.data();
+    for (int r = 0; r < 4; ++r)
+        for (int c = r + 1; c < 4; ++c) {
+            const float t = a[r + 4 * c];
+            a[r + 4 * c] = a[c + 4 * r];
+            a[c + 4 * r] = t;
+        }
+}
+
 // In-place inverse of a rigid transform: [R | t] -> [R^T | -R^T t]
 inline void invert_transform(float4x4 &m)
 {

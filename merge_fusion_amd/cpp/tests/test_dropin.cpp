@@ -186,6 +186,41 @@ int main(int argc, char **argv)
         pipe.Mesh(verts);
         check(!verts.empty(), "pipeline_mesh_nonempty");
     }
+    // ---- DepthStream + Integrator + Mesher on a .depth file (the reference's
+    // UnitTests/Integrator.cpp:21-52 scenario: Volume(256, 2, 0.02), KinectParams
+    // 640x480 far), when a file is given as argv[2]
+    if (argc >= 3) {
+        DepthStream ds(argv[2]);
+        check(ds.Good() && ds.FrameWidth() == 640 && ds.FrameHeight() == 480, "depthstream_header");
+        util::vector2d<float> depth;
+        float4x4 view;
+        const bool got = ds.NextFrame(depth, view);
+        check(got, "depthstream_first_frame");
+        if (got) {
+            dump("ds_depth_m.f32", depth.data(), depth.size());
+            dump("ds_view.f32", view.data(), 16);
+            Volume vol(256, 2.0f, 0.02f);
+            Integrator integrator;
+            integrator.Integrate(vol, depth,
+                                 DepthSensorParams::KinectParams(KinectDepthSensorResolution640x480,
+                                                                 KinectDepthSensorModeFar),
+                                 view);
+            VolumeData data = vol.Data();
+            check(data.size() > 1000, "depthstream_volume_nonempty");
+            dump("ds_keys.u32", data.keys_cbegin(), data.size());
+            dump("ds_values.f32x2", data.values_cbegin(), data.size());
+            Mesher mesher;
+            std::vector<float3> verts;
+            mesher.Mesh(vol, verts);
+            dump("ds_verts.f32x3", &verts[0].x, verts.size() * 3);
+        }
+        // the file holds no more frames than its header says, and none past its end
+        int n = got ? 1 : 0;
+        while (ds.NextFrame(depth, view))
+            ++n;
+        check(n <= ds.FrameCount(), "depthstream_frames_within_header");
+        std::printf("depthstream frames read %d of %d\n", n, ds.FrameCount());
+    }
     std::printf("DONE failures=%d\n", g_fail);
     return g_fail;
 }

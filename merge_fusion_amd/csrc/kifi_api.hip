@@ -8,7 +8,7 @@
 
 namespace kifi {
 
-kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+kifi_status integrate_dispatch(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
                           size_t *nverts, size_t *nidx, int *stalled);
@@ -457,21 +457,24 @@ static kifi_status check_cam(kifi_ctx *ctx, const kifi_intrinsics *cam)
     return KIFI_OK;
 }
 
-kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, const kifi_intrinsics *cam,
-                                  const float worldToEye[16])
+static kifi_status integrate_device(kifi_volume *v, const void *depth_dev, bool metres, const kifi_intrinsics *cam,
+                                   const float worldToEye[16])
 {
     if (!v || !depth_dev || !worldToEye)
         return KIFI_ERR_ARG;
     kifi_status st = check_cam(v->ctx, cam);
     if (st != KIFI_OK)
         return st;
-    return integrate_dispatch(v, depth_dev, cam, worldToEye, nullptr, nullptr);
+    DepthIn d;
+    d.p = depth_dev;
+    d.metres = metres;
+    return integrate_dispatch(v, d, cam, worldToEye, nullptr, nullptr);
 }
 
-kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_intrinsics *cam,
-                           const float worldToEye[16])
+static kifi_status integrate_host(kifi_volume *v, const void *depth, size_t texel, const kifi_intrinsics *cam,
+                                  const float worldToEye[16])
 {
-    if (!v || !depth_mm || !worldToEye)
+    if (!v || !depth || !worldToEye)
         return KIFI_ERR_ARG;
     kifi_ctx *ctx = v->ctx;
     kifi_status st = check_cam(ctx, cam);
@@ -484,11 +487,38 @@ kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_
             KIFI_HIP(ctx, hipFree(ctx->d_depth));
         ctx->d_depth = nullptr;
         ctx->depth_pixels = 0;
-        KIFI_HIP(ctx, hipMalloc(&ctx->d_depth, P * 2));
+        KIFI_HIP(ctx, hipMalloc(&ctx->d_depth, P * 4));
         ctx->depth_pixels = P;
     }
-    KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream));
-    return integrate_dispatch(v, ctx->d_depth, cam, worldToEye, nullptr, nullptr);
+    KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth, P * texel, hipMemcpyHostToDevice, ctx->stream));
+    DepthIn d;
+    d.p = ctx->d_depth;
+    d.metres = texel == 4;
+    return integrate_dispatch(v, d, cam, worldToEye, nullptr, nullptr);
+}
+
+kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, const kifi_intrinsics *cam,
+                                  const float worldToEye[16])
+{
+    return integrate_device(v, depth_dev, false, cam, worldToEye);
+}
+
+kifi_status kifi_integrate(kifi_volume *v, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                           const float worldToEye[16])
+{
+    return integrate_host(v, depth_mm, 2, cam, worldToEye);
+}
+
+kifi_status kifi_integrate_device_f32(kifi_volume *v, const float *depth_dev, const kifi_intrinsics *cam,
+                                      const float worldToEye[16])
+{
+    return integrate_device(v, depth_dev, true, cam, worldToEye);
+}
+
+kifi_status kifi_integrate_f32(kifi_volume *v, const float *depth_m, const kifi_intrinsics *cam,
+                               const float worldToEye[16])
+{
+    return integrate_host(v, depth_m, 4, cam, worldToEye);
 }
 
 kifi_status kifi_integrate_stats(kifi_volume *v, size_t stats[4])

@@ -788,11 +788,12 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 // state control kernels
 // Frame start: the state from the guess (one thread) and the level-0 depth map
 // in metres (DepthStream.cpp:89-90), in one launch.
+template <typename D>
 __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val,
-                            const uint16_t *__restrict__ mm, int n, float *__restrict__ lv0)
+                            const D *__restrict__ depth, int n, float *__restrict__ lv0)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        lv0[i] = (float)mm[i] * 0.001f;
+        lv0[i] = depth_m(depth, i);
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     const double *g = guess_dev ? guess_dev : guess_val.m;
@@ -1117,7 +1118,7 @@ void icp_free(IcpWork *w)
 static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels);
 
 // Runs the whole pyramid schedule asynchronously on the context's stream.
-kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
+kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip, Mat4 *w2e_out)
 {
     kifi_ctx *ctx = m->ctx;
@@ -1128,8 +1129,12 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const
     for (int i = 0; i < 16; ++i)
         gv.m[i] = guess_host ? guess_host[i] : 0.0;
     const int t_icp = timer_begin(ctx, KIFI_STAGE_ICP);
-    hipLaunchKernelGGL(k_icp_begin, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv, d_depth_mm,
-                       P, w->lv[0]);
+    if (depth.metres)
+        hipLaunchKernelGGL(k_icp_begin<float>, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv,
+                           (const float *)depth.p, P, w->lv[0]);
+    else
+        hipLaunchKernelGGL(k_icp_begin<uint16_t>, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv,
+                           (const uint16_t *)depth.p, P, w->lv[0]);
     const int levels = ip->levels < 1 ? 1 : (ip->levels > 3 ? 3 : ip->levels);
     if (levels == 3)
         hipLaunchKernelGGL(k_depth_down2, dim3((W / 4 * (H / 4) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,

@@ -95,9 +95,17 @@ kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, 
 kifi_status model_from_depth(kifi_model *m, const uint16_t *d_mm, const float e2w[16], const float w2e[16]);
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 void        icp_free(IcpWork *w);
-kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
+kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
                     Mat4 *w2e_out = nullptr);
+inline kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
+                           const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
+                           Mat4 *w2e_out = nullptr)
+{
+    DepthIn d;
+    d.p = d_depth_mm;
+    return icp_run(m, w, d, cam, guess_host, guess_dev, ip, w2e_out);
+}
 void        icp_w2e(IcpWork *w, Mat4 *d_out, hipStream_t st);
 void        store_w2e(const float *w2e, Mat4 *d_out, hipStream_t st);
 kifi_status horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16]);

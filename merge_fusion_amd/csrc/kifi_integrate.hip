@@ -57,12 +57,14 @@ struct KeygenParams {
 // the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
 // 218, 242).  Output order is irrelevant: the sort that follows fixes it.
 // pixel_key: the key of one pixel (Integrator.cpp:109-178); false when it emits none.
-template <typename K>
-__device__ __forceinline__ bool pixel_key(const uint16_t *__restrict__ depth, const KeygenParams &p, int pix,
-                                          K &key)
+template <typename K, typename D>
+__device__ __forceinline__ bool pixel_key(const D *__restrict__ depth, const KeygenParams &p, int pix,
+                                          K &key, float *__restrict__ dm_out)
 {
     const int v = pix / p.W, u = pix - v * p.W;
-    const float d = (float)depth[pix] * 0.001f;                   // DepthStream.cpp:89-90
+    const float d = depth_m(depth, pix);                          // DepthStream.cpp:89-90
+    if (dm_out)
+        dm_out[pix] = d;                                          // the metre frame of the update
     if (!(d > 0.0f))
         return false;
     float q[4] = { (float)u + 0.0f, 0.0f + -(float)v, 0.0f + -1.0f, 0.0f + 0.0f };
@@ -88,10 +90,11 @@ __device__ __forceinline__ bool pixel_key(const uint16_t *__restrict__ depth, co
 
 constexpr int kKgItems = 8;   // pixels per thread: one counter atomic per 2048 pixels
 
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict__ depth,
+// dm_out (uint16 frames): receives the frame in metres for the voxel update
+template <typename K, typename D>
+__global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth,
                                                      KeygenParams p, K *__restrict__ out,
-                                                     FrameCounters *__restrict__ cnt)
+                                                     FrameCounters *__restrict__ cnt, float *__restrict__ dm_out)
 {
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     __shared__ uint32_t base_s;
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const uint16_t *__restrict_
     for (int j = 0; j < kKgItems; ++j) {
         const int pix = (blockIdx.x * kKgItems + j) * kThreads + threadIdx.x;
         K k = 0;
-        const bool ok = pix < P && pixel_key<K>(depth, p, pix, k);
+        const bool ok = pix < P && pixel_key<K, D>(depth, p, pix, k, dm_out);
         keys[j] = k;
         flags |= (uint32_t)ok << j;
         c += ok;
@@ -493,7 +496,7 @@ struct UpdateParams {
     const FrameMats *mats;   // device: w2c of this frame
     float vl, c0, trunc, ndcU, ndcV, maxU, maxV;
     int   W, H;
-    const uint16_t *depth;
+    const float *depth;      // the frame in metres
 };
 
 // Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356), in two halves so
@@ -535,18 +538,17 @@ __device__ __forceinline__ int voxel_pixel(K key, const UpdateParams &p, const M
     return -1;
 }
 
-// voxel_update: with the gathered depth (mm) of the voxel's pixel, true and the
+// voxel_update: with the gathered depth (m) of the voxel's pixel, true and the
 // clamped signed distance when the voxel receives an update (:319-323).  w is
 // recomputed: the same products and sums in the same order as mat_vec's row 3.
 template <typename K>
-__device__ __forceinline__ bool voxel_update(K key, uint32_t dmm, const UpdateParams &p, const Mat4 &w2c,
+__device__ __forceinline__ bool voxel_update(K key, float d, const UpdateParams &p, const Mat4 &w2c,
                                              float &sd_out)
 {
     uint32_t x, y, z;
     unpack_key<K>(key, x, y, z);
     const float cx = voxel_centre<K>(x, p), cy = voxel_centre<K>(y, p), cz = voxel_centre<K>(z, p);
     const float kw = (w2c.m[3] * cx + w2c.m[7] * cy) + (w2c.m[11] * cz + w2c.m[15] * 1.0f);
-    const float d = (float)dmm * 0.001f;
     const float sd = d - kw;                                       // :319-320
     if (d > 0.0f && sd >= -p.trunc) {                              // :322
         sd_out = p.trunc < sd ? p.trunc : sd;                      // std::min, :323
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     }
     // unconditional loads (pixel 0 stands in for "none"): a load under a branch
     // would be waited for before its value merges, one gather at a time
-    uint32_t dmm[PER];
+    float dmm[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q)
         dmm[q] = up.depth[pix[q] >= 0 ? pix[q] : 0];
@@ -862,6 +864,7 @@ struct IntegrateScratch {
     uint32_t *rs_counts;     // [256][Tb]
     uint32_t *rs_totals;     // [passes][256]
     uint32_t *uq_tiles;      // [Tb]
+    float    *depth_m;       // [P] a uint16 frame in metres
     size_t    bytes;
 };
 
@@ -878,12 +881,13 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     s.rs_counts = (uint32_t *)(base + off); off += align256(256 * Tb * 4);
     s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * 256 * 4);
     s.uq_tiles = (uint32_t *)(base + off); off += align256((Tb + 1) * 4);
+    s.depth_m = (float *)(base + off); off += align256(P * 4);
     s.bytes = off;
     return s;
 }
 
 template <typename K, typename V>
-static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                   const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat)
 {
     kifi_ctx *ctx = vol->ctx;
@@ -942,9 +946,14 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     kp.vc = vol->consts;
     kp.mats = vol->d_mats;
     const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);
-    hipLaunchKernelGGL(k_keygen<K>, dim3((unsigned)((P + kThreads * kKgItems - 1) / (kThreads * kKgItems))),
-                       dim3(kThreads), 0,
-                       stream, d_depth, kp, (K *)s.candA, cnt);
+    const dim3 kg_grid((unsigned)((P + kThreads * kKgItems - 1) / (kThreads * kKgItems)));
+    const float *frame_m = d_depth.metres ? (const float *)d_depth.p : s.depth_m;
+    if (d_depth.metres)
+        hipLaunchKernelGGL((k_keygen<K, float>), kg_grid, dim3(kThreads), 0, stream, (const float *)d_depth.p, kp,
+                           (K *)s.candA, cnt, (float *)nullptr);
+    else
+        hipLaunchKernelGGL((k_keygen<K, uint16_t>), kg_grid, dim3(kThreads), 0, stream,
+                           (const uint16_t *)d_depth.p, kp, (K *)s.candA, cnt, s.depth_m);
 
     // radix sort of the candidates (even pass count: result back in candA)
     K *src = (K *)s.candA, *dst = (K *)s.candB;
@@ -986,7 +995,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     up.maxU = (float)W;                                            // :265
     up.maxV = (float)H;
     up.W = W; up.H = H;
-    up.depth = d_depth;
+    up.depth = frame_m;
     const int t_mu = timer_begin(ctx, KIFI_STAGE_MERGE_UPDATE);
     if (splat)
         hipLaunchKernelGGL((k_merge_update<K, V, true>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
@@ -1008,7 +1017,7 @@ static kifi_status integrate_impl(kifi_volume *vol, const uint16_t *d_depth, con
     return KIFI_OK;
 }
 
-kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+kifi_status integrate_dispatch(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat)
 {
     if (vol->desc.key_bits == 32) {

@@ -150,6 +150,16 @@ struct FrameMats {
     Mat4 e2w;   // invert_transform(worldToEye)
 };
 
+// A depth frame on the device: uint16 millimetres (converted as DepthStream.cpp:89-90,
+// mm * 0.001f) or binary32 metres (DepthStream's FLOAT texels, used as they are).
+struct DepthIn {
+    const void *p = nullptr;
+    bool metres = false;
+};
+
+__host__ __device__ __forceinline__ float depth_m(const uint16_t *d, size_t i) { return (float)d[i] * 0.001f; }
+__host__ __device__ __forceinline__ float depth_m(const float *d, size_t i) { return d[i]; }
+
 // Optional model splat folded into the merge + update voxel pass.
 struct SplatParams {
     const Mat4 *w2e;                 // device: world->eye of this frame
@@ -185,7 +195,7 @@ struct kifi_ctx {
     size_t       scratch_bytes = 0;
     void        *out_scratch = nullptr;  // output staging (mesh, model maps)
     size_t       out_bytes = 0;
-    uint16_t    *d_depth = nullptr;   // one frame in mm
+    void        *d_depth = nullptr;   // one host frame uploaded (4 bytes per pixel)
     size_t       depth_pixels = 0;
     int          num_cus = 0;
     // stage timers (HIP event pairs recorded on `stream` while enabled)

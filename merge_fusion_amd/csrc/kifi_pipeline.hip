@@ -7,7 +7,7 @@
 #include "kifi_internal.hpp"
 
 namespace kifi {
-kifi_status integrate_dispatch(kifi_volume *vol, const uint16_t *d_depth, const kifi_intrinsics *cam,
+kifi_status integrate_dispatch(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st);
 bool twistnet_guess(kifi_twistnet *n, const uint16_t *d_mm, int W, int H, const double *prev_pose_dev,
@@ -26,7 +26,7 @@ struct kifi_pipeline {
     kifi_icp_params  params{};
     IcpWork          work;
     Mat4            *d_w2e = nullptr;   // world->eye of the current frame (binary32)
-    uint16_t        *d_depth = nullptr; // staging for host frames
+    void            *d_depth = nullptr; // staging for host frames (mm or metres)
     double          *d_guess = nullptr;
     bool             have_guess = false;
     kifi_twistnet   *net = nullptr;     // learned initial guess (optional, not owned)
@@ -233,12 +233,12 @@ kifi_status kifi_icp_normal_equations(kifi_model *m, const uint16_t *depth_mm, c
     return s;
 }
 
-kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const kifi_intrinsics *cam,
-                                  const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
-                                  const float synthEyeToWorld[16], int iterations, float out[16],
-                                  kifi_icp_stats *stats)
+static kifi_status align_points(kifi_ctx *ctx, const void *depth, bool metres, const kifi_intrinsics *cam,
+                                const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
+                                const float synthEyeToWorld[16], int iterations, float out[16],
+                                kifi_icp_stats *stats)
 {
-    if (!ctx || !depth_mm || !guess || !synth_xyz || !synthEyeToWorld || !out || iterations < 0)
+    if (!ctx || !depth || !guess || !synth_xyz || !synthEyeToWorld || !out || iterations < 0)
         return KIFI_ERR_ARG;
     kifi_status s = bad_cam(ctx, cam);
     if (s != KIFI_OK)
@@ -259,8 +259,9 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
     m.mp.cx = cam->cx;
     m.mp.cy = cam->cy;
     IcpWork w;
-    uint16_t *d_depth = nullptr;
+    void *d_depth = nullptr;
     const size_t P = (size_t)cam->width * cam->height;
+    const size_t texel = metres ? 4 : 2;
     s = model_alloc(&m);
     if (s == KIFI_OK) {
         float w2e[16];
@@ -272,9 +273,9 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
     if (s == KIFI_OK)
         s = icp_alloc(&w, ctx, cam->width, cam->height);
     if (s == KIFI_OK) {
-        hipError_t e = hipMalloc(&d_depth, P * 2);
+        hipError_t e = hipMalloc(&d_depth, P * texel);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(d_depth, depth_mm, P * 2, hipMemcpyHostToDevice, ctx->stream);
+            e = hipMemcpyAsync(d_depth, depth, P * texel, hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess)
             s = hip_error(ctx, e, "kifi_icp_align_points upload");
     }
@@ -293,7 +294,10 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
         for (int i = 0; i < 16; ++i)
             g[i] = guess[i];
         icp_set_pose(&w, g, nullptr, ctx->stream);
-        s = icp_run(&m, &w, d_depth, cam, g, nullptr, &ip);
+        DepthIn din;
+        din.p = d_depth;
+        din.metres = metres;
+        s = icp_run(&m, &w, din, cam, g, nullptr, &ip);
         if (s == KIFI_OK) {
             hipError_t e = hipMemcpyAsync(&hs, w.d_state, sizeof hs, hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess)
@@ -314,6 +318,24 @@ kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const
     icp_free(&w);
     model_free(&m);
     return s;
+}
+
+kifi_status kifi_icp_align_points(kifi_ctx *ctx, const uint16_t *depth_mm, const kifi_intrinsics *cam,
+                                  const float guess[16], const float *synth_xyz, int synth_width, int synth_height,
+                                  const float synthEyeToWorld[16], int iterations, float out[16],
+                                  kifi_icp_stats *stats)
+{
+    return align_points(ctx, depth_mm, false, cam, guess, synth_xyz, synth_width, synth_height, synthEyeToWorld,
+                        iterations, out, stats);
+}
+
+kifi_status kifi_icp_align_points_f32(kifi_ctx *ctx, const float *depth_m, const kifi_intrinsics *cam,
+                                      const float guess[16], const float *synth_xyz, int synth_width,
+                                      int synth_height, const float synthEyeToWorld[16], int iterations,
+                                      float out[16], kifi_icp_stats *stats)
+{
+    return align_points(ctx, depth_m, true, cam, guess, synth_xyz, synth_width, synth_height, synthEyeToWorld,
+                        iterations, out, stats);
 }
 
 kifi_status kifi_horn_align(kifi_ctx *ctx, const float *src, const float *dst, size_t n, float out[16])
@@ -349,7 +371,7 @@ kifi_status kifi_pipeline_create(kifi_ctx *ctx, const kifi_volume_desc *vd, cons
     if (s == KIFI_OK) {
         hipError_t e = hipMalloc(&p->d_w2e, sizeof(Mat4));
         if (e == hipSuccess)
-            e = hipMalloc(&p->d_depth, (size_t)cam->width * cam->height * 2);
+            e = hipMalloc(&p->d_depth, (size_t)cam->width * cam->height * 4);
         if (e == hipSuccess)
             e = hipMalloc(&p->d_guess, 16 * sizeof(double));
         if (e != hipSuccess)
@@ -413,33 +435,40 @@ kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double e2w[16])
     return KIFI_OK;
 }
 
-kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int on_device, const float w2e[16])
+// One frame of the loop; depth is uint16 mm or binary32 metres (a metre frame
+// does not feed the learned initial pose, which reads mm).
+static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool metres, int on_device,
+                                      const float w2e[16])
 {
     if (!p || !depth)
         return KIFI_ERR_ARG;
     kifi_ctx *ctx = p->ctx;
     hipStream_t st = ctx->stream;
     const size_t P = (size_t)p->cam.width * p->cam.height;
-    const uint16_t *d_depth = depth;
+    DepthIn din;
+    din.p = depth;
+    din.metres = metres;
     if (!on_device) {
-        KIFI_HIP(ctx, hipMemcpyAsync(p->d_depth, depth, P * 2, hipMemcpyHostToDevice, st));
-        d_depth = p->d_depth;
+        KIFI_HIP(ctx, hipMemcpyAsync(p->d_depth, depth, P * (metres ? 4 : 2), hipMemcpyHostToDevice, st));
+        din.p = p->d_depth;
     }
+    const uint16_t *d_depth = (const uint16_t *)din.p;
+    kifi_twistnet *net = metres ? nullptr : p->net;
     kifi_status s = KIFI_OK;
     bool tracked = false;     // icp_run wrote this frame's worldToEye
     if (w2e) {
         // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
-        if (p->net)
-            twistnet_observe(p->net, d_depth, p->cam.width, p->cam.height, st);
+        if (net)
+            twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
     } else if (p->frame > 0) {
         // Pipeline.cpp:37-39: track against the model of the previous frame,
         // from a caller's guess, the network's prediction, or the previous pose
         kifi_icp_params ip = p->params;
         const double *guess_dev = prev_pose_dev(&p->work);
         bool guessed = p->have_guess;
-        if (!guessed && p->net &&
-            twistnet_guess(p->net, d_depth, p->cam.width, p->cam.height, prev_pose_dev(&p->work), p->d_guess, st)) {
+        if (!guessed && net &&
+            twistnet_guess(net, d_depth, p->cam.width, p->cam.height, prev_pose_dev(&p->work), p->d_guess, st)) {
             guess_dev = p->d_guess;
             guessed = true;
         }
@@ -447,14 +476,14 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
             ip.fallback_rms = 0.0f;   // the guess is the previous pose: nothing to fall back to
             ip.fallback_iters = 0;
         }
-        s = icp_run(p->model, &p->work, d_depth, &p->cam, p->have_guess ? p->guess : nullptr,
+        s = icp_run(p->model, &p->work, din, &p->cam, p->have_guess ? p->guess : nullptr,
                     p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e);
         tracked = true;
         p->have_guess = false;
         if (s != KIFI_OK)
             return s;
-    } else if (p->net) {
-        twistnet_observe(p->net, d_depth, p->cam.width, p->cam.height, st);
+    } else if (net) {
+        twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
     }
     // Pipeline.cpp:41-46: worldToEye = invert(eyeToWorld), integrate; the
     // model for the next frame (Pipeline.cpp:48-52, seen from this pose) is
@@ -467,7 +496,7 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
     model_begin(p->model, p->vol, nullptr, p->d_w2e);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
     const SplatParams sp = model_splat_params(p->model, p->d_w2e);
-    s = integrate_dispatch(p->vol, d_depth, &p->cam, nullptr, p->d_w2e, &sp);
+    s = integrate_dispatch(p->vol, din, &p->cam, nullptr, p->d_w2e, &sp);
     if (s != KIFI_OK)
         return s;
     t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
@@ -476,6 +505,16 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
     KIFI_HIP(ctx, hipGetLastError());
     ++p->frame;
     return KIFI_OK;
+}
+
+kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int on_device, const float w2e[16])
+{
+    return pipeline_integrate(p, depth, false, on_device, w2e);
+}
+
+kifi_status kifi_pipeline_integrate_f32(kifi_pipeline *p, const float *depth_m, int on_device, const float w2e[16])
+{
+    return pipeline_integrate(p, depth_m, true, on_device, w2e);
 }
 
 kifi_status kifi_pipeline_pose(kifi_pipeline *p, double e2w[16])

@@ -72,6 +72,8 @@ def lib():
             "orc_volume_write": ([ctypes.POINTER(OVolume), U64P, vp, sz], i32),
             "orc_integrate": ([ctypes.POINTER(OVolume), ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(Camera),
                                F32P, ctypes.POINTER(ctypes.c_size_t)], sz),
+            "orc_integrate_f32": ([ctypes.POINTER(OVolume), F32P, ctypes.POINTER(Camera),
+                                   F32P, ctypes.POINTER(ctypes.c_size_t)], sz),
             "orc_voxel_update_f32": ([F32P, ctypes.c_float], None),
             "orc_voxel_distance_f32": ([F32P], ctypes.c_float),
             "orc_voxel_update_f16u8": ([ctypes.POINTER(ctypes.c_uint32), ctypes.c_float], None),
@@ -170,12 +172,17 @@ class Volume:
         if lib().orc_volume_write(self.h, u64p(k), v.ctypes.data, k.shape[0]):
             raise MemoryError
 
-    def integrate(self, depth_mm, cam, world_to_eye):
-        d = np.ascontiguousarray(depth_mm, np.uint16)
+    def integrate(self, depth, cam, world_to_eye):
+        """uint16 mm frames convert as DepthStream.cpp:89-90; float32 metres are used as they are."""
         m = np.ascontiguousarray(world_to_eye, np.float32).reshape(16)
         st = (ctypes.c_size_t * 4)()
-        r = lib().orc_integrate(self.h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
-                                ctypes.byref(camera(cam)), f32p(m), st)
+        if np.asarray(depth).dtype == np.float32:
+            d = np.ascontiguousarray(depth, np.float32)
+            r = lib().orc_integrate_f32(self.h, f32p(d), ctypes.byref(camera(cam)), f32p(m), st)
+        else:
+            d = np.ascontiguousarray(depth, np.uint16)
+            r = lib().orc_integrate(self.h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                    ctypes.byref(camera(cam)), f32p(m), st)
         if r == ctypes.c_size_t(-1).value:
             raise ValueError("orc_integrate rejected its input")
         return {"keys": st[0], "unique": st[1], "new": st[2], "size": st[3]}

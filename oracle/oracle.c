@@ -494,21 +494,43 @@ int orc_volume_write(orc_volume *v, const uint64_t *keys, const void *values, si
 
 /* Integrator::Integrate, Integrator.cpp:26-71, with the frame given in mm and
  * converted as DepthStream.cpp:89-90. */
+static size_t integrate_m(orc_volume *v, const float *depth, const orc_camera *cam, const float worldToEye[16],
+                          size_t stats[4]);
+
 size_t orc_integrate(orc_volume *v, const uint16_t *mm, const orc_camera *cam,
                      const float worldToEye[16], size_t stats[4])
 {
     const size_t P = (size_t)cam->width * cam->height;
-    if (cam->width % 4 != 0 || v->desc.res < 2)          /* :89-90 asserts */
+    float *depth = (float *)malloc((P ? P : 1) * sizeof(float));
+    if (!depth)
         return (size_t)-1;
-    float *depth = (float *)malloc(P * sizeof(float));
-    uint64_t *keys = (uint64_t *)malloc(8 * P * sizeof(uint64_t));
-    uint64_t *scr = (uint64_t *)malloc(8 * P * sizeof(uint64_t));
-    if (!depth || !keys || !scr) {
-        free(depth); free(keys); free(scr);
-        return (size_t)-1;
-    }
     for (size_t i = 0; i < P; ++i)
         depth[i] = (float)mm[i] * 0.001f;
+    size_t r = integrate_m(v, depth, cam, worldToEye, stats);
+    free(depth);
+    return r;
+}
+
+/* Integrate of a vector2d<float> frame in metres, used as it is (DepthStream's
+ * FLOAT texels, DepthStream.cpp:78-80). */
+size_t orc_integrate_f32(orc_volume *v, const float *depth_m, const orc_camera *cam,
+                         const float worldToEye[16], size_t stats[4])
+{
+    return integrate_m(v, depth_m, cam, worldToEye, stats);
+}
+
+static size_t integrate_m(orc_volume *v, const float *depth, const orc_camera *cam, const float worldToEye[16],
+                          size_t stats[4])
+{
+    const size_t P = (size_t)cam->width * cam->height;
+    if (cam->width % 4 != 0 || v->desc.res < 2)          /* :89-90 asserts */
+        return (size_t)-1;
+    uint64_t *keys = (uint64_t *)malloc(8 * P * sizeof(uint64_t));
+    uint64_t *scr = (uint64_t *)malloc(8 * P * sizeof(uint64_t));
+    if (!keys || !scr) {
+        free(keys); free(scr);
+        return (size_t)-1;
+    }
 
     float clip[16], w2c[16], e2w[16];
     orc_eye_to_clip_rh(cam, clip);
@@ -561,7 +583,7 @@ size_t orc_integrate(orc_volume *v, const uint16_t *mm, const orc_camera *cam,
     if (stats) {
         stats[0] = K; stats[1] = U; stats[2] = E; stats[3] = V1;
     }
-    free(depth); free(keys); free(scr);
+    free(keys); free(scr);
     return E;
 }
 

@@ -102,6 +102,9 @@ int         orc_volume_write(orc_volume *v, const uint64_t *keys, const void *va
  * K (keygen), U (unique), E (expanded), V1 (volume size after merge). */
 size_t orc_integrate(orc_volume *v, const uint16_t *depth_mm, const orc_camera *cam,
                      const float worldToEye[16], size_t stats[4]);
+/* the same with a frame in binary32 metres, used as it is */
+size_t orc_integrate_f32(orc_volume *v, const float *depth_m, const orc_camera *cam,
+                         const float worldToEye[16], size_t stats[4]);
 
 /* ---- voxel helpers ---- */
 void     orc_voxel_update_f32(float *sum_weight, float d);      /* Voxel.h:53-57 */

@@ -12,6 +12,7 @@ import pytest
 
 import merge_fusion_amd as mf
 import oracle as orc
+from test_depthstream import fixture
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "merge_fusion_amd", "cpp", "tests", "test_dropin")
@@ -32,7 +33,8 @@ def run(tmp_path_factory):
     if not os.path.exists(BIN):
         pytest.fail("C++ drop-in test program not built (python -m merge_fusion_amd.build)")
     out = str(tmp_path_factory.mktemp("dropin"))
-    r = subprocess.run([BIN, out], capture_output=True, text=True, timeout=300)
+    depth_file = fixture(out, "imrod_v2.depth")
+    r = subprocess.run([BIN, out, depth_file], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     return r, out
 
@@ -104,3 +106,27 @@ def test_pipeline_given_poses_bit_exact(run):
     vals = _load(out, "pipe_values.f32x2", np.float32).reshape(-1, 2)
     assert np.array_equal(keys.astype(np.uint64), ok)
     assert np.array_equal(vals.view(np.uint32), oval.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_depthstream_integrator_mesher_bit_exact(run):
+    """kifi::DepthStream + Integrator + Mesher on the reference's imrod_v2.depth
+    (UnitTests/Integrator.cpp:21-52, UnitTests/Mesher.cpp): the frame in metres as
+    the reference reads it, integrated and meshed bit for bit as the oracle."""
+    r, out = run
+    assert "depthstream frames read 1 of 1" in r.stdout
+    depth = _load(out, "ds_depth_m.f32", np.float32).reshape(480, 640)
+    view = _load(out, "ds_view.f32", np.float32)
+    with mf.DepthStream(os.path.join(out, "imrod_v2.depth")) as ds:
+        d, v = ds.next_frame()
+    assert np.array_equal(depth, d) and np.array_equal(view, v)
+    cam = mf.Intrinsics(640, 480, 571.26, 571.26)
+    cam.cx, cam.cy, cam.znear, cam.zfar = 320.0, 240.0, 0.8, 4.0
+    ov = orc.Volume(256, 2.0, 0.02)
+    ov.integrate(depth, cam, view)
+    ok, oval = ov.read()
+    assert np.array_equal(_load(out, "ds_keys.u32", np.uint32), ok.astype(np.uint32))
+    assert np.array_equal(_load(out, "ds_values.f32x2", np.float32).reshape(-1, 2).view(np.uint32),
+                          oval.view(np.uint32))
+    assert np.array_equal(_load(out, "ds_verts.f32x3", np.float32).reshape(-1, 3).view(np.uint32),
+                          ov.mesh().view(np.uint32))
