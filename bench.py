@@ -389,6 +389,19 @@ def main():
             traffic_note = "measured traffic/algorithmic ratio %.3f (%s, %s) x this run's algorithmic bytes" % (
                 pmc["traffic_over_alg"], pmc.get("launches"), pmc.get("source"))
 
+    # the same frames from HBM without meshing: the reference point of the host-frame pass
+    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
+    for f in range(args.warmup):
+        pipe.integrate_device(dframes.address(f * fbytes),
+                              mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for f in range(args.warmup, n):
+        pipe.integrate_device(dframes.address(f * fbytes))
+    ctx.synchronize()
+    fps_no_mesh = args.steps / (time.perf_counter() - t0)
+    pipe.close()
+
     # the same frames handed over from page-locked host memory (PCIe copy inside each step)
     hframes = ctx.host_empty(frames.shape, np.uint16)
     hframes[:] = frames
@@ -455,6 +468,7 @@ def main():
                              "with every stage timer on (%.1f frames/s)"
                              % (args.steps / roof["elapsed"], args.steps / diag["elapsed"]),
         "fps_with_h2d": round(fps_h2d, 3),
+        "fps_no_mesh": round(fps_no_mesh, 3),
         "learned_vs_plain": learned,
         "precision_variant_2mm": precision,
     }

@@ -83,6 +83,12 @@ kifi_status kifi_volume_size(kifi_volume *vol, size_t *n);        /* flat_map::s
 kifi_status kifi_volume_download(kifi_volume *vol, void *keys, void *values, size_t cap, size_t *n);
 /* Replaces the content with n sorted unique keys and their values. */
 kifi_status kifi_volume_upload(kifi_volume *vol, const void *keys, const void *values, size_t n);
+/* Checkpoint / resume: the volume's description, keys and values to a file, and
+ * a new volume from one (keys checked to be strictly increasing).  No
+ * counterpart in the reference; integration continues from a loaded volume as
+ * from the saved one.  Synchronous. */
+kifi_status kifi_volume_save(kifi_volume *vol, const char *path);
+kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out);
 
 /* ---- integration: kifi::Integrator::Integrate (Integrator.h:19-26, Integrator.cpp:26-71) ----
  * depth_mm: host pointer (kifi_integrate) or device pointer already in HBM

@@ -31,7 +31,7 @@ HIP_EXPORTS = [
     "kifi_ctx_create", "kifi_ctx_destroy", "kifi_last_error", "kifi_ctx_synchronize", "kifi_ctx_stream",
     "kifi_volume_create", "kifi_volume_destroy", "kifi_volume_clear", "kifi_volume_size",
     "kifi_volume_download", "kifi_volume_upload", "kifi_integrate", "kifi_integrate_device",
-    "kifi_integrate_f32", "kifi_integrate_device_f32", "kifi_pipeline_integrate_f32", "kifi_icp_align_points_f32",
+    "kifi_volume_save", "kifi_volume_load", "kifi_integrate_f32", "kifi_integrate_device_f32", "kifi_pipeline_integrate_f32", "kifi_icp_align_points_f32",
     "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_bin",
     "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
@@ -134,6 +134,8 @@ def load_hip():
             "kifi_integrate": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_device": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
+            "kifi_volume_save": ([vp, ctypes.c_char_p], i32),
+            "kifi_volume_load": ([vp, ctypes.c_char_p, ctypes.POINTER(vp)], i32),
             "kifi_integrate_device_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_stats": ([vp, szp], i32),
             "kifi_mesh": ([vp, f32p, sz, szp], i32),
@@ -384,6 +386,18 @@ class DeviceBuffer:
             pass
 
 
+def load_volume(ctx, path):
+    """A volume from a kifi_volume_save file (description, keys and values)."""
+    with open(path, "rb") as f:
+        head = f.read(36)
+    if len(head) < 36 or head[:8] != b"KIFIVOL\n":
+        raise KifiError("not a volume file: %s" % path)
+    res, = np.frombuffer(head[12:16], "<i4")
+    side, trunc = np.frombuffer(head[16:24], "<f4")
+    kb, storage = np.frombuffer(head[24:32], "<i4")
+    return Volume(ctx, int(res), float(side), float(trunc), int(kb), int(storage), _path=path)
+
+
 class HostBuffer:
     def __init__(self, ctx, ptr, nbytes):
         self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
@@ -399,12 +413,15 @@ class HostBuffer:
 class Volume:
     """kifi::Volume (Volume.h:15-31) resident on the GPU."""
 
-    def __init__(self, ctx, res, side, trunc, key_bits=32, storage=STORAGE_F32):
+    def __init__(self, ctx, res, side, trunc, key_bits=32, storage=STORAGE_F32, _path=None):
         self.ctx = ctx
         self.desc = VolumeDesc(res, side, trunc, key_bits, storage)
         h = ctypes.c_void_p()
-        ctx.check(ctx._lib.kifi_volume_create(ctx.handle, ctypes.byref(self.desc), ctypes.byref(h)),
-                  "kifi_volume_create")
+        if _path is None:
+            ctx.check(ctx._lib.kifi_volume_create(ctx.handle, ctypes.byref(self.desc), ctypes.byref(h)),
+                      "kifi_volume_create")
+        else:
+            ctx.check(ctx._lib.kifi_volume_load(ctx.handle, os.fsencode(_path), ctypes.byref(h)), "kifi_volume_load")
         self._h = h
         ctx._adopt(self)
         self.key_dtype = np.uint32 if key_bits == 32 else np.uint64
@@ -433,6 +450,10 @@ class Volume:
 
     def clear(self):
         self.ctx.check(self.ctx._lib.kifi_volume_clear(self._h), "kifi_volume_clear")
+
+    def save(self, path):
+        """Checkpoint to a file (kifi_volume_save)."""
+        self.ctx.check(self.ctx._lib.kifi_volume_save(self._h, os.fsencode(path)), "kifi_volume_save")
 
     def download(self):
         """(keys, values) of the sorted flat map."""

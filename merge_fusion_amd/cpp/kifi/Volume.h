@@ -6,6 +6,9 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "kifi/Voxel.h"
@@ -111,6 +114,35 @@ public:
 
     const kifi_volume_desc &Desc() const { return m_desc; }
     kifi_volume *Handle() const { return m_h; }
+
+    // checkpoint / resume (kifi_volume_save / kifi_volume_load; no reference counterpart)
+    void Save(const char *path) const { detail::check(kifi_volume_save(m_h, path), "kifi_volume_save"); }
+    static Volume Load(const char *path)
+    {
+        kifi_volume *h = nullptr;
+        detail::check(kifi_volume_load(detail::context(), path, &h), "kifi_volume_load");
+        // the description from the file's header: magic, version, res, side, trunc, key bits, storage
+        kifi_volume_desc d{};
+        char head[32];
+        FILE *f = std::fopen(path, "rb");
+        const bool ok = f && std::fread(head, 1, 32, f) == 32;
+        if (f)
+            std::fclose(f);
+        if (!ok) {
+            kifi_volume_destroy(h);
+            std::fprintf(stderr, "kifi: cannot read %s\n", path);
+            std::abort();
+        }
+        int32_t i;
+        std::memcpy(&i, head + 12, 4); d.res = i;
+        std::memcpy(&d.side, head + 16, 4);
+        std::memcpy(&d.trunc, head + 20, 4);
+        std::memcpy(&i, head + 24, 4); d.key_bits = i;
+        std::memcpy(&i, head + 28, 4); d.storage = i;
+        Volume v(h, d);
+        v.m_owned = true;
+        return v;
+    }
 
     // A volume owned by the library (a pipeline's); not destroyed by this object.
     static Volume Borrow(kifi_volume *h, const kifi_volume_desc &d) { return Volume(h, d); }

@@ -118,6 +118,16 @@ int main(int argc, char **argv)
         dump("c1_idx.u32", idx.data(), idx.size());
         const std::string obj = g_dir + "/c1_mesh.obj";
         Mesher::Mesh2Obj(iverts, idx, obj.c_str());
+
+        // ---- checkpoint / resume: the volume to a file and back
+        const std::string vf = g_dir + "/c1.kvol";
+        vol.Save(vf.c_str());
+        Volume back = Volume::Load(vf.c_str());
+        VolumeData d2 = back.Data();
+        check(back.Resolution() == 1024 && d2.size() == data.size() &&
+                  std::memcmp(d2.keys_cbegin(), data.keys_cbegin(), data.size() * 4) == 0 &&
+                  std::memcmp(d2.values_cbegin(), data.values_cbegin(), data.size() * sizeof(Voxel)) == 0,
+              "volume_save_load_round_trip");
     }
 
     // ---- Renderer::Bin + ICP::Align: the reference's tracking chain (Pipeline.cpp:37-52)

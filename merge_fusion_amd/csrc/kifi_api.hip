@@ -272,7 +272,14 @@ void kifi_ctx_destroy(kifi_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->out_scratch) (void)hipFree(ctx->out_scratch);
-    if (ctx->d_depth) (void)hipFree(ctx->d_depth);
+    if (ctx->copy_stream)
+        (void)hipStreamSynchronize(ctx->copy_stream);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
+        if (ctx->stage_copied[i]) (void)hipEventDestroy(ctx->stage_copied[i]);
+        if (ctx->stage_used[i]) (void)hipEventDestroy(ctx->stage_used[i]);
+    }
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     for (auto &v : ctx->timers)
         for (auto &pr : v) {
             (void)hipEventDestroy(pr.first);
@@ -448,6 +455,88 @@ kifi_status kifi_volume_upload(kifi_volume *v, const void *keys, const void *val
     return KIFI_OK;
 }
 
+// ---- volume file: checkpoint / resume
+// "KIFIVOL\n", uint32 version 1, int32 res, float side, float trunc, int32
+// key_bits, int32 storage, uint64 n, n keys, n values (the device layouts),
+// little endian.
+static const char kVolMagic[8] = { 'K', 'I', 'F', 'I', 'V', 'O', 'L', '\n' };
+
+kifi_status kifi_volume_save(kifi_volume *v, const char *path)
+{
+    if (!v || !path)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = v->ctx;
+    size_t n = 0;
+    kifi_status st = kifi_volume_download(v, nullptr, nullptr, 0, &n);
+    if (st != KIFI_OK && st != KIFI_ERR_CAPACITY)
+        return st;
+    std::vector<char> keys(n * v->key_bytes), vals(n * v->val_bytes);
+    size_t got = 0;
+    st = kifi_volume_download(v, keys.data(), vals.data(), n, &got);
+    if (st != KIFI_OK)
+        return st;
+    FILE *f = std::fopen(path, "wb");
+    if (!f)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_volume_save: cannot open the file");
+    const uint32_t version = 1;
+    const int32_t res = v->desc.res, kb = v->desc.key_bits, storage = v->desc.storage;
+    const float side = v->desc.side, trunc = v->desc.trunc;
+    const uint64_t n64 = n;
+    bool ok = std::fwrite(kVolMagic, 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 &&
+              std::fwrite(&res, 4, 1, f) == 1 && std::fwrite(&side, 4, 1, f) == 1 &&
+              std::fwrite(&trunc, 4, 1, f) == 1 && std::fwrite(&kb, 4, 1, f) == 1 &&
+              std::fwrite(&storage, 4, 1, f) == 1 && std::fwrite(&n64, 8, 1, f) == 1 &&
+              std::fwrite(keys.data(), 1, keys.size(), f) == keys.size() &&
+              std::fwrite(vals.data(), 1, vals.size(), f) == vals.size();
+    ok = std::fclose(f) == 0 && ok;
+    return ok ? KIFI_OK : set_error(ctx, KIFI_ERR_ARG, "kifi_volume_save: write failed");
+}
+
+kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out)
+{
+    if (!ctx || !path || !out)
+        return KIFI_ERR_ARG;
+    *out = nullptr;
+    FILE *f = std::fopen(path, "rb");
+    if (!f)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_volume_load: cannot open the file");
+    char magic[8];
+    uint32_t version = 0;
+    kifi_volume_desc d{};
+    int32_t res = 0, kb = 0, storage = 0;
+    uint64_t n = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kVolMagic, 8) == 0 &&
+              std::fread(&version, 4, 1, f) == 1 && version == 1 && std::fread(&res, 4, 1, f) == 1 &&
+              std::fread(&d.side, 4, 1, f) == 1 && std::fread(&d.trunc, 4, 1, f) == 1 &&
+              std::fread(&kb, 4, 1, f) == 1 && std::fread(&storage, 4, 1, f) == 1 && std::fread(&n, 8, 1, f) == 1;
+    d.res = res;
+    d.key_bits = kb;
+    d.storage = storage;
+    kifi_volume *v = nullptr;
+    kifi_status st = ok ? kifi_volume_create(ctx, &d, &v) : KIFI_ERR_ARG;
+    if (st == KIFI_OK) {
+        std::vector<char> keys(n * v->key_bytes), vals(n * v->val_bytes);
+        ok = std::fread(keys.data(), 1, keys.size(), f) == keys.size() &&
+             std::fread(vals.data(), 1, vals.size(), f) == vals.size();
+        // keys must be strictly increasing: the merge relies on it
+        for (uint64_t i = 1; ok && i < n; ++i) {
+            if (v->key_bytes == 4)
+                ok = ((const uint32_t *)keys.data())[i - 1] < ((const uint32_t *)keys.data())[i];
+            else
+                ok = ((const uint64_t *)keys.data())[i - 1] < ((const uint64_t *)keys.data())[i];
+        }
+        st = ok ? kifi_volume_upload(v, keys.data(), vals.data(), n) : KIFI_ERR_ARG;
+    }
+    std::fclose(f);
+    if (st != KIFI_OK) {
+        if (v)
+            kifi_volume_destroy(v);
+        return st == KIFI_ERR_ARG ? set_error(ctx, KIFI_ERR_ARG, "kifi_volume_load: not a valid volume file") : st;
+    }
+    *out = v;
+    return KIFI_OK;
+}
+
 static kifi_status check_cam(kifi_ctx *ctx, const kifi_intrinsics *cam)
 {
     // Integrator.cpp:35-36, 89-90 asserts
@@ -471,6 +560,55 @@ static kifi_status integrate_device(kifi_volume *v, const void *depth_dev, bool 
     return integrate_dispatch(v, d, cam, worldToEye, nullptr, nullptr);
 }
 
+} // extern "C"
+
+namespace kifi {
+
+kifi_status stage_frame(kifi_ctx *ctx, const void *host, size_t bytes, const void **dev, int *slot)
+{
+    if (!ctx->copy_stream) {
+        KIFI_HIP(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i) {
+            KIFI_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_copied[i], hipEventDisableTiming));
+            KIFI_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_used[i], hipEventDisableTiming));
+        }
+    }
+    if (bytes > ctx->stage_bytes) {
+        KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        KIFI_HIP(ctx, hipStreamSynchronize(ctx->copy_stream));
+        for (int i = 0; i < 2; ++i) {
+            if (ctx->stage[i])
+                KIFI_HIP(ctx, hipFree(ctx->stage[i]));
+            ctx->stage[i] = nullptr;
+            ctx->stage_busy[i] = false;
+        }
+        ctx->stage_bytes = 0;
+        for (int i = 0; i < 2; ++i)
+            KIFI_HIP(ctx, hipMalloc(&ctx->stage[i], bytes));
+        ctx->stage_bytes = bytes;
+    }
+    const int s = ctx->stage_slot;
+    ctx->stage_slot ^= 1;
+    if (ctx->stage_busy[s])       // the frame before last may still read this buffer
+        KIFI_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->stage_used[s], 0));
+    KIFI_HIP(ctx, hipMemcpyAsync(ctx->stage[s], host, bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+    KIFI_HIP(ctx, hipEventRecord(ctx->stage_copied[s], ctx->copy_stream));
+    KIFI_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->stage_copied[s], 0));
+    *dev = ctx->stage[s];
+    *slot = s;
+    return KIFI_OK;
+}
+
+void stage_done(kifi_ctx *ctx, int slot)
+{
+    if (hipEventRecord(ctx->stage_used[slot], ctx->stream) == hipSuccess)
+        ctx->stage_busy[slot] = true;
+}
+
+} // namespace kifi
+
+extern "C" {
+
 static kifi_status integrate_host(kifi_volume *v, const void *depth, size_t texel, const kifi_intrinsics *cam,
                                   const float worldToEye[16])
 {
@@ -481,20 +619,15 @@ static kifi_status integrate_host(kifi_volume *v, const void *depth, size_t texe
     if (st != KIFI_OK)
         return st;
     const size_t P = (size_t)cam->width * cam->height;
-    if (P > ctx->depth_pixels) {
-        KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->d_depth)
-            KIFI_HIP(ctx, hipFree(ctx->d_depth));
-        ctx->d_depth = nullptr;
-        ctx->depth_pixels = 0;
-        KIFI_HIP(ctx, hipMalloc(&ctx->d_depth, P * 4));
-        ctx->depth_pixels = P;
-    }
-    KIFI_HIP(ctx, hipMemcpyAsync(ctx->d_depth, depth, P * texel, hipMemcpyHostToDevice, ctx->stream));
     DepthIn d;
-    d.p = ctx->d_depth;
+    int slot = 0;
+    st = stage_frame(ctx, depth, P * texel, &d.p, &slot);
+    if (st != KIFI_OK)
+        return st;
     d.metres = texel == 4;
-    return integrate_dispatch(v, d, cam, worldToEye, nullptr, nullptr);
+    st = integrate_dispatch(v, d, cam, worldToEye, nullptr, nullptr);
+    stage_done(ctx, slot);
+    return st;
 }
 
 kifi_status kifi_integrate_device(kifi_volume *v, const uint16_t *depth_dev, const kifi_intrinsics *cam,

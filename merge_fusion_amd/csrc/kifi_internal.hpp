@@ -195,8 +195,15 @@ struct kifi_ctx {
     size_t       scratch_bytes = 0;
     void        *out_scratch = nullptr;  // output staging (mesh, model maps)
     size_t       out_bytes = 0;
-    void        *d_depth = nullptr;   // one host frame uploaded (4 bytes per pixel)
-    size_t       depth_pixels = 0;
+    // host frames: uploaded on a copy stream into two staging buffers in turn,
+    // so the upload of frame t+1 overlaps the kernels of frame t
+    hipStream_t  copy_stream = nullptr;
+    void        *stage[2] = { nullptr, nullptr };
+    size_t       stage_bytes = 0;
+    hipEvent_t   stage_copied[2] = { nullptr, nullptr };   // on copy_stream: the upload is done
+    hipEvent_t   stage_used[2] = { nullptr, nullptr };     // on stream: the frame's kernels are done
+    bool         stage_busy[2] = { false, false };
+    int          stage_slot = 0;
     int          num_cus = 0;
     // stage timers (HIP event pairs recorded on `stream` while enabled)
     unsigned     timing_mask = 0;   // bit s: time stage s
@@ -205,6 +212,11 @@ struct kifi_ctx {
 };
 
 namespace kifi {
+// Uploads a host frame of `bytes` into the next staging buffer on the copy stream
+// and makes the context's stream wait for it; *dev receives the device copy.
+// stage_done(ctx, slot) after the frame's last kernel releases the buffer.
+kifi_status stage_frame(kifi_ctx *ctx, const void *host, size_t bytes, const void **dev, int *slot);
+void        stage_done(kifi_ctx *ctx, int slot);
 // Records a start event for `stage` when timing is enabled; returns the slot.
 int  timer_begin(kifi_ctx *ctx, int stage);
 void timer_end(kifi_ctx *ctx, int stage, int slot);

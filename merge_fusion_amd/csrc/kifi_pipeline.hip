@@ -26,7 +26,6 @@ struct kifi_pipeline {
     kifi_icp_params  params{};
     IcpWork          work;
     Mat4            *d_w2e = nullptr;   // world->eye of the current frame (binary32)
-    void            *d_depth = nullptr; // staging for host frames (mm or metres)
     double          *d_guess = nullptr;
     bool             have_guess = false;
     kifi_twistnet   *net = nullptr;     // learned initial guess (optional, not owned)
@@ -371,8 +370,6 @@ kifi_status kifi_pipeline_create(kifi_ctx *ctx, const kifi_volume_desc *vd, cons
     if (s == KIFI_OK) {
         hipError_t e = hipMalloc(&p->d_w2e, sizeof(Mat4));
         if (e == hipSuccess)
-            e = hipMalloc(&p->d_depth, (size_t)cam->width * cam->height * 4);
-        if (e == hipSuccess)
             e = hipMalloc(&p->d_guess, 16 * sizeof(double));
         if (e != hipSuccess)
             s = hip_error(ctx, e, "kifi_pipeline_create");
@@ -399,7 +396,6 @@ void kifi_pipeline_destroy(kifi_pipeline *p)
     if (p->model) kifi_model_destroy(p->model);
     icp_free(&p->work);
     if (p->d_w2e) (void)hipFree(p->d_w2e);
-    if (p->d_depth) (void)hipFree(p->d_depth);
     if (p->d_guess) (void)hipFree(p->d_guess);
     delete p;
 }
@@ -448,10 +444,21 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
     DepthIn din;
     din.p = depth;
     din.metres = metres;
+    int slot = -1;            // a host frame goes through the context's copy stream
     if (!on_device) {
-        KIFI_HIP(ctx, hipMemcpyAsync(p->d_depth, depth, P * (metres ? 4 : 2), hipMemcpyHostToDevice, st));
-        din.p = p->d_depth;
+        kifi_status s0 = stage_frame(ctx, depth, P * (metres ? 4 : 2), &din.p, &slot);
+        if (s0 != KIFI_OK)
+            return s0;
     }
+    struct Release {          // the staging buffer is free once this frame's kernels are
+        kifi_ctx *ctx;
+        int slot;
+        ~Release()
+        {
+            if (slot >= 0)
+                stage_done(ctx, slot);
+        }
+    } release{ ctx, slot };
     const uint16_t *d_depth = (const uint16_t *)din.p;
     kifi_twistnet *net = metres ? nullptr : p->net;
     kifi_status s = KIFI_OK;

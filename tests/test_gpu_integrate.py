@@ -139,3 +139,64 @@ def test_update_covers_floor_v_over_4(ctx):
     n = len(k)
     assert np.all(v["weight"][n // 4 * 4:] == 0)
     assert np.all(v["weight"][: n // 4 * 4] <= 1)
+
+
+@pytest.mark.parametrize("key_bits,storage", [(32, mf.STORAGE_F32), (64, mf.STORAGE_F16U8)])
+def test_volume_checkpoint_resume(ctx, tmp_path, key_bits, storage):
+    """kifi_volume_save / kifi_volume_load: a volume saved after 3 frames and loaded
+    continues exactly as the uninterrupted one (and as the oracle)."""
+    res = 1024 if key_bits == 32 else 2048
+    a = mf.Volume(ctx, res, 4.096, 0.02, key_bits, storage)
+    ov = orc.Volume(res, 4.096, 0.02, 10 if key_bits == 32 else 21, storage)
+    frames = [frame(mf.SCENE_ROOM, 4, mf.TRAJ_CIRCLE, 2 * f) for f in range(5)]
+    for d, w2e in frames[:3]:
+        a.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    path = str(tmp_path / "v.kvol")
+    a.save(path)
+    b = mf.load_volume(ctx, path)
+    assert (b.desc.res, b.desc.key_bits, b.desc.storage) == (res, key_bits, storage)
+    for d, w2e in frames[3:]:
+        a.integrate(d, CAM, w2e)
+        b.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    assert_volume_equal(b, ov)
+    ka, va = a.download()
+    kb, vb = b.download()
+    assert np.array_equal(ka, kb) and np.array_equal(np.asarray(va).view(np.uint32), np.asarray(vb).view(np.uint32))
+    # a damaged file is refused
+    raw = bytearray(open(path, "rb").read())
+    raw[40:48] = raw[48:56]          # two keys out of order
+    bad = str(tmp_path / "bad.kvol")
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(mf.KifiError):
+        mf.load_volume(ctx, bad)
+
+
+def test_overlapped_host_frames_equal_device_frames(ctx):
+    """Host frames go up on the copy stream into two staging buffers in turn while
+    the previous frame computes.  Twelve tracked frames from page-locked memory,
+    enqueued back to back, give the same poses and volume as the same frames
+    already in HBM."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 5)
+    poses = [mf.trajectory_pose(mf.TRAJ_CIRCLE, 5, f) for f in range(12)]
+    frames = np.stack([mf.render(sc, CAM, p, noise_seed=70 + i) for i, p in enumerate(poses)])
+    pinned = ctx.host_empty(frames.shape, np.uint16)
+    pinned[:] = frames
+    dev = ctx.device_upload(frames)
+    fb = CAM.width * CAM.height * 2
+    w0 = mf.invert_rigid(poses[0]).astype(np.float32)
+    out = []
+    for host in (True, False):
+        pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+        for i in range(len(poses)):
+            if host:
+                pipe.integrate(pinned[i], w0 if i == 0 else None)
+            else:
+                pipe.integrate_device(dev.address(i * fb), w0 if i == 0 else None)
+        out.append((pipe.pose(), pipe.volume.download()))
+        pipe.close()
+    (pa, (ka, va)), (pb, (kb, vb)) = out
+    assert np.array_equal(pa, pb) and np.array_equal(ka, kb)
+    assert np.array_equal(np.asarray(va).view(np.uint32), np.asarray(vb).view(np.uint32))
+    dev.free()
