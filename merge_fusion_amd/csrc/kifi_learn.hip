@@ -269,6 +269,47 @@ struct OpFcBwdW {
     __device__ void store(int m, int n, float v, int z) const { part[((size_t)z * M + m) * N + n] = v; }
 };
 
+// Fully connected forward for a few rows (the pipeline's single-sample
+// prediction): one wave per output, the K products strided over the lanes in
+// 16-byte loads and summed in binary32 by a wave reduction.  The MFMA tile path
+// would run a handful of workgroups along the whole of K for M = 1.
+template <bool OUT_F32>
+__global__ __launch_bounds__(256) void k_fc_small(const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
+                                                  const float *__restrict__ bias, int M, int N, int K, int vec,
+                                                  uint16_t *__restrict__ Y, float *__restrict__ Yf)
+{
+    const int wave = (int)((blockIdx.x * 256u + threadIdx.x) / 64u), lane = threadIdx.x % 64;
+    if (wave >= M * N)
+        return;                                   // whole waves: the index is uniform per wave
+    const int m = wave / N, n = wave - m * N;
+    const uint16_t *x = X + (size_t)m * K, *w = W + (size_t)n * K;
+    float s = 0.0f;
+    if (vec) {                                    // K % 8 == 0 and 16-byte aligned rows
+        for (int k = lane * 8; k < K; k += 64 * 8) {
+            const uint4 xa = *reinterpret_cast<const uint4 *>(x + k), wa = *reinterpret_cast<const uint4 *>(w + k);
+            const uint32_t xs[4] = { xa.x, xa.y, xa.z, xa.w }, ws[4] = { wa.x, wa.y, wa.z, wa.w };
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s += bf2f((uint16_t)(xs[j] & 0xffffu)) * bf2f((uint16_t)(ws[j] & 0xffffu));
+                s += bf2f((uint16_t)(xs[j] >> 16)) * bf2f((uint16_t)(ws[j] >> 16));
+            }
+        }
+    } else {
+        for (int k = lane; k < K; k += 64)
+            s += bf2f(x[k]) * bf2f(w[k]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        const float v = s + bias[n];
+        if (OUT_F32)
+            Yf[(size_t)m * N + n] = v;
+        else
+            Y[(size_t)m * N + n] = f2bf(v > 0.0f ? v : 0.0f);
+    }
+}
+
 // fixed-order sum of split partials into the flat gradient
 __global__ void k_sum_splits(const float *__restrict__ part, int splits, int n, float *__restrict__ out)
 {
@@ -710,6 +751,8 @@ ConvGeom conv_geom(int l, int B)
     return g;
 }
 
+constexpr int kFcSmallRows = 4;   // batch sizes whose FC layers run as one wave per output
+
 // forward of B inputs already in net->act[0]; outputs net->y (B x 6, normalised)
 void net_forward(kifi_twistnet *n, int B, hipStream_t st)
 {
@@ -727,6 +770,19 @@ void net_forward(kifi_twistnet *n, int B, hipStream_t st)
         op.kchunk = (op.K + 15) / 16 * 16;
         launch_gemm(op, 1, st);
     }
+    const int K5 = (int)act_elems(5);
+    if (B <= kFcSmallRows) {
+        // a few rows: one wave per output instead of MFMA tiles along all of K
+        auto vec = [](const void *a, const void *b, int K) {
+            return (int)(K % 8 == 0 && ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0);
+        };
+        const uint16_t *w5 = n->pb + L.w[5], *w6 = n->pb + L.w[6];
+        hipLaunchKernelGGL(k_fc_small<false>, dim3((B * kFc1 * 64 + 255) / 256), dim3(256), 0, st, n->act[5], w5,
+                           n->p + L.b[5], B, kFc1, K5, vec(n->act[5], w5, K5), n->act[6], (float *)nullptr);
+        hipLaunchKernelGGL(k_fc_small<true>, dim3((B * kOut * 64 + 255) / 256), dim3(256), 0, st, n->act[6], w6,
+                           n->p + L.b[6], B, kOut, kFc1, vec(n->act[6], w6, kFc1), (uint16_t *)nullptr, n->y);
+        return;
+    }
     OpFcFwd f1;
     f1.X = n->act[5];
     f1.W = n->pb + L.w[5];
@@ -735,7 +791,7 @@ void net_forward(kifi_twistnet *n, int B, hipStream_t st)
     f1.Yf = nullptr;
     f1.M = B;
     f1.N = kFc1;
-    f1.K = (int)act_elems(5);
+    f1.K = K5;
     f1.kchunk = f1.K;
     launch_gemm(f1, 1, st);
     OpFcFwd f2;

@@ -130,3 +130,18 @@ def test_pipeline_with_learned_init_needs_fewer_iterations(ctx, trained):
     # the previous pose is up to 15 deg off: plain ICP loses the camera, the learned start keeps it
     assert out["learned"][2][0] < 0.15 and out["learned"][2][0] < out["plain"][2][0], out
     assert out["learned"][0] < out["plain"][0], out
+
+
+def test_single_sample_forward_matches_oracle(ctx):
+    """Batches of up to 4 (the pipeline's per-frame prediction) run their FC layers
+    one wave per output instead of MFMA tiles: the same oracle bound holds, and the
+    result agrees with the batched path."""
+    net = mf.TwistNet(ctx, 5)
+    x, _ = _net_inputs(3, 8)
+    p = tn.bf16_round(net.get_params()).astype(np.float64)
+    big = net.forward(x)
+    for b in (1, 3):
+        y = net.forward(x[:b])
+        yr = tn.forward(p, tn.bf16_round(x[:b]).astype(np.float64), store=tn.bf16_store)
+        assert np.abs(y - yr).max() <= 1e-2 * np.abs(yr).max()
+        assert np.abs(y - big[:b]).max() <= 1e-2 * np.abs(big[:b]).max()
