@@ -13,7 +13,9 @@
 //     cyclic Jacobi for Horn's 4x4), pose composed in FP64.  Every iteration is one
 //     kernel: the workgroup that finishes last reduces the partials and solves, and
 //     a device-side convergence flag turns the level's remaining launches into
-//     early exits, so a frame needs no host sync.
+//     early exits, so a frame needs no host sync.  The fallback pass behind a
+//     learned start is one launch whose workgroups hand the iterations over among
+//     themselves (it rarely runs, and then costs more per iteration).
 #include <cfloat>
 #include <vector>
 
@@ -353,11 +355,16 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 // workgroup acquires (one L2 invalidate per launch).  A full __threadfence()
 // here would invalidate the XCD's L2 once per workgroup and make every other
 // workgroup re-fetch the model maps from HBM.
-template <int MODE, bool COMP, int NV>
+// GEN (the multi-iteration kernel): nblocks workgroups take part (the first
+// ones of the grid), and the solving workgroup then releases its state writes
+// and sets st->gen = gen, which all workgroups wait for.
+template <int MODE, bool COMP, int NV, bool GEN = false>
 __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, CSum *partials, IcpState *st, int level,
-                                            const IcpSolveParams &sp, const ModelPose *mpose)
+                                            const IcpSolveParams &sp, const ModelPose *mpose, unsigned gen = 0,
+                                            int nblocks = 0)
 {
     __shared__ bool last;
+    const unsigned nb = GEN ? (unsigned)nblocks : gridDim.x;
     block_csum<NV, COMP>(acc, lds);
     if (threadIdx.x < NV)
         partials[(size_t)blockIdx.x * NV + threadIdx.x] = lds.result[threadIdx.x];
@@ -365,41 +372,43 @@ __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, C
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned t = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = t == gridDim.x - 1;
+        last = t == nb - 1;
     }
     __syncthreads();
     if (!last)
         return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // see every workgroup's partial
-    icp_finish<MODE, COMP>(partials, (int)gridDim.x, st, level, sp, mpose, lds);
-    if (threadIdx.x == 0)
+    icp_finish<MODE, COMP>(partials, (int)nb, st, level, sp, mpose, lds);
+    if (threadIdx.x == 0) {
         st->ticket = 0;
+        if (GEN) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&st->gen, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
-template <int MODE, bool COMP>
-__global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, const ModelPose *__restrict__ mpose,
-                                                   const float4 *__restrict__ mpts,
-                                                   const float4 *__restrict__ mnrm,
-                                                   IcpState *__restrict__ st, float dist_thresh,
-                                                   CSum *__restrict__ partials, int level, IcpSolveParams sp)
+// One association pass of a level with the live pose `pose`, strided over
+// nblocks workgroups (workgroups beyond them add nothing): the correspondence
+// sums of this thread in acc.
+template <int MODE, bool COMP, int NV>
+__device__ __forceinline__ void icp_accumulate(const IcpLevel &lv, const ModelParams &mp,
+                                               const ModelPose *__restrict__ mpose, const float4 *__restrict__ mpts,
+                                               const float4 *__restrict__ mnrm, const Mat4 &pose, float dist_thresh,
+                                               int nblocks, CSum (&acc)[NV])
 {
-    constexpr int NV = IcpNV<MODE>::nv;
-    constexpr int MODE_OF_KERNEL = MODE;
-    __shared__ CsumLds<NV> lds;
-    if (st->level_done[level])
-        return;
-    CSum acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j)
         acc[j] = CSum{ 0.0f, 0.0f };
-    for (int q = blockIdx.x * kIT + threadIdx.x; q < lv.W * lv.H; q += gridDim.x * kIT) {
+    const int P = (int)blockIdx.x < nblocks ? lv.W * lv.H : 0;
+    for (int q = blockIdx.x * kIT + threadIdx.x; q < P; q += nblocks * kIT) {
         const int v = q / lv.W, u = q - v * lv.W;
         const float d = lv.depth[q];
         if (d > 0.0f) {
             const float pe[4] = { ((float)u + 0.5f - lv.cx) / lv.fx * d, (lv.cy - (float)v - 0.5f) / lv.fy * d, -d,
                                   1.0f };
             float p[4];
-            mat_vec(st->pose_f, pe, p);
+            mat_vec(pose, pe, p);
             float e[4];
             mat_vec(mpose->w2e, p, e);
             const float z = -e[2];
@@ -452,6 +461,22 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, 
             }
         }
     }
+}
+
+template <int MODE, bool COMP>
+__global__ __launch_bounds__(kIT) void k_icp_assoc(IcpLevel lv, ModelParams mp, const ModelPose *__restrict__ mpose,
+                                                   const float4 *__restrict__ mpts,
+                                                   const float4 *__restrict__ mnrm,
+                                                   IcpState *__restrict__ st, float dist_thresh,
+                                                   CSum *__restrict__ partials, int level, IcpSolveParams sp)
+{
+    constexpr int NV = IcpNV<MODE>::nv;
+    constexpr int MODE_OF_KERNEL = MODE;
+    __shared__ CsumLds<NV> lds;
+    if (st->level_done[level])
+        return;
+    CSum acc[NV];
+    icp_accumulate<MODE, COMP>(lv, mp, mpose, mpts, mnrm, st->pose_f, dist_thresh, (int)gridDim.x, acc);
     icp_publish<MODE_OF_KERNEL, COMP>(acc, lds, partials, st, level, sp, mpose);
 }
 
@@ -723,7 +748,7 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
     // count, sum r^2 and (point-to-plane) sum r^2 / d^4: the residual scaled to
     // 1 m depth by the sensor noise model sigma ~ z^2
     constexpr int IC = MODE == KIFI_ICP_POINT_TO_PLANE ? 27 : 15;
-    const double cnt = v[IC], e2 = v[IC + 1], ew2 = MODE == KIFI_ICP_POINT_TO_PLANE ? v[IC + 2] : v[IC + 1];
+    const double cnt = v[IC], e2 = v[IC + 1], ew2 = v[MODE == KIFI_ICP_POINT_TO_PLANE ? IC + 2 : IC + 1];
     st->rms = cnt > 0.0 ? (float)sqrt(e2 / cnt) : 0.0f;
     st->rms_1m = cnt > 0.0 ? (float)sqrt(ew2 / cnt) : 0.0f;
     st->inliers = (int)cnt;
@@ -820,39 +845,34 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
 // Fallback gate (no host round trip): when the first pass started from an
 // external guess and its 1 m-scaled residual or its iteration count exceeds the
 // limits, the second pass re-runs plain ICP from the previous frame's pose; the
-// first pass's result is kept aside for k_icp_fallback_pick.
-__global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, int max_iters)
+// first pass's result is kept aside for fallback_pick.
+__device__ __forceinline__ bool fallback_needed(const IcpState *st, float max_rms, int max_iters)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    const bool bad = st->failed || (max_rms > 0.0f && st->rms_1m > max_rms) ||
-                     (max_iters > 0 && st->total_iters >= max_iters);
-    for (int l = 0; l < 4; ++l)
-        st->level_done[l] = bad ? 0 : 1;     // the second pass runs only after a bad first one
-    if (bad) {
-        st->fallback = 1;
-        st->active = 1;
-        st->alt_rms = st->failed ? FLT_MAX : st->rms_1m;
-        st->alt_inliers = st->inliers;
-        st->failed = 0;
-        for (int i = 0; i < 16; ++i) {
-            st->alt_pose[i] = st->pose[i];
-            st->pose[i] = st->prev[i];
-            st->pose_f.m[i] = (float)st->prev[i];
-        }
-    } else {
-        st->active = 0;
+    return st->failed || (max_rms > 0.0f && st->rms_1m > max_rms) || (max_iters > 0 && st->total_iters >= max_iters);
+}
+
+__device__ void fallback_setup(IcpState *st)
+{
+    st->fallback = 1;
+    st->active = 1;
+    st->alt_rms = st->failed ? FLT_MAX : st->rms_1m;
+    st->alt_inliers = st->inliers;
+    st->failed = 0;
+    for (int i = 0; i < 16; ++i) {
+        st->alt_pose[i] = st->pose[i];
+        st->pose[i] = st->prev[i];
+        st->pose_f.m[i] = (float)st->prev[i];
     }
+    for (int l = 0; l < 4; ++l)
+        st->level_done[l] = 0;
 }
 
 // ICP judges the two starts: after a fallback pass, the first result stands
 // unless it failed or broke the residual limit and the fallback's residual is
 // lower (a converged pass with an acceptable residual is never replaced: a
 // far-off start can reach a wrong minimum with a smaller residual).
-__global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
+__device__ void fallback_pick(IcpState *st, float max_rms)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || !st->fallback)
-        return;
     const float second = st->failed ? FLT_MAX : st->rms_1m;
     const bool first_bad = st->alt_rms == FLT_MAX || (max_rms > 0.0f && st->alt_rms > max_rms);
     if (!first_bad || st->alt_rms <= second) {
@@ -863,6 +883,144 @@ __global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
         st->rms_1m = st->alt_rms;
         st->inliers = st->alt_inliers;
         st->failed = 0;
+    }
+}
+
+// Gate of the second pass when it runs as one launch per iteration: a good
+// first pass marks every level done, which turns those launches into early exits.
+__global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, int max_iters)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    if (fallback_needed(st, max_rms, max_iters)) {
+        fallback_setup(st);
+    } else {
+        for (int l = 0; l < 4; ++l)
+            st->level_done[l] = 1;
+        st->active = 0;
+    }
+}
+
+__global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !st->fallback)
+        return;
+    fallback_pick(st, max_rms);
+}
+
+// ------------------------------------------------ fallback pass in one launch
+// The whole fallback pass (levels levels-1 down to 0) in one launch, so that a
+// frame whose first pass is good pays one early-exit launch for it.  Every
+// workgroup evaluates the gate on the first pass's state; when it holds, the
+// last workgroup to arrive (so no workgroup can still be reading the gate's
+// inputs) sets the state up.  Every workgroup is resident (the host launches
+// at most one per CU after checking that a CU holds one), so a workgroup can
+// wait for the others: each iteration ends in icp_publish's ticket reduction,
+// whose solving workgroup releases the new state and then publishes st->gen;
+// the others poll gen with relaxed loads and read the pose and level flag with
+// coherent loads once it moves, so a converged level ends the loop for every
+// workgroup at the same point.  Each level runs on the workgroup count that
+// its launch-per-iteration kernel has (the first nb[l] of the grid; the others
+// only wait), so the sums are bit-identical to a pass launched iteration by
+// iteration.  Workgroup 0 judges the two passes at the end.  Waits are bounded:
+// a workgroup that polls spin_limit times counts a timeout and leaves; the
+// others then miss its ticket and time out in turn, so the launch always ends
+// (and the host reports the timeout).  In-kernel iterations cost more than
+// launched ones here (the hand-off is slower than a kernel boundary), which is
+// why only the rarely needed second pass runs this way.
+struct IcpFallback {
+    IcpLevel lv[3];
+    float    gate[3];
+    int      nb[3];             // workgroups of a level's association (as one launch per iteration has)
+    int      iters[3];
+    int      levels;
+    float    max_rms;
+    int      max_iters;
+    unsigned spin_limit;
+};
+
+// Waits (thread 0 polling, the workgroup at the barrier) until st->gen reaches
+// target; false after spin_limit polls.  No acquire fence follows: the state a
+// waiting workgroup needs next (pose and level flag) it reads with coherent
+// atomic loads after the wait, which the solving workgroup's release made
+// visible before it moved gen.  An agent-scope acquire here would invalidate the
+// L2 of the XCD once per wave, evicting the model maps that the other
+// workgroups are gathering from.
+__device__ bool icp_wait_gen(IcpState *st, unsigned target, unsigned spin_limit)
+{
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        int good = 1;
+        unsigned n = 0;
+        while ((int)(__hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+            if (++n > spin_limit) {
+                good = 0;
+                __hip_atomic_fetch_add(&st->spin_timeouts, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kIT) void k_icp_fallback_pass(IcpFallback a, ModelParams mp,
+                                                           const ModelPose *__restrict__ mpose,
+                                                           const float4 *__restrict__ mpts,
+                                                           const float4 *__restrict__ mnrm, IcpState *st,
+                                                           CSum *__restrict__ partials, IcpSolveParams sp)
+{
+    constexpr int MODE = KIFI_ICP_POINT_TO_PLANE;
+    constexpr int NV = IcpNV<MODE>::nv;
+    __shared__ CsumLds<NV> lds;
+    __shared__ float s_pose[16];
+    __shared__ int s_done;
+    if (!fallback_needed(st, a.max_rms, a.max_iters))
+        return;
+    unsigned gen = __hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == gridDim.x - 1) {
+            fallback_setup(st);
+            st->ticket = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&st->gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (!icp_wait_gen(st, ++gen, a.spin_limit))
+        return;
+    for (int l = a.levels - 1; l >= 0; --l) {
+        for (int it = 0; it < a.iters[l]; ++it) {
+            if (threadIdx.x < 16)
+                s_pose[threadIdx.x] = __hip_atomic_load(&st->pose_f.m[threadIdx.x], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            else if (threadIdx.x == 16)
+                s_done = __hip_atomic_load(&st->level_done[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const bool done = s_done != 0;
+            Mat4 pose;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                pose.m[i] = s_pose[i];
+            __syncthreads();     // read by every wave before the next level's loads rewrite them
+            if (done)
+                break;
+            ++gen;
+            if ((int)blockIdx.x < a.nb[l]) {   // the rest of the grid has no pixels at this level: it only waits
+                CSum acc[NV];
+                icp_accumulate<MODE, COMP>(a.lv[l], mp, mpose, mpts, mnrm, pose, a.gate[l], a.nb[l], acc);
+                icp_publish<MODE, COMP, NV, true>(acc, lds, partials, st, l, sp, mpose, gen, a.nb[l]);
+            }
+            if (!icp_wait_gen(st, gen, a.spin_limit))
+                return;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the last solve's state
+        fallback_pick(st, a.max_rms);
     }
 }
 
@@ -1116,6 +1274,8 @@ void icp_free(IcpWork *w)
 }
 
 static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels);
+static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip,
+                               int levels);
 
 // Runs the whole pyramid schedule asynchronously on the context's stream.
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
@@ -1143,7 +1303,7 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrins
         hipLaunchKernelGGL(k_depth_down, dim3((W / 2 * (H / 2) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
                            w->lv[1]);
     icp_pass(m, w, cam, ip, levels);
-    if (ip->fallback_rms > 0.0f || ip->fallback_iters > 0) {
+    if ((ip->fallback_rms > 0.0f || ip->fallback_iters > 0) && !icp_fallback_multi(m, w, cam, ip, levels)) {
         hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
                            ip->fallback_iters);
         icp_pass(m, w, cam, ip, levels);
@@ -1170,36 +1330,54 @@ void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t s
     }
 }
 
-static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels)
+// Pyramid level l of the live frame (the reference's camera scaled by 2^-l) and
+// its association gate.
+static IcpLevel icp_level(const IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int l, float *gate)
 {
-    hipStream_t st = m->ctx->stream;
-    const int W = cam->width, H = cam->height;
-    int Wl[3] = { W, W / 2, W / 4 }, Hl[3] = { H, H / 2, H / 4 };
+    IcpLevel lv;
+    const float sc = 1.0f / (float)(1 << l);
+    lv.W = cam->width >> l;
+    lv.H = cam->height >> l;
+    lv.fx = cam->fx * sc;
+    lv.fy = cam->fy * sc;
+    lv.cx = cam->cx * sc;
+    lv.cy = cam->cy * sc;
+    lv.depth = w->lv[l];
+    float g = ip->dist_thresh;
+    for (int k = 0; k < l; ++k)
+        g *= ip->level_gate_growth > 0.0f ? ip->level_gate_growth : 1.0f;
+    *gate = g;
+    return lv;
+}
+
+static IcpSolveParams solve_params(const kifi_icp_params *ip)
+{
     IcpSolveParams sp;
     sp.eps_rot = ip->eps_rot;
     sp.eps_trans = ip->eps_trans;
     sp.min_inliers = ip->min_inliers;
+    return sp;
+}
+
+// grid-stride: at most one workgroup per CU, so the last workgroup's reduction
+// covers few partials
+static int icp_blocks(const kifi_ctx *ctx, const IcpLevel &lv)
+{
+    return std::min((lv.W * lv.H + kIT - 1) / kIT, std::max(1, ctx->num_cus));
+}
+
+static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels)
+{
+    hipStream_t st = m->ctx->stream;
+    const IcpSolveParams sp = solve_params(ip);
     // Horn mode is the reference's ICP::Align: full resolution only
     for (int l = ip->mode == KIFI_ICP_HORN ? 0 : levels - 1; l >= 0; --l) {
-        IcpLevel lv;
-        const float sc = 1.0f / (float)(1 << l);
-        lv.W = Wl[l];
-        lv.H = Hl[l];
-        lv.fx = cam->fx * sc;
-        lv.fy = cam->fy * sc;
-        lv.cx = cam->cx * sc;
-        lv.cy = cam->cy * sc;
-        lv.depth = w->lv[l];
-        // grid-stride: at most one workgroup per CU, so the last workgroup's
-        // reduction covers few partials
-        const int cap = std::max(1, m->ctx->num_cus);
-        const int nb = std::min((lv.W * lv.H + kIT - 1) / kIT, cap);
-        float gate = ip->dist_thresh;
-        for (int k = 0; k < l; ++k)
-            gate *= ip->level_gate_growth > 0.0f ? ip->level_gate_growth : 1.0f;
+        float gate;
+        const IcpLevel lv = icp_level(w, cam, ip, l, &gate);
+        const int nb = icp_blocks(m->ctx, lv);
         for (int it = 0; it < ip->iters[l]; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
-                const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, cap);
+                const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, std::max(1, m->ctx->num_cus));
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
                                    w->d_state, (CSum *)w->partials, l, sp);
             } else {
@@ -1214,6 +1392,50 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
             }
         }
     }
+}
+
+// The point-to-plane fallback pass as one launch of k_icp_fallback_pass: a
+// frame whose first pass is good pays one early-exit launch instead of one per
+// iteration slot.  Its workgroups wait for each other, so all of them must be
+// resident at once: the grid is at most one workgroup per CU, and the occupancy
+// query (once per context) confirms that a CU holds one.  An ordinary launch
+// then suffices; hipLaunchCooperativeKernel gives the same guarantee at the
+// cost of a device-wide synchronisation per launch.  False without that
+// confirmation (the caller then enqueues the pass launch by launch).
+static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip,
+                               int levels)
+{
+    kifi_ctx *ctx = m->ctx;
+    if (ip->mode != KIFI_ICP_POINT_TO_PLANE)
+        return false;
+    if (ctx->multi_ok < 0) {
+        int a = 0, b = 0;
+        const hipError_t e0 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_icp_fallback_pass<true>, kIT, 0);
+        const hipError_t e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_icp_fallback_pass<false>, kIT, 0);
+        (void)hipGetLastError();
+        ctx->multi_ok = e0 == hipSuccess && e1 == hipSuccess && a >= 1 && b >= 1 ? 1 : 0;
+    }
+    if (!ctx->multi_ok)
+        return false;
+    IcpFallback a{};
+    for (int l = 0; l < 3; ++l) {
+        a.lv[l] = icp_level(w, cam, ip, l, &a.gate[l]);
+        a.nb[l] = icp_blocks(ctx, a.lv[l]);
+        a.iters[l] = l < levels ? ip->iters[l] : 0;
+    }
+    a.levels = levels;
+    a.max_rms = ip->fallback_rms;
+    a.max_iters = ip->fallback_iters;
+    a.spin_limit = 1u << 18;    // ~0.1 s of polling; an iteration takes tens of microseconds
+    const dim3 grid(icp_blocks(ctx, a.lv[0]));   // <= num_cus
+    const IcpSolveParams sp = solve_params(ip);
+    if (ip->naive_sums)
+        hipLaunchKernelGGL(k_icp_fallback_pass<false>, grid, dim3(kIT), 0, ctx->stream, a, m->mp, m->d_pose,
+                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials, sp);
+    else
+        hipLaunchKernelGGL(k_icp_fallback_pass<true>, grid, dim3(kIT), 0, ctx->stream, a, m->mp, m->d_pose,
+                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials, sp);
+    return true;
 }
 
 __global__ void k_store_mat(Mat4 v, Mat4 *__restrict__ out)

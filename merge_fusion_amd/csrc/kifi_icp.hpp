@@ -56,6 +56,8 @@ struct IcpState {
     float  alt_rms;
     int    alt_inliers;
     unsigned ticket;        // workgroups done in the current association launch
+    unsigned gen;           // iterations solved by multi-iteration launches (monotone)
+    unsigned spin_timeouts; // waits of a multi-iteration launch that gave up (0 unless broken)
     double sums[30];        // reduced sums of the last association (icp_finish layout)
     // running totals over frames
     long long sum_iters;

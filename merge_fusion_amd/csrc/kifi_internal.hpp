@@ -205,6 +205,7 @@ struct kifi_ctx {
     bool         stage_busy[2] = { false, false };
     int          stage_slot = 0;
     int          num_cus = 0;
+    int          multi_ok = -1;    // multi-iteration ICP launch fits one workgroup per CU (-1: not yet asked)
     // stage timers (HIP event pairs recorded on `stream` while enabled)
     unsigned     timing_mask = 0;   // bit s: time stage s
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timers[KIFI_STAGE_COUNT];

@@ -310,6 +310,110 @@ __global__ __launch_bounds__(256) void k_fc_small(const uint16_t *__restrict__ X
     }
 }
 
+// Direct 3x3 / stride-2 / pad-1 convolution for a few inputs (the per-frame
+// prediction), bias + ReLU -> bf16 as the GEMM path stores it.  Workgroup =
+// (band of R output rows, group of CG output channels, input b): the band's
+// input rows (every channel; zero rows outside the image and a zero column
+// left of it) and the group's weights are staged in LDS by 16- and 4-byte loads
+// issued eight at a time, then thread (pixel p, sub s) accumulates in binary32
+// the outputs of channels s, s + nsub, ... of the group.  The implicit-GEMM
+// tiles would run M = Ho Wo (16 .. 3392 here) through a few workgroups walking
+// the whole of K.  Needs Wi % 8 == 0 (so Wo = Wi / 2 and column 2 Wo - 1 is in
+// the image) and 16-byte aligned input rows; the host checks.
+struct ConvSmall {
+    ConvGeom g;
+    const uint16_t *X, *W;
+    const float *bias;
+    uint16_t *Y;
+    int R, CG, nsub;
+};
+
+constexpr int kXPad = 8;   // LDS row: 8 zero elements (the last is column -1), then Wi
+
+template <int J>
+__global__ __launch_bounds__(256) void k_conv_small(ConvSmall c)
+{
+    extern __shared__ __align__(16) uint16_t smem[];
+    const ConvGeom &g = c.g;
+    const int r0 = blockIdx.x * c.R, co0 = blockIdx.y * c.CG, b = blockIdx.z;
+    const int rows_out = min(c.R, g.Ho - r0), rows_in = 2 * rows_out + 1, hi0 = 2 * r0 - 1;
+    const int Wp = g.Wi + kXPad, K = g.Ci * 9, Kp = K + 2;
+    const int ncg = min(c.CG, g.Co - co0);
+    uint16_t *sx = smem;                                  // [Ci][rows_in][Wp]
+    uint16_t *sw = smem + (size_t)g.Ci * rows_in * Wp;    // [CG][Kp]
+    const int cpr = Wp / 8, nch = g.Ci * rows_in * cpr;   // 16-byte chunks; chunk 0 of a row is the pad
+    for (int base = threadIdx.x; base < nch; base += 8 * 256) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = min(base + u * 256, nch - 1);
+            const int row = i / cpr, ch = i - row * cpr, ci = row / rows_in, hi = hi0 + (row - ci * rows_in);
+            const int hic = min(max(hi, 0), g.Hi - 1), chc = max(ch - 1, 0);
+            const uint4 t = *reinterpret_cast<const uint4 *>(c.X + ((size_t)(b * g.Ci + ci) * g.Hi + hic) * g.Wi +
+                                                             chc * 8);
+            v[u] = ch > 0 && hi >= 0 && hi < g.Hi ? t : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (base + u * 256 < nch)
+                reinterpret_cast<uint4 *>(sx)[base + u * 256] = v[u];
+    }
+    const uint32_t *wsrc = reinterpret_cast<const uint32_t *>(c.W + (size_t)co0 * K);
+    const int K2 = K / 2, nw = ncg * K2;
+    for (int base = threadIdx.x; base < nw; base += 8 * 256) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v[u] = wsrc[min(base + u * 256, nw - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = base + u * 256;
+            if (i < nw) {
+                const int co = i / K2;
+                reinterpret_cast<uint32_t *>(sw + co * Kp)[i - co * K2] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int pp = c.R * g.Wo, p = threadIdx.x % pp, sub = threadIdx.x / pp;
+    if (sub >= c.nsub || p >= rows_out * g.Wo)
+        return;
+    const int hl = p / g.Wo, wo = p - hl * g.Wo;
+    const uint16_t *xr0 = sx + 2 * hl * Wp + kXPad - 1 + 2 * wo;   // channel 0, tap (0, 0): column 2 wo - 1
+    const uint16_t *wr0 = sw + sub * Kp;
+    float acc[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        acc[j] = 0.0f;
+#pragma unroll 2
+    for (int ci = 0; ci < g.Ci; ++ci) {
+        const uint16_t *xr = xr0 + ci * rows_in * Wp;
+        float xv[9];
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx)
+                xv[ky * 3 + kx] = bf2f(xr[ky * Wp + kx]);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const uint16_t *wr = wr0 + j * c.nsub * Kp + ci * 9;
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+                acc[j] += xv[t] * bf2f(wr[t]);
+        }
+    }
+    const int ho = r0 + hl;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int cl = sub + j * c.nsub;
+        if (cl < ncg) {
+            const int co = co0 + cl;
+            const float v = acc[j] + c.bias[co];
+            c.Y[((size_t)(b * g.Co + co) * g.Ho + ho) * g.Wo + wo] = f2bf(v > 0.0f ? v : 0.0f);
+        }
+    }
+}
+
 // fixed-order sum of split partials into the flat gradient
 __global__ void k_sum_splits(const float *__restrict__ part, int splits, int n, float *__restrict__ out)
 {
@@ -751,7 +855,36 @@ ConvGeom conv_geom(int l, int B)
     return g;
 }
 
-constexpr int kFcSmallRows = 4;   // batch sizes whose FC layers run as one wave per output
+constexpr int kSmallBatch = 4;   // batch sizes that run the direct convolution and one wave per FC output
+
+// k_conv_small for one layer; false when its shape or alignment does not fit
+// (the caller then runs the GEMM path).
+bool launch_conv_small(const OpConvFwd &op, hipStream_t st)
+{
+    const ConvGeom &g = op.g;
+    const int K = g.Ci * 9, Wp = g.Wi + kXPad;
+    if (g.Wi % 8 != 0 || K % 2 != 0 || (reinterpret_cast<uintptr_t>(op.X) & 15u) ||
+        (reinterpret_cast<uintptr_t>(op.W) & 3u))
+        return false;
+    // a band of output rows fills the 256 threads at most, its input rows stay within 48 KiB
+    int R = std::min(g.Ho, std::max(1, 256 / g.Wo));
+    while (R > 1 && (size_t)g.Ci * (2 * R + 1) * Wp * 2 > 48 * 1024)
+        --R;
+    const int nsub = std::max(1, 256 / (R * g.Wo));
+    const int J = K <= 36 ? 8 : K <= 160 ? 4 : K <= 320 ? 2 : 1;   // ~600 products per thread
+    ConvSmall c{ g, op.X, op.W, op.bias, op.Y, R, nsub * J, nsub };
+    const size_t lds = ((size_t)g.Ci * (2 * R + 1) * Wp + (size_t)c.CG * (K + 2)) * 2;
+    if (R * g.Wo > 256 || lds > 64 * 1024)
+        return false;
+    const dim3 grid((g.Ho + R - 1) / R, (g.Co + c.CG - 1) / c.CG, g.B);
+    switch (J) {
+    case 8: hipLaunchKernelGGL(k_conv_small<8>, grid, dim3(256), lds, st, c); break;
+    case 4: hipLaunchKernelGGL(k_conv_small<4>, grid, dim3(256), lds, st, c); break;
+    case 2: hipLaunchKernelGGL(k_conv_small<2>, grid, dim3(256), lds, st, c); break;
+    default: hipLaunchKernelGGL(k_conv_small<1>, grid, dim3(256), lds, st, c); break;
+    }
+    return true;
+}
 
 // forward of B inputs already in net->act[0]; outputs net->y (B x 6, normalised)
 void net_forward(kifi_twistnet *n, int B, hipStream_t st)
@@ -768,10 +901,11 @@ void net_forward(kifi_twistnet *n, int B, hipStream_t st)
         op.N = op.g.Co;
         op.K = op.g.Ci * 9;
         op.kchunk = (op.K + 15) / 16 * 16;
-        launch_gemm(op, 1, st);
+        if (B > kSmallBatch || !launch_conv_small(op, st))
+            launch_gemm(op, 1, st);
     }
     const int K5 = (int)act_elems(5);
-    if (B <= kFcSmallRows) {
+    if (B <= kSmallBatch) {
         // a few rows: one wave per output instead of MFMA tiles along all of K
         auto vec = [](const void *a, const void *b, int K) {
             return (int)(K % 8 == 0 && ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0);

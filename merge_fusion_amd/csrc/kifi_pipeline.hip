@@ -129,6 +129,15 @@ kifi_status kifi_model_download(kifi_model *m, float *points, float *normals)
     return KIFI_OK;
 }
 
+// A multi-iteration ICP launch whose workgroups stopped waiting for each other
+// left the tracking state incomplete.
+static kifi_status state_ok(kifi_ctx *ctx, const IcpState &s)
+{
+    return s.spin_timeouts ? set_error(ctx, KIFI_ERR_STATE, "ICP: a multi-iteration launch timed out waiting for "
+                                                           "its workgroups")
+                           : KIFI_OK;
+}
+
 static void fill_stats(const IcpState &s, kifi_icp_stats *st)
 {
     st->iterations = s.total_iters;
@@ -178,6 +187,8 @@ static kifi_status align_host(kifi_model *m, const uint16_t *depth_mm, const kif
             e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess)
             s = hip_error(ctx, e, "kifi_icp_align");
+        else
+            s = state_ok(ctx, hs);
     }
     (void)hipFree(d_depth);
     icp_free(&w);
@@ -533,7 +544,7 @@ kifi_status kifi_pipeline_pose(kifi_pipeline *p, double e2w[16])
     KIFI_HIP(p->ctx, hipStreamSynchronize(p->ctx->stream));
     for (int i = 0; i < 16; ++i)
         e2w[i] = hs.pose[i];
-    return KIFI_OK;
+    return state_ok(p->ctx, hs);
 }
 
 kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats)
@@ -544,7 +555,7 @@ kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats)
     KIFI_HIP(p->ctx, hipMemcpyAsync(&hs, p->work.d_state, sizeof hs, hipMemcpyDeviceToHost, p->ctx->stream));
     KIFI_HIP(p->ctx, hipStreamSynchronize(p->ctx->stream));
     fill_stats(hs, stats);
-    return KIFI_OK;
+    return state_ok(p->ctx, hs);
 }
 
 } // extern "C"

@@ -175,3 +175,66 @@ def test_icp_align_points_is_reference_align(ctx):
     assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da)
     # and it moves the guess towards the true pose
     assert mf.pose_error(gpose.astype(np.float64), pt)[0] < mf.pose_error(p2, pt)[0]
+
+
+def _rot_y(deg):
+    a = np.radians(deg)
+    R = np.eye(4)
+    R[0, 0], R[0, 2], R[2, 0], R[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+    return R
+
+
+def _track(ctx, frames, w2e0, prm, guesses=None):
+    """Poses after every frame and the final stats of a tracking pipeline;
+    guesses: {frame: function(previous pose) -> initial pose}."""
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02, params=prm)
+    poses = []
+    for f, d in enumerate(frames):
+        if f == 0:
+            pipe.integrate(d, w2e0)
+        else:
+            if guesses and f in guesses:
+                pipe.set_guess(guesses[f](poses[-1]))
+            pipe.integrate(d)
+        poses.append(pipe.pose())
+    st = pipe.stats()
+    pipe.close()
+    return poses, st
+
+
+def test_fallback_pass_reruns_plain_icp_from_previous_pose(ctx):
+    """The fallback pass (one multi-iteration launch) re-runs ICP from the
+    previous pose when the first pass fails the gate, and both passes stride the
+    same workgroups over the pixels as plain tracking does:
+    - a start 60 deg / 1 m off loses every correspondence, the pass fails, and
+      the fallback's result is kept: every pose equals plain tracking's bit for bit;
+    - a gate that always holds re-runs a pass that started at the previous pose
+      anyway: twice the iterations, and the first (identical) result stands;
+    - a start at the previous pose under the normal gate changes nothing."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    truth = [mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f) for f in range(6)]
+    frames = [mf.render(sc, CAM, p, noise_seed=500 + f) for f, p in enumerate(truth)]
+    w2e0 = mf.invert_rigid(truth[0]).astype(np.float32)
+    plain, st_plain = _track(ctx, frames, w2e0, mf.IcpParams.default())
+    assert st_plain["fallbacks"] == 0
+
+    def far(prev):
+        P = np.asarray(prev, np.float64).reshape(4, 4).T @ _rot_y(60.0)
+        P[:3, 3] += (1.0, 0.0, 0.5)
+        return P.T.reshape(16)
+
+    def same(prev):
+        return prev
+
+    for gate, guesses, fallbacks, extra_iters in ((0.005, {3: far}, 1, None),
+                                                  (1e-9, {2: same, 4: same}, 2, True),
+                                                  (0.005, {2: same, 4: same}, 0, False)):
+        prm = mf.IcpParams.default()
+        prm.fallback_rms = gate
+        prm.fallback_iters = 0
+        poses, st = _track(ctx, frames, w2e0, prm, guesses)
+        assert st["fallbacks"] == fallbacks, (gate, st)
+        if extra_iters is not None:
+            assert (st["sum_iters"] > st_plain["sum_iters"]) == extra_iters, (gate, st, st_plain)
+        for f in range(6):
+            assert np.array_equal(poses[f], plain[f]), (gate, f)

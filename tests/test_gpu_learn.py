@@ -133,14 +133,14 @@ def test_pipeline_with_learned_init_needs_fewer_iterations(ctx, trained):
 
 
 def test_single_sample_forward_matches_oracle(ctx):
-    """Batches of up to 4 (the pipeline's per-frame prediction) run their FC layers
-    one wave per output instead of MFMA tiles: the same oracle bound holds, and the
-    result agrees with the batched path."""
+    """Batches of up to 4 (the pipeline's per-frame prediction) run direct
+    convolutions from LDS and one wave per FC output instead of MFMA tiles: the
+    same oracle bound holds, and the result agrees with the batched path."""
     net = mf.TwistNet(ctx, 5)
     x, _ = _net_inputs(3, 8)
     p = tn.bf16_round(net.get_params()).astype(np.float64)
     big = net.forward(x)
-    for b in (1, 3):
+    for b in (1, 3, 4):
         y = net.forward(x[:b])
         yr = tn.forward(p, tn.bf16_round(x[:b]).astype(np.float64), store=tn.bf16_store)
         assert np.abs(y - yr).max() <= 1e-2 * np.abs(yr).max()
