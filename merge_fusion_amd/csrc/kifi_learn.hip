@@ -312,106 +312,109 @@ __global__ __launch_bounds__(256) void k_fc_small(const uint16_t *__restrict__ X
 
 // Direct 3x3 / stride-2 / pad-1 convolution for a few inputs (the per-frame
 // prediction), bias + ReLU -> bf16 as the GEMM path stores it.  Workgroup =
-// (band of R output rows, group of CG output channels, input b): the band's
-// input rows (every channel; zero rows outside the image and a zero column
-// left of it) and the group's weights are staged in LDS by 16- and 4-byte loads
-// issued eight at a time, then thread (pixel p, sub s) accumulates in binary32
-// the outputs of channels s, s + nsub, ... of the group.  The implicit-GEMM
-// tiles would run M = Ho Wo (16 .. 3392 here) through a few workgroups walking
-// the whole of K.  Needs Wi % 8 == 0 (so Wo = Wi / 2 and column 2 Wo - 1 is in
-// the image) and 16-byte aligned input rows; the host checks.
+// (band of R output rows, output channel co, input b): the band's input rows
+// (every channel; zero rows outside the image and a zero column left of it)
+// and the channel's weights are staged in LDS, every global load of a round
+// (NX 16-byte input chunks and NW weight loads per thread, clamped so that none
+// is conditional) issued before the first LDS store, so a layer waits for one
+// memory round trip.  Thread (pixel p, slice s) then sums the input channels
+// s, s + nks, ... of its pixel in binary32, and the nks slices of a pixel are
+// added in slice order through LDS.  One channel per workgroup and K split over
+// threads keep 64..224 workgroups and at most 18 x Ci / nks products per
+// thread; the implicit-GEMM tiles would run M = Ho Wo (16 .. 3392 here) through
+// a few workgroups walking the whole of K.  Needs Wi % 8 == 0 (so Wo = Wi / 2
+// and column 2 Wo - 1 is in the image) and 16-byte aligned input rows; WV
+// (16-byte weight chunks) also K % 8 == 0 and 16-byte aligned weights.  The
+// host checks.
 struct ConvSmall {
     ConvGeom g;
     const uint16_t *X, *W;
     const float *bias;
     uint16_t *Y;
-    int R, CG, nsub;
+    int R, nks;
 };
 
 constexpr int kXPad = 8;   // LDS row: 8 zero elements (the last is column -1), then Wi
 
-template <int J>
+__host__ __device__ constexpr int conv_small_kp(int K, bool wv) { return wv ? K + 8 : K + 2; }
+
+template <bool WV>
 __global__ __launch_bounds__(256) void k_conv_small(ConvSmall c)
 {
+    constexpr int NX = 12, NW = 1;
     extern __shared__ __align__(16) uint16_t smem[];
     const ConvGeom &g = c.g;
-    const int r0 = blockIdx.x * c.R, co0 = blockIdx.y * c.CG, b = blockIdx.z;
+    const int r0 = blockIdx.x * c.R, co = blockIdx.y, b = blockIdx.z;
     const int rows_out = min(c.R, g.Ho - r0), rows_in = 2 * rows_out + 1, hi0 = 2 * r0 - 1;
-    const int Wp = g.Wi + kXPad, K = g.Ci * 9, Kp = K + 2;
-    const int ncg = min(c.CG, g.Co - co0);
+    const int Wp = g.Wi + kXPad, K = g.Ci * 9;
     uint16_t *sx = smem;                                  // [Ci][rows_in][Wp]
-    uint16_t *sw = smem + (size_t)g.Ci * rows_in * Wp;    // [CG][Kp]
-    const int cpr = Wp / 8, nch = g.Ci * rows_in * cpr;   // 16-byte chunks; chunk 0 of a row is the pad
-    for (int base = threadIdx.x; base < nch; base += 8 * 256) {
-        uint4 v[8];
+    uint16_t *sw = smem + (size_t)g.Ci * rows_in * Wp;    // [Kp]
+    float *part = reinterpret_cast<float *>(sw + conv_small_kp(K, WV));   // [nks][R Wo]
+    const int cpr = Wp / 8, nch = g.Ci * rows_in * cpr;   // 16-byte input chunks; chunk 0 of a row is the pad
+    const int nwl = WV ? K / 8 : K;                       // weight loads: chunks or elements
+    const uint16_t *wsrc = c.W + (size_t)co * K;
+    for (int r = 0; r * NX * 256 < nch || r * NW * 256 < nwl; ++r) {
+        uint4 vx[NX];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = min(base + u * 256, nch - 1);
+        for (int u = 0; u < NX; ++u) {
+            const int i = min((r * NX + u) * 256 + (int)threadIdx.x, nch - 1);
             const int row = i / cpr, ch = i - row * cpr, ci = row / rows_in, hi = hi0 + (row - ci * rows_in);
             const int hic = min(max(hi, 0), g.Hi - 1), chc = max(ch - 1, 0);
             const uint4 t = *reinterpret_cast<const uint4 *>(c.X + ((size_t)(b * g.Ci + ci) * g.Hi + hic) * g.Wi +
                                                              chc * 8);
-            v[u] = ch > 0 && hi >= 0 && hi < g.Hi ? t : make_uint4(0u, 0u, 0u, 0u);
+            vx[u] = ch > 0 && hi >= 0 && hi < g.Hi ? t : make_uint4(0u, 0u, 0u, 0u);
         }
+        const int iw = min(r * NW * 256 + (int)threadIdx.x, nwl - 1);
+        uint4 vw = make_uint4(0u, 0u, 0u, 0u);
+        uint16_t ew = 0;
+        if constexpr (WV)
+            vw = reinterpret_cast<const uint4 *>(wsrc)[iw];
+        else
+            ew = wsrc[iw];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (base + u * 256 < nch)
-                reinterpret_cast<uint4 *>(sx)[base + u * 256] = v[u];
-    }
-    const uint32_t *wsrc = reinterpret_cast<const uint32_t *>(c.W + (size_t)co0 * K);
-    const int K2 = K / 2, nw = ncg * K2;
-    for (int base = threadIdx.x; base < nw; base += 8 * 256) {
-        uint32_t v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            v[u] = wsrc[min(base + u * 256, nw - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = base + u * 256;
-            if (i < nw) {
-                const int co = i / K2;
-                reinterpret_cast<uint32_t *>(sw + co * Kp)[i - co * K2] = v[u];
-            }
+        for (int u = 0; u < NX; ++u) {
+            const int i = (r * NX + u) * 256 + (int)threadIdx.x;
+            if (i < nch)
+                reinterpret_cast<uint4 *>(sx)[i] = vx[u];
+        }
+        if (r * NW * 256 + (int)threadIdx.x < nwl) {
+            if constexpr (WV)
+                reinterpret_cast<uint4 *>(sw)[iw] = vw;
+            else
+                sw[iw] = ew;
         }
     }
     __syncthreads();
-    const int pp = c.R * g.Wo, p = threadIdx.x % pp, sub = threadIdx.x / pp;
-    if (sub >= c.nsub || p >= rows_out * g.Wo)
-        return;
-    const int hl = p / g.Wo, wo = p - hl * g.Wo;
-    const uint16_t *xr0 = sx + 2 * hl * Wp + kXPad - 1 + 2 * wo;   // channel 0, tap (0, 0): column 2 wo - 1
-    const uint16_t *wr0 = sw + sub * Kp;
-    float acc[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-        acc[j] = 0.0f;
+    const int pp = c.R * g.Wo, p = threadIdx.x % pp, ks = threadIdx.x / pp;
+    const bool live = ks < c.nks && p < rows_out * g.Wo;
+    float acc = 0.0f;
+    if (live) {
+        const int hl = p / g.Wo, wo = p - hl * g.Wo;
+        const uint16_t *xr0 = sx + 2 * hl * Wp + kXPad - 1 + 2 * wo;   // channel 0, tap (0, 0): column 2 wo - 1
 #pragma unroll 2
-    for (int ci = 0; ci < g.Ci; ++ci) {
-        const uint16_t *xr = xr0 + ci * rows_in * Wp;
-        float xv[9];
+        for (int ci = ks; ci < g.Ci; ci += c.nks) {
+            const uint16_t *xr = xr0 + ci * rows_in * Wp, *wr = sw + ci * 9;
 #pragma unroll
-        for (int ky = 0; ky < 3; ++ky)
+            for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-            for (int kx = 0; kx < 3; ++kx)
-                xv[ky * 3 + kx] = bf2f(xr[ky * Wp + kx]);
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            const uint16_t *wr = wr0 + j * c.nsub * Kp + ci * 9;
-#pragma unroll
-            for (int t = 0; t < 9; ++t)
-                acc[j] += xv[t] * bf2f(wr[t]);
+                for (int kx = 0; kx < 3; ++kx)
+                    acc += bf2f(xr[ky * Wp + kx]) * bf2f(wr[ky * 3 + kx]);
         }
     }
-    const int ho = r0 + hl;
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int cl = sub + j * c.nsub;
-        if (cl < ncg) {
-            const int co = co0 + cl;
-            const float v = acc[j] + c.bias[co];
-            c.Y[((size_t)(b * g.Co + co) * g.Ho + ho) * g.Wo + wo] = f2bf(v > 0.0f ? v : 0.0f);
-        }
+    if (c.nks > 1) {
+        if (live)
+            part[ks * pp + p] = acc;
+        __syncthreads();
+        if (!live || ks != 0)
+            return;
+        for (int k = 1; k < c.nks; ++k)
+            acc += part[k * pp + p];
+    } else if (!live) {
+        return;
     }
+    const int hl = p / g.Wo, wo = p - hl * g.Wo;
+    const float v = acc + c.bias[co];
+    c.Y[((size_t)(b * g.Co + co) * g.Ho + r0 + hl) * g.Wo + wo] = f2bf(v > 0.0f ? v : 0.0f);
 }
 
 // fixed-order sum of split partials into the flat gradient
@@ -863,26 +866,24 @@ bool launch_conv_small(const OpConvFwd &op, hipStream_t st)
 {
     const ConvGeom &g = op.g;
     const int K = g.Ci * 9, Wp = g.Wi + kXPad;
-    if (g.Wi % 8 != 0 || K % 2 != 0 || (reinterpret_cast<uintptr_t>(op.X) & 15u) ||
-        (reinterpret_cast<uintptr_t>(op.W) & 3u))
+    if (g.Wi % 8 != 0 || K % 2 != 0 || (reinterpret_cast<uintptr_t>(op.X) & 15u))
         return false;
+    const bool wv = K % 8 == 0 && (reinterpret_cast<uintptr_t>(op.W) & 15u) == 0;
     // a band of output rows fills the 256 threads at most, its input rows stay within 48 KiB
     int R = std::min(g.Ho, std::max(1, 256 / g.Wo));
     while (R > 1 && (size_t)g.Ci * (2 * R + 1) * Wp * 2 > 48 * 1024)
         --R;
-    const int nsub = std::max(1, 256 / (R * g.Wo));
-    const int J = K <= 36 ? 8 : K <= 160 ? 4 : K <= 320 ? 2 : 1;   // ~600 products per thread
-    ConvSmall c{ g, op.X, op.W, op.bias, op.Y, R, nsub * J, nsub };
-    const size_t lds = ((size_t)g.Ci * (2 * R + 1) * Wp + (size_t)c.CG * (K + 2)) * 2;
-    if (R * g.Wo > 256 || lds > 64 * 1024)
+    const int nks = std::max(1, std::min(g.Ci, 256 / (R * g.Wo)));
+    const ConvSmall c{ g, op.X, op.W, op.bias, op.Y, R, nks };
+    const size_t lds = ((size_t)g.Ci * (2 * R + 1) * Wp + conv_small_kp(K, wv)) * 2 + (size_t)256 * 4;
+    const int nch = g.Ci * (2 * R + 1) * (Wp / 8), nwl = wv ? K / 8 : K;
+    if (R * g.Wo > 256 || lds > 64 * 1024 || nch > 12 * 256 * 4 || nwl > 256 * 4)
         return false;
-    const dim3 grid((g.Ho + R - 1) / R, (g.Co + c.CG - 1) / c.CG, g.B);
-    switch (J) {
-    case 8: hipLaunchKernelGGL(k_conv_small<8>, grid, dim3(256), lds, st, c); break;
-    case 4: hipLaunchKernelGGL(k_conv_small<4>, grid, dim3(256), lds, st, c); break;
-    case 2: hipLaunchKernelGGL(k_conv_small<2>, grid, dim3(256), lds, st, c); break;
-    default: hipLaunchKernelGGL(k_conv_small<1>, grid, dim3(256), lds, st, c); break;
-    }
+    const dim3 grid((g.Ho + R - 1) / R, g.Co, g.B);
+    if (wv)
+        hipLaunchKernelGGL(k_conv_small<true>, grid, dim3(256), lds, st, c);
+    else
+        hipLaunchKernelGGL(k_conv_small<false>, grid, dim3(256), lds, st, c);
     return true;
 }
 
