@@ -158,6 +158,10 @@ typedef struct {
                              /* large initial errors)                                  */
     int   naive_sums;      /* 1: plain binary32 sums without compensation (precision   */
                            /*    study only; default 0)                                */
+    float eps_level_growth; /* level l converges at eps_rot, eps_trans x growth^l;     */
+                            /* default 4: a coarser level has half the resolution and */
+                            /* a quarter of the samples, and stops at that precision  */
+                            /* (1: the same test on every level)                      */
 } kifi_icp_params;
 
 typedef struct {

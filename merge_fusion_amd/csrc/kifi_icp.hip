@@ -930,6 +930,7 @@ __global__ void k_icp_fallback_pick(IcpState *__restrict__ st, float max_rms)
 // why only the rarely needed second pass runs this way.
 struct IcpFallback {
     IcpLevel lv[3];
+    IcpSolveParams sp[3];
     float    gate[3];
     int      nb[3];             // workgroups of a level's association (as one launch per iteration has)
     int      iters[3];
@@ -971,7 +972,7 @@ __global__ __launch_bounds__(kIT) void k_icp_fallback_pass(IcpFallback a, ModelP
                                                            const ModelPose *__restrict__ mpose,
                                                            const float4 *__restrict__ mpts,
                                                            const float4 *__restrict__ mnrm, IcpState *st,
-                                                           CSum *__restrict__ partials, IcpSolveParams sp)
+                                                           CSum *__restrict__ partials)
 {
     constexpr int MODE = KIFI_ICP_POINT_TO_PLANE;
     constexpr int NV = IcpNV<MODE>::nv;
@@ -1012,7 +1013,7 @@ __global__ __launch_bounds__(kIT) void k_icp_fallback_pass(IcpFallback a, ModelP
             if ((int)blockIdx.x < a.nb[l]) {   // the rest of the grid has no pixels at this level: it only waits
                 CSum acc[NV];
                 icp_accumulate<MODE, COMP>(a.lv[l], mp, mpose, mpts, mnrm, pose, a.gate[l], a.nb[l], acc);
-                icp_publish<MODE, COMP, NV, true>(acc, lds, partials, st, l, sp, mpose, gen, a.nb[l]);
+                icp_publish<MODE, COMP, NV, true>(acc, lds, partials, st, l, a.sp[l], mpose, gen, a.nb[l]);
             }
             if (!icp_wait_gen(st, gen, a.spin_limit))
                 return;
@@ -1350,11 +1351,15 @@ static IcpLevel icp_level(const IcpWork *w, const kifi_intrinsics *cam, const ki
     return lv;
 }
 
-static IcpSolveParams solve_params(const kifi_icp_params *ip)
+// Convergence test of level l: the increments below eps x eps_level_growth^l.
+static IcpSolveParams solve_params(const kifi_icp_params *ip, int l)
 {
+    double g = 1.0;
+    for (int k = 0; k < l; ++k)
+        g *= ip->eps_level_growth > 0.0f ? ip->eps_level_growth : 1.0f;
     IcpSolveParams sp;
-    sp.eps_rot = ip->eps_rot;
-    sp.eps_trans = ip->eps_trans;
+    sp.eps_rot = ip->eps_rot * g;
+    sp.eps_trans = ip->eps_trans * g;
     sp.min_inliers = ip->min_inliers;
     return sp;
 }
@@ -1369,9 +1374,9 @@ static int icp_blocks(const kifi_ctx *ctx, const IcpLevel &lv)
 static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels)
 {
     hipStream_t st = m->ctx->stream;
-    const IcpSolveParams sp = solve_params(ip);
     // Horn mode is the reference's ICP::Align: full resolution only
     for (int l = ip->mode == KIFI_ICP_HORN ? 0 : levels - 1; l >= 0; --l) {
+        const IcpSolveParams sp = solve_params(ip, l);
         float gate;
         const IcpLevel lv = icp_level(w, cam, ip, l, &gate);
         const int nb = icp_blocks(m->ctx, lv);
@@ -1420,6 +1425,7 @@ static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics 
     IcpFallback a{};
     for (int l = 0; l < 3; ++l) {
         a.lv[l] = icp_level(w, cam, ip, l, &a.gate[l]);
+        a.sp[l] = solve_params(ip, l);
         a.nb[l] = icp_blocks(ctx, a.lv[l]);
         a.iters[l] = l < levels ? ip->iters[l] : 0;
     }
@@ -1428,13 +1434,12 @@ static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics 
     a.max_iters = ip->fallback_iters;
     a.spin_limit = 1u << 18;    // ~0.1 s of polling; an iteration takes tens of microseconds
     const dim3 grid(icp_blocks(ctx, a.lv[0]));   // <= num_cus
-    const IcpSolveParams sp = solve_params(ip);
     if (ip->naive_sums)
         hipLaunchKernelGGL(k_icp_fallback_pass<false>, grid, dim3(kIT), 0, ctx->stream, a, m->mp, m->d_pose,
-                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials, sp);
+                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials);
     else
         hipLaunchKernelGGL(k_icp_fallback_pass<true>, grid, dim3(kIT), 0, ctx->stream, a, m->mp, m->d_pose,
-                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials, sp);
+                           m->d_pts, m->d_nrm, w->d_state, (CSum *)w->partials);
     return true;
 }
 

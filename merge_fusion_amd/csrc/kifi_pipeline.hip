@@ -65,6 +65,7 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->level_gate_growth = 1.0f;
     p->fallback_iters = 0;
     p->naive_sums = 0;
+    p->eps_level_growth = 4.0f;   // half the resolution and a quarter of the samples per coarser level
     return KIFI_OK;
 }
 

@@ -338,9 +338,10 @@ def depth_pyramid(depth_mm, levels=3):
 
 
 def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dist_thresh=0.1,
-            eps_rot=2e-4, eps_trans=2e-4, gate_growth=1.0):
+            eps_rot=2e-4, eps_trans=2e-4, gate_growth=1.0, eps_growth=4.0):
     """FP64 point-to-plane ICP over the pyramid; returns (pose, total iterations, last rms).
-    Level l gates correspondences at dist_thresh x gate_growth^l."""
+    Level l gates correspondences at dist_thresh x gate_growth^l and converges
+    at eps_rot, eps_trans x eps_growth^l."""
     L = lib()
     if not hasattr(L, "_icp_ready"):
         _icp_sigs(L)
@@ -357,8 +358,8 @@ def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dis
         s = 1.0 / (1 << l)
         n = L.orc_icp_level_f64(f32p(d), d.shape[1], d.shape[0], cam.fx * s, cam.fy * s, cam.cx * s, cam.cy * s,
                                 f32p(P), f32p(N), ctypes.byref(c), W2E.ctypes.data_as(F64P),
-                                pose.ctypes.data_as(F64P), iters[l], dist_thresh * gate_growth ** l, eps_rot, eps_trans,
-                                st.ctypes.data_as(F64P))
+                                pose.ctypes.data_as(F64P), iters[l], dist_thresh * gate_growth ** l,
+                                eps_rot * eps_growth ** l, eps_trans * eps_growth ** l, st.ctypes.data_as(F64P))
         total += n
     return pose, total, st[0]
 

@@ -44,8 +44,10 @@ def test_model_maps_bit_exact(ctx, storage):
     assert gp[..., 3].mean() > 0.9 and gn[..., 3].mean() > 0.8
 
 
-@pytest.mark.parametrize("target", [3, 6, 9])
-def test_icp_matches_fp64_oracle(ctx, target):
+@pytest.mark.parametrize("target,eps_growth", [(3, 4.0), (6, 4.0), (9, 4.0), (6, 1.0)])
+def test_icp_matches_fp64_oracle(ctx, target, eps_growth):
+    """Default schedule (coarse levels converge at 4x / 16x the thresholds) and
+    the same thresholds on every level."""
     sc, gv, ov = _room(ctx, [0, 1, 2])
     p2 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)
     w2e = mf.invert_rigid(p2).astype(np.float32)
@@ -54,8 +56,11 @@ def test_icp_matches_fp64_oracle(ctx, target):
     pts, nrm = m.download()
     pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, target)
     d = mf.render(sc, CAM, pt, noise_seed=99 + target)
-    gpose, st = m.icp_align(d, p2)
-    opose, oit, orms = orc.icp_f64(d, CAM, pts, nrm, w2e.astype(np.float64), p2)
+    prm = mf.IcpParams.default()
+    assert prm.eps_level_growth == 4.0
+    prm.eps_level_growth = eps_growth
+    gpose, st = m.icp_align(d, p2, prm)
+    opose, oit, orms = orc.icp_f64(d, CAM, pts, nrm, w2e.astype(np.float64), p2, eps_growth=eps_growth)
     dt, da = mf.pose_error(gpose, opose)
     assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da, st, oit)
     gt_t, gt_a = mf.pose_error(gpose, pt)
