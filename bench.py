@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-initialisation comparison")
     ap.add_argument("--fast-frames", type=int, default=1000, help="frames of the fast trajectory")
-    ap.add_argument("--train-pairs", type=int, default=60000)
+    ap.add_argument("--train-pairs", type=int, default=100000)   # the 100k pairs of the training workload
     ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--no-precision", action="store_true", help="skip the 2 mm / 64-bit-key variant at scale")
     ap.add_argument("--precision-voxels", type=int, default=50_000_000,
