@@ -428,18 +428,36 @@ __global__ void k_sum_splits(const float *__restrict__ part, int splits, int n, 
     }
 }
 
-// bias gradient: db[c] = sum over b, p of dY[b][c][p]  (one workgroup per channel)
-__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dY, int B, int C, int HW,
-                                                   float *__restrict__ db)
+// Bias gradient db[c] = sum over b, p of dY[b][c][p], part 1 of 2: workgroup
+// (c, s) sums the planes of batch slice b = s, s + S, ... (16-byte loads along
+// a plane when HW % 8 == 0) into part[s][c]; k_sum_splits then adds the S
+// slices of a channel in slice order, so the gradient is deterministic.  One
+// workgroup per channel ran 16 workgroups over 868k values each for conv1.
+__global__ __launch_bounds__(256) void k_bias_part(const uint16_t *__restrict__ dY, int B, int C, int HW, int S,
+                                                   float *__restrict__ part)
 {
     __shared__ float red[256];
-    const int c = blockIdx.x;
-    float s = 0.0f;
-    for (int i = threadIdx.x; i < B * HW; i += 256) {
-        const int b = i / HW, p = i - b * HW;
-        s += bf2f(dY[((size_t)b * C + c) * HW + p]);
+    const int c = blockIdx.x, s = blockIdx.y, nb = (B - s + S - 1) / S;
+    float acc = 0.0f;
+    if (HW % 8 == 0) {
+        const int per = HW / 8;
+        for (int i = threadIdx.x; i < nb * per; i += 256) {
+            const int j = i / per, q = i - j * per;
+            const uint4 v = *reinterpret_cast<const uint4 *>(dY + ((size_t)(s + j * S) * C + c) * HW + q * 8);
+            const uint32_t w[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc += bf2f((uint16_t)(w[k] & 0xffffu));
+                acc += bf2f((uint16_t)(w[k] >> 16));
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < nb * HW; i += 256) {
+            const int j = i / HW, p = i - j * HW;
+            acc += bf2f(dY[((size_t)(s + j * S) * C + c) * HW + p]);
+        }
     }
-    red[threadIdx.x] = s;
+    red[threadIdx.x] = acc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o)
@@ -447,7 +465,7 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        db[c] = red[0];
+        part[(size_t)s * C + c] = red[0];
 }
 
 // loss 0.5/B sum (y - t)^2 and dy = (y - t)/B (bf16); single workgroup
@@ -944,6 +962,17 @@ void net_forward(kifi_twistnet *n, int B, hipStream_t st)
 
 int splits_for(int K) { return std::max(1, std::min(256, K / 2048)); }
 
+void sum_splits(kifi_twistnet *n, int splits, int count, float *out, hipStream_t st);
+
+// db of a layer whose output gradient dY is B x C x HW: about 512 workgroups of
+// batch slices, then the fixed-order sum of the slices (uses n->part)
+void bias_grad(kifi_twistnet *n, const uint16_t *dY, int B, int C, int HW, float *db, hipStream_t st)
+{
+    const int S = std::max(1, std::min(B, 512 / C));
+    hipLaunchKernelGGL(k_bias_part, dim3(C, S), dim3(256), 0, st, dY, B, C, HW, S, n->part);
+    sum_splits(n, S, C, db, st);
+}
+
 void sum_splits(kifi_twistnet *n, int splits, int count, float *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sum_splits, dim3(std::min(1024, (count + 255) / 256)), dim3(256), 0, st, n->part, splits,
@@ -968,7 +997,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         op.ldy = kOut;
         launch_gemm(op, 1, st);
         sum_splits(n, 1, kOut * kFc1, n->g + L.w[6], st);
-        hipLaunchKernelGGL(k_bias_grad, dim3(kOut), dim3(256), 0, st, n->dy, B, kOut, 1, n->g + L.b[6]);
+        bias_grad(n, n->dy, B, kOut, 1, n->g + L.b[6], st);
         OpFcBwdData od;
         od.dY = n->dy;
         od.W = n->pb + L.w[6];
@@ -994,7 +1023,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         op.ldy = kFc1;
         launch_gemm(op, 1, st);
         sum_splits(n, 1, kFc1 * fin, n->g + L.w[5], st);
-        hipLaunchKernelGGL(k_bias_grad, dim3(kFc1), dim3(256), 0, st, n->dact[6], B, kFc1, 1, n->g + L.b[5]);
+        bias_grad(n, n->dact[6], B, kFc1, 1, n->g + L.b[5], st);
         OpFcBwdData od;
         od.dY = n->dact[6];
         od.W = n->pb + L.w[5];
@@ -1021,8 +1050,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         const int used = (ow.K + ow.kchunk - 1) / ow.kchunk;
         launch_gemm(ow, used, st);
         sum_splits(n, used, ow.M * ow.N, n->g + L.w[l], st);
-        hipLaunchKernelGGL(k_bias_grad, dim3(g.Co), dim3(256), 0, st, n->dact[l + 1], B, g.Co, g.Ho * g.Wo,
-                           n->g + L.b[l]);
+        bias_grad(n, n->dact[l + 1], B, g.Co, g.Ho * g.Wo, n->g + L.b[l], st);
         if (l == 0)
             break;
         OpConvBwdData od;
