@@ -162,6 +162,10 @@ typedef struct {
                             /* default 4: a coarser level has half the resolution and */
                             /* a quarter of the samples, and stops at that precision  */
                             /* (1: the same test on every level)                      */
+    int   spin_limit;      /* polls before a waiting workgroup of the one-launch      */
+                           /* fallback pass gives up (0: 2^18, about 0.1 s).  A test  */
+                           /* hook: a tiny limit makes the pass time out on purpose,  */
+                           /* and the next state read returns KIFI_ERR_STATE          */
 } kifi_icp_params;
 
 typedef struct {

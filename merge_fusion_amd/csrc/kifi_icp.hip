@@ -1432,7 +1432,8 @@ static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics 
     a.levels = levels;
     a.max_rms = ip->fallback_rms;
     a.max_iters = ip->fallback_iters;
-    a.spin_limit = 1u << 18;    // ~0.1 s of polling; an iteration takes tens of microseconds
+    // ~0.1 s of polling by default; an iteration takes tens of microseconds
+    a.spin_limit = ip->spin_limit > 0 ? (unsigned)ip->spin_limit : 1u << 18;
     const dim3 grid(icp_blocks(ctx, a.lv[0]));   // <= num_cus
     if (ip->naive_sums)
         hipLaunchKernelGGL(k_icp_fallback_pass<false>, grid, dim3(kIT), 0, ctx->stream, a, m->mp, m->d_pose,

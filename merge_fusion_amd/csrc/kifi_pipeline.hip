@@ -66,6 +66,7 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->fallback_iters = 0;
     p->naive_sums = 0;
     p->eps_level_growth = 4.0f;   // half the resolution and a quarter of the samples per coarser level
+    p->spin_limit = 0;
     return KIFI_OK;
 }
 

@@ -243,3 +243,31 @@ def test_fallback_pass_reruns_plain_icp_from_previous_pose(ctx):
             assert (st["sum_iters"] > st_plain["sum_iters"]) == extra_iters, (gate, st, st_plain)
         for f in range(6):
             assert np.array_equal(poses[f], plain[f]), (gate, f)
+
+
+def test_fallback_pass_wait_timeout_is_reported(ctx):
+    """Every wait of the one-launch fallback pass is bounded.  A spin limit of one
+    poll makes the waits give up on purpose: every workgroup leaves, the frames
+    complete, and the next state read reports KIFI_ERR_STATE instead of a pose.
+    A new pipeline on the same context then tracks normally."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    truth = [mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f) for f in range(4)]
+    frames = [mf.render(sc, CAM, p, noise_seed=700 + f) for f, p in enumerate(truth)]
+    w2e0 = mf.invert_rigid(truth[0]).astype(np.float32)
+    prm = mf.IcpParams.default()
+    prm.fallback_rms = 1e-9          # the gate always holds, so the pass runs every frame
+    prm.fallback_iters = 0
+    prm.spin_limit = 1
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02, params=prm)
+    pipe.integrate(frames[0], w2e0)
+    prev = truth[0].reshape(16)
+    for f in (1, 2, 3):
+        pipe.set_guess(prev)
+        pipe.integrate(frames[f])
+    with pytest.raises(mf.KifiError, match="status %d" % mf.KIFI_ERR_STATE):
+        pipe.stats()
+    pipe.close()
+    ctx.synchronize()
+    poses, st = _track(ctx, frames, w2e0, mf.IcpParams.default())
+    dt, da = mf.pose_error(poses[-1], truth[-1])
+    assert st["frames"] == 3 and dt < 0.02 and da < 0.5, (dt, da, st)
