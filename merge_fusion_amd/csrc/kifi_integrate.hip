@@ -56,15 +56,12 @@ struct KeygenParams {
 // Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
 // the eight keys k + {0,1}x + {0,1}y + {0,1}z (integer adds, Integrator.cpp:192-194,
 // 218, 242).  Output order is irrelevant: the sort that follows fixes it.
-// pixel_key: the key of one pixel (Integrator.cpp:109-178); false when it emits none.
-template <typename K, typename D>
-__device__ __forceinline__ bool pixel_key(const D *__restrict__ depth, const KeygenParams &p, int pix,
-                                          K &key, float *__restrict__ dm_out)
+// pixel_key: the key of pixel pix at depth d metres (Integrator.cpp:109-178);
+// false when it emits none.
+template <typename K>
+__device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, int pix, K &key)
 {
     const int v = pix / p.W, u = pix - v * p.W;
-    const float d = depth_m(depth, pix);                          // DepthStream.cpp:89-90
-    if (dm_out)
-        dm_out[pix] = d;                                          // the metre frame of the update
     if (!(d > 0.0f))
         return false;
     float q[4] = { (float)u + 0.0f, 0.0f + -(float)v, 0.0f + -1.0f, 0.0f + 0.0f };
@@ -101,11 +98,19 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
     const int P = p.W * p.H;
     K keys[kKgItems];
     uint32_t flags = 0, c = 0;
+    // every depth load in flight before the first use (a load under a branch
+    // would be waited for on its own)
+    float dm[kKgItems];
+#pragma unroll
+    for (int j = 0; j < kKgItems; ++j)
+        dm[j] = depth_m(depth, (size_t)min((blockIdx.x * kKgItems + j) * kThreads + threadIdx.x, P - 1));
 #pragma unroll
     for (int j = 0; j < kKgItems; ++j) {
         const int pix = (blockIdx.x * kKgItems + j) * kThreads + threadIdx.x;
+        if (dm_out && pix < P)
+            dm_out[pix] = dm[j];                                  // the metre frame of the update
         K k = 0;
-        const bool ok = pix < P && pixel_key<K, D>(depth, p, pix, k, dm_out);
+        const bool ok = pix < P && pixel_key<K>(dm[j], p, pix, k);   // DepthStream.cpp:89-90
         keys[j] = k;
         flags |= (uint32_t)ok << j;
         c += ok;
@@ -152,11 +157,15 @@ __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ k
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kRsTile;
     if (base < n) {
-#pragma unroll 4
+        K kk[kRsItems];    // all loads in flight together: clamped, not conditional
+#pragma unroll
+        for (int j = 0; j < kRsItems; ++j)
+            kk[j] = keys[min(base + (size_t)j * kThreads + threadIdx.x, (size_t)n - 1)];
+#pragma unroll
         for (int j = 0; j < kRsItems; ++j) {
             const size_t i = base + (size_t)j * kThreads + threadIdx.x;
             const bool ok = i < n;
-            const uint32_t d = ok ? (uint32_t)(keys[i] >> shift) & 0xffu : 0u;
+            const uint32_t d = ok ? (uint32_t)(kk[j] >> shift) & 0xffu : 0u;
             const uint64_t act = __ballot(ok);
             const uint64_t m = match_digit(d, 8, act);
             if (ok && (m & lanemask_lt()) == 0)
@@ -226,11 +235,13 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     uint32_t rank[kRsItems];
     const size_t wbase = tbase + (size_t)w * (kWave * kRsItems);
 #pragma unroll
+    for (int j = 0; j < kRsItems; ++j)   // all loads in flight together: clamped, not conditional
+        keys[j] = in[min(wbase + (size_t)j * kWave + lane, (size_t)n - 1)];
+#pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
         const size_t i = wbase + (size_t)j * kWave + lane;
         const bool ok = i < n;
-        const K k = ok ? in[i] : K(0);
-        keys[j] = k;
+        const K k = keys[j];
         const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
         const uint64_t act = __ballot(ok);
         const uint64_t m = match_digit(d, 8, act);
@@ -290,15 +301,18 @@ __global__ __launch_bounds__(kThreads) void k_unique_count(const K *__restrict__
     const size_t base = (size_t)blockIdx.x * kRsTile;
     uint32_t c = 0;
     if (base < n) {
-        const size_t mine = base + (size_t)threadIdx.x * kRsItems;
-        K prev = mine > 0 && mine - 1 < n ? keys[mine - 1] : K(0);
+        const size_t mine = base + (size_t)threadIdx.x * kRsItems, last = (size_t)n - 1;
+        // all loads in flight together: clamped, not conditional
+        K prev = keys[mine > 0 ? min(mine - 1, last) : 0];
+        K kk[kRsItems];
+#pragma unroll
+        for (int q = 0; q < kRsItems; ++q)
+            kk[q] = keys[min(mine + q, last)];
+#pragma unroll
         for (int q = 0; q < kRsItems; ++q) {
             const size_t i = mine + q;
-            if (i >= n)
-                break;
-            const K k = keys[i];
-            c += (i == 0 || k != prev);
-            prev = k;
+            c += i < n && (i == 0 || kk[q] != prev);
+            prev = kk[q];
         }
     }
     c = block_reduce_sum<kThreads>(c, tmp);
