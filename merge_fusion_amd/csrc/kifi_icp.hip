@@ -42,19 +42,36 @@ __device__ __forceinline__ uint32_t orderable_f(float z)
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// Consumes the words of a loaded value in an empty asm statement: the load is
+// issued before this point, so loads placed ahead of it wait together instead
+// of being sunk, one by one, below the tests that use them.
+template <typename T>
+__device__ __forceinline__ void loaded_here(const T &v)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k)
+        asm volatile("" ::"v"(w[k]));
+}
+
 template <typename K, typename V>
 __device__ __forceinline__ bool model_sample(const K *keys, const V *vals, uint64_t i, const ModelParams &mp,
                                              const Mat4 &w2e, float &ex, float &ey, float &z, float &zs,
                                              float c[3])
 {
     const V val = vals[i];
+    const K key = keys[i];      // both loads in flight before the first test
+    loaded_here(val);
+    loaded_here(key);
     if (!(vw(val) > 0.0f))
         return false;
     const float D = vd(val);
     if (!(fabsf(D) < mp.dmax))
         return false;
     uint32_t x, y, zz;
-    unpack_key<K>(keys[i], x, y, zz);
+    unpack_key<K>(key, x, y, zz);
     c[0] = (float)x * mp.vl + mp.c0;
     c[1] = (float)y * mp.vl + mp.c0;
     c[2] = (float)zz * mp.vl + mp.c0;
@@ -136,20 +153,24 @@ __global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict_
     if (q >= W * H)
         return;
     const int y = q / W, x = q - y * W;
-    const float4 P = pts[q];
-    float4 N = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     constexpr int NB = 4;   // central-difference baseline in pixels
+    const int dxs[4] = { NB, -NB, 0, 0 }, dys[4] = { 0, 0, NB, -NB };
+    // the point and its four neighbours in flight together (clamped to the image;
+    // the tests below decide whether they are used)
+    const float4 P = pts[q];
+    float4 nb[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+        nb[d] = pts[(size_t)min(max(y + dys[d], 0), H - 1) * W + min(max(x + dxs[d], 0), W - 1)];
+    float4 N = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (P.w != 0.0f && x >= NB && x < W - NB && y >= NB && y < H - NB) {
         const float disc2 = disc * disc;
-        float4 nb[4];
         bool ok = true;
-        const int dxs[4] = { NB, -NB, 0, 0 }, dys[4] = { 0, 0, NB, -NB };
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            const float4 Q = pts[(size_t)(y + dys[d]) * W + (x + dxs[d])];
+            const float4 Q = nb[d];
             const float a = Q.x - P.x, b = Q.y - P.y, c = Q.z - P.z;
             ok = ok && Q.w != 0.0f && (a * a + b * b) + c * c < disc2;
-            nb[d] = Q;
         }
         if (ok) {
             const float4 r = nb[0], l = nb[1], dn = nb[2], up = nb[3];

@@ -355,14 +355,15 @@ __global__ __launch_bounds__(256) void k_conv_small(ConvSmall c)
     const uint16_t *wsrc = c.W + (size_t)co * K;
     for (int r = 0; r * NX * 256 < nch || r * NW * 256 < nwl; ++r) {
         uint4 vx[NX];
+        uint32_t inside = 0;   // the zero of a pad chunk is applied at the store: writing it into a
+                               // load's destination register would wait for that load
 #pragma unroll
         for (int u = 0; u < NX; ++u) {
             const int i = min((r * NX + u) * 256 + (int)threadIdx.x, nch - 1);
             const int row = i / cpr, ch = i - row * cpr, ci = row / rows_in, hi = hi0 + (row - ci * rows_in);
             const int hic = min(max(hi, 0), g.Hi - 1), chc = max(ch - 1, 0);
-            const uint4 t = *reinterpret_cast<const uint4 *>(c.X + ((size_t)(b * g.Ci + ci) * g.Hi + hic) * g.Wi +
-                                                             chc * 8);
-            vx[u] = ch > 0 && hi >= 0 && hi < g.Hi ? t : make_uint4(0u, 0u, 0u, 0u);
+            vx[u] = *reinterpret_cast<const uint4 *>(c.X + ((size_t)(b * g.Ci + ci) * g.Hi + hic) * g.Wi + chc * 8);
+            inside |= (uint32_t)(ch > 0 && hi >= 0 && hi < g.Hi) << u;
         }
         const int iw = min(r * NW * 256 + (int)threadIdx.x, nwl - 1);
         uint4 vw = make_uint4(0u, 0u, 0u, 0u);
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) void k_conv_small(ConvSmall c)
         for (int u = 0; u < NX; ++u) {
             const int i = (r * NX + u) * 256 + (int)threadIdx.x;
             if (i < nch)
-                reinterpret_cast<uint4 *>(sx)[i] = vx[u];
+                reinterpret_cast<uint4 *>(sx)[i] = inside >> u & 1u ? vx[u] : make_uint4(0u, 0u, 0u, 0u);
         }
         if (r * NW * 256 + (int)threadIdx.x < nwl) {
             if constexpr (WV)
@@ -534,16 +535,21 @@ __global__ void k_downsample(const uint16_t *__restrict__ mm, int W, int H, uint
     if (q >= kInW * kInH)
         return;
     const int y = q / kInW, x = q - y * kInW;
-    float s = 0.0f, c = 0.0f;
+    uint16_t v[16];   // all 16 loads in flight together (clamped, the bounds applied after)
+#pragma unroll
     for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            v[4 * j + i] = mm[(size_t)min(4 * y + j, H - 1) * W + min(4 * x + i, W - 1)];
+    float s = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int yy = 4 * y + j, xx = 4 * x + i;
-            if (yy < H && xx < W) {
-                const uint16_t d = mm[(size_t)yy * W + xx];
-                if (d) {
-                    s += (float)d * 0.001f;
-                    c += 1.0f;
-                }
+            const uint16_t d = 4 * y + j < H && 4 * x + i < W ? v[4 * j + i] : (uint16_t)0;
+            if (d) {
+                s += (float)d * 0.001f;
+                c += 1.0f;
             }
         }
     out[q] = f2bf(c > 0.0f ? (s / c) * 0.25f : 0.0f);
