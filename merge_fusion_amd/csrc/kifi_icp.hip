@@ -583,9 +583,12 @@ __global__ __launch_bounds__(kIT) void k_icp_assoc_window(IcpLevel lv, ModelPara
 }
 
 // ---------------------------------------------------------------- FP64 solve
+// One thread solves A x = b: six FP64 divisions (the reciprocals of the
+// diagonal) instead of 27; each one is a long dependent sequence on the
+// solving thread, which the other workgroups' next launch waits for.
 __device__ bool cholesky6(const double *A, const double *b, double *x)
 {
-    double L[36];
+    double L[36], inv[6];
     for (int i = 0; i < 36; ++i)
         L[i] = 0.0;
     for (int i = 0; i < 6; ++i)
@@ -597,8 +600,9 @@ __device__ bool cholesky6(const double *A, const double *b, double *x)
                 if (!(s > 0.0))
                     return false;
                 L[6 * i + i] = sqrt(s);
+                inv[i] = 1.0 / L[6 * i + i];
             } else {
-                L[6 * i + j] = s / L[6 * j + j];
+                L[6 * i + j] = s * inv[j];
             }
         }
     double y[6];
@@ -606,13 +610,13 @@ __device__ bool cholesky6(const double *A, const double *b, double *x)
         double s = b[i];
         for (int k = 0; k < i; ++k)
             s -= L[6 * i + k] * y[k];
-        y[i] = s / L[6 * i + i];
+        y[i] = s * inv[i];
     }
     for (int i = 5; i >= 0; --i) {
         double s = y[i];
         for (int k = i + 1; k < 6; ++k)
             s -= L[6 * k + i] * x[k];
-        x[i] = s / L[6 * i + i];
+        x[i] = s * inv[i];
     }
     return true;
 }
@@ -627,8 +631,11 @@ __device__ void twist_to_mat(const double xi[6], double *M)
         a = 1.0;
         b = 0.5;
     } else {
-        a = sin(th) / th;
-        b = (1.0 - cos(th)) / (th * th);
+        double s, c;
+        sincos(th, &s, &c);
+        const double it = 1.0 / th;
+        a = s * it;
+        b = (1.0 - c) * (it * it);
     }
     for (int i = 0; i < 16; ++i)
         M[i] = 0.0;
@@ -770,8 +777,9 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
     // 1 m depth by the sensor noise model sigma ~ z^2
     constexpr int IC = MODE == KIFI_ICP_POINT_TO_PLANE ? 27 : 15;
     const double cnt = v[IC], e2 = v[IC + 1], ew2 = v[MODE == KIFI_ICP_POINT_TO_PLANE ? IC + 2 : IC + 1];
-    st->rms = cnt > 0.0 ? (float)sqrt(e2 / cnt) : 0.0f;
-    st->rms_1m = cnt > 0.0 ? (float)sqrt(ew2 / cnt) : 0.0f;
+    const double icnt = cnt > 0.0 ? 1.0 / cnt : 0.0;
+    st->rms = (float)sqrt(e2 * icnt);
+    st->rms_1m = (float)sqrt(ew2 * icnt);
     st->inliers = (int)cnt;
     if (cnt < (double)sp.min_inliers) {
         st->level_done[level] = 1;
@@ -825,9 +833,10 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
         st->pose_f.m[i] = (float)st->pose[i];
     st->iters[level] += 1;
     st->total_iters += 1;
-    const double rot = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
-    const double tr = sqrt(xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
-    if (rot < sp.eps_rot && tr < sp.eps_trans)
+    // |w| < eps_rot and |t| < eps_trans, compared as squares
+    const double rot2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+    const double tr2 = xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5];
+    if (rot2 < sp.eps_rot * sp.eps_rot && tr2 < sp.eps_trans * sp.eps_trans)
         st->level_done[level] = 1;
 }
 
