@@ -576,6 +576,8 @@ class Model:
         """ICP::Align: returns (eye->world float64[16], stats dict)."""
         p = params if params is not None else IcpParams.default()
         d = np.ascontiguousarray(depth_mm, np.uint16)
+        if d.shape != (self.cam.height, self.cam.width):
+            raise KifiError("frame shape does not match the intrinsics")
         g = np.ascontiguousarray(guess_e2w, np.float64).reshape(16)
         out = np.zeros(16, np.float64)
         st = IcpStats()
@@ -589,6 +591,8 @@ class Model:
         (21 sum w J_i J_j for i >= j row by row, 6 sum w J_i r, count, sum r^2,
         sum w r^2) as the tracker sums them, in float64."""
         d = np.ascontiguousarray(depth_mm, np.uint16)
+        if d.shape != (self.cam.height, self.cam.width):
+            raise KifiError("frame shape does not match the intrinsics")
         g = np.ascontiguousarray(e2w, np.float64).reshape(16)
         out = np.zeros(30, np.float64)
         self.ctx.check(self.ctx._lib.kifi_icp_normal_equations(self._h, d.ctypes.data, ctypes.byref(self.cam), _f64p(g),
@@ -669,6 +673,8 @@ class Pipeline:
         mm or float32 metres."""
         metres = np.asarray(depth).dtype == np.float32
         d = np.ascontiguousarray(depth, np.float32 if metres else np.uint16)
+        if d.shape != (self.cam.height, self.cam.width):
+            raise KifiError("frame shape does not match the intrinsics")
         m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
         fn = self.ctx._lib.kifi_pipeline_integrate_f32 if metres else self.ctx._lib.kifi_pipeline_integrate
         self.ctx.check(fn(self._h, d.ctypes.data, 0, m), "kifi_pipeline_integrate")

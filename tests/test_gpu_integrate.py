@@ -200,3 +200,27 @@ def test_overlapped_host_frames_equal_device_frames(ctx):
     assert np.array_equal(pa, pb) and np.array_equal(ka, kb)
     assert np.array_equal(np.asarray(va).view(np.uint32), np.asarray(vb).view(np.uint32))
     dev.free()
+
+
+def test_invalid_arguments_are_refused(ctx):
+    """The reference asserts (Volume.cpp:22-26, Integrator.cpp:35-36, 89-90) become
+    KIFI_ERR_ARG with a message; nothing is launched and the context stays usable."""
+    for res, side, trunc, kb in ((1000, 4.096, 0.02, 32), (2048, 4.096, 0.02, 32), (1024, 0.0, 0.02, 32),
+                                 (1024, 4.096, 0.0, 32), (1024, 4.096, 0.02, 16)):
+        with pytest.raises(mf.KifiError, match="status %d" % mf.KIFI_ERR_ARG):
+            mf.Volume(ctx, res, side, trunc, kb)
+    v = mf.Volume(ctx, 1024, 4.096, 0.02)
+    bad = mf.Intrinsics(510, 424, 365.0, 365.0)             # width not a multiple of 4
+    with pytest.raises(mf.KifiError, match="status %d" % mf.KIFI_ERR_ARG):
+        v.integrate(np.zeros((424, 510), np.uint16), bad, np.eye(4, dtype=np.float32).reshape(16))
+    with pytest.raises(mf.KifiError):
+        v.integrate(np.zeros((400, 512), np.uint16), CAM, np.eye(4, dtype=np.float32).reshape(16))
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+    with pytest.raises(mf.KifiError):
+        pipe.integrate(np.zeros((424, 256), np.uint16), np.eye(4, dtype=np.float32).reshape(16))
+    pipe.close()
+    sc = mf.make_scene(mf.SCENE_PLANE_SPHERES, 1)
+    d = mf.render(sc, CAM, np.eye(4), noise_seed=1)
+    v.integrate(d, CAM, np.eye(4, dtype=np.float32).reshape(16))
+    assert v.size() > 100000
+    v.close()
