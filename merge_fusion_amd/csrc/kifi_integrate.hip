@@ -448,6 +448,11 @@ __global__ __launch_bounds__(kThreads) void k_merge_dups(const K *__restrict__ A
         return;
     const uint64_t tot = cnt->v_old + cnt->n_new;
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
+    if (mt.nb == 0) {   // no new keys in the tile (most tiles of a grown volume): nothing to look up
+        if (threadIdx.x == 0)
+            tile_dups[blockIdx.x] = 0;
+        return;
+    }
     constexpr int PER = kMgTile / kThreads;
     K ak[PER];
 #pragma unroll
