@@ -863,7 +863,6 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     for (int l = 0; l < 4; ++l)
         st->level_done[l] = 0;
     st->failed = 0;
-    st->active = 1;
     st->fallback = 0;
     st->rms = 0.0f;
     st->rms_1m = 0.0f;
@@ -884,7 +883,6 @@ __device__ __forceinline__ bool fallback_needed(const IcpState *st, float max_rm
 __device__ void fallback_setup(IcpState *st)
 {
     st->fallback = 1;
-    st->active = 1;
     st->alt_rms = st->failed ? FLT_MAX : st->rms_1m;
     st->alt_inliers = st->inliers;
     st->failed = 0;
@@ -927,7 +925,6 @@ __global__ void k_icp_fallback_check(IcpState *__restrict__ st, float max_rms, i
     } else {
         for (int l = 0; l < 4; ++l)
             st->level_done[l] = 1;
-        st->active = 0;
     }
 }
 

@@ -46,7 +46,6 @@ struct IcpState {
     int    total_iters;
     int    level_done[4];   // per pyramid level: converged, failed or not run
     int    failed;
-    int    active;          // gate of the current pass
     int    fallback;        // this frame re-ran plain ICP
     float  rms;
     float  rms_1m;          // RMS of r / d^2 x 1 m^2 (noise-normalised residual)
