@@ -289,7 +289,7 @@ def calibrate_copy(mf, mib):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the fusion path is not sharded); "
+        print("bench.py: the fusion path runs on one GPU (it is not sharded across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import merge_fusion_amd as mf

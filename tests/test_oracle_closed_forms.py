@@ -4,7 +4,7 @@ invariants, brute force on small inputs and textbook special cases.
 Reference functions covered (KinectFission/kifi/...): Integrator.cpp:76-182
 (keygen), :186-250 (expand), algorithm.h:138-198 (set_union), Integrator.cpp:253-358
 (update), Mesher.cpp:85-229 (mesher), Renderer.cpp:39-80 (Bin), ICP.cpp:166-264
-(Horn), and the binary16 storage of this build.
+(Horn), and the binary16 + u8 voxel storage (no reference counterpart).
 """
 import ctypes
 
