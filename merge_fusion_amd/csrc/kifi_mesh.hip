@@ -226,37 +226,49 @@ __device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals,
 {
     constexpr K d1 = 1, d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
     lut = -1;
-    const K k = keys[i];
+    // the voxel's key, its successor's key and its value in flight together
+    // (i <= V - 2, so i + 1 is a voxel)
+    const K k = keys[i], kNext = keys[i + 1];
+    const V v0 = vals[i];
     K kTop, kTop1, kFront, kFront1, kFT = 0, kFT1 = 0;
     const int64_t iTop = win_walk<K, TRIS>(keys, n, mw, wr, 0, k + d2, kTop, kTop1);       // :122-123
     const int64_t iFront = win_walk<K, TRIS>(keys, n, mw, wr, 1, k + d4, kFront, kFront1);  // :125-126
     int64_t iFrontTop = 0;
     if (TRIS)
         iFrontTop = win_walk<K, TRIS>(keys, n, mw, wr, 2, k + d4 + d2, kFT, kFT1);          // :174-175
-    const V v0 = vals[i];
     if (vweight(v0) == 0.0f)                                       // :119-120
         return 0;
     cn.c[0] = (int64_t)i;
-    cn.c[1] = keys[i + 1] == k + d1 ? (int64_t)i + 1 : -1;         // :132
+    cn.c[1] = kNext == k + d1 ? (int64_t)i + 1 : -1;               // :132
     cn.c[2] = kTop == k + d2 ? iTop : -1;                          // :129
     cn.c[4] = kFront == k + d4 ? iFront : -1;                      // :130
-    const float dself = vdist(v0);
-    uint32_t flags = 0;
-    const int axis_corner[3] = { 1, 2, 4 };
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {                                  // :140-168
-        const int64_t c = cn.c[axis_corner[a]];
-        if (c >= 0) {
-            const V vn = vals[c];
-            if (vweight(vn) > 0.0f && dself * vdist(vn) <= 0.0f)
-                flags |= 1u << a;
-        }
-    }
     if (TRIS) {
         cn.c[6] = kFT == k + d4 + d2 ? iFrontTop : -1;             // :177
         cn.c[3] = kTop1 == k + d2 + d1 ? iTop + 1 : -1;            // :179
         cn.c[5] = kFront1 == k + d4 + d1 ? iFront + 1 : -1;
         cn.c[7] = kFT1 == k + d4 + d2 + d1 ? iFrontTop + 1 : -1;
+    }
+    // every corner value in flight together: absent corners read the voxel's
+    // own value (in bounds), and the tests below ignore it
+    constexpr int NC = TRIS ? 8 : 5;
+    V vc[NC];
+#pragma unroll
+    for (int c = 1; c < NC; ++c)
+        if (TRIS || c == 1 || c == 2 || c == 4)
+            vc[c] = vals[cn.c[c] >= 0 ? (uint64_t)cn.c[c] : i];
+    const float dself = vdist(v0);
+    uint32_t flags = 0;
+    const int axis_corner[3] = { 1, 2, 4 };
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {                                  // :140-168
+        const int ci = axis_corner[a];
+        if (cn.c[ci] >= 0) {
+            const V vn = vc[ci];
+            if (vweight(vn) > 0.0f && dself * vdist(vn) <= 0.0f)
+                flags |= 1u << a;
+        }
+    }
+    if (TRIS) {
         bool skip = false;
         float d[8];
         d[0] = dself;
@@ -267,10 +279,9 @@ __device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals,
                 d[c] = 0.0f;
                 continue;
             }
-            const V vc = vals[cn.c[c]];
-            if (vweight(vc) == 0.0f)
+            if (vweight(vc[c]) == 0.0f)
                 skip = true;
-            d[c] = skip ? 0.0f : vdist(vc);
+            d[c] = skip ? 0.0f : vdist(vc[c]);
         }
         if (!skip)                                                 // :191-199
             lut = (d[0] < 0.0f) << 2 | (d[1] < 0.0f) << 3 | (d[2] < 0.0f) << 6 |
