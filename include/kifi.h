@@ -104,7 +104,8 @@ kifi_status kifi_integrate_f32(kifi_volume *vol, const float *depth_m, const kif
                                const float worldToEye[16]);
 kifi_status kifi_integrate_device_f32(kifi_volume *vol, const float *depth_m_dev,
                                       const kifi_intrinsics *cam, const float worldToEye[16]);
-/* Counts of the last integration: [0] K keygen keys, [1] 8-neighbourhood candidates,
+/* Counts of the last integration: [0] K keygen keys, [1] 8-neighbourhood candidates
+ * sorted (at most 8K: a half block equal to the previous pixel's is not written),
  * [2] E keys handed to the merge, [3] volume size after the merge.  Synchronous. */
 kifi_status kifi_integrate_stats(kifi_volume *vol, size_t stats[4]);
 /* Running totals since the volume was created: [0] integrations, [1] sum of V0,

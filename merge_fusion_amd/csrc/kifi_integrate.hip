@@ -115,27 +115,56 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
         flags |= (uint32_t)ok << j;
         c += ok;
     }
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan<kThreads>(c, tmp, tot);
-    if (threadIdx.x == 0 && tot) {
-        base_s = atomicAdd(&cnt->n_cand, tot * 8u);
-        atomicAdd(&cnt->n_keys, tot);
-    }
-    __syncthreads();
-    if (!c)
-        return;
+    // The 2x2x2 block of a key is its x+0 half (key + {0, dy, dz, dz+dy}) and its
+    // x+1 half (the same + dx).  A half already in the left lane's block (the
+    // previous pixel of the row, same key or one x step away) is not written:
+    // the sort and unique that follow see the same key set (every dropped key
+    // is written by some lane further left), from fewer candidates.
     constexpr K dx = 1, dy = K(1) << KeyTraits<K>::kb, dz = K(1) << (2 * KeyTraits<K>::kb);
-    K *o = out + base_s + ex * 8u;
+    const int lane = __lane_id();
+    uint32_t halves = 0, nw = 0;   // 2 bits per pixel: bit 0 = x+0 half, bit 1 = x+1 half
 #pragma unroll
     for (int j = 0; j < kKgItems; ++j) {
-        if (!(flags >> j & 1u))
-            continue;
+        const bool ok = flags >> j & 1u;
+        const K lk = __shfl_up(keys[j], 1, kWave);
+        const int lok = __shfl_up((int)ok, 1, kWave);
+        uint32_t h = ok ? 3u : 0u;
+        if (ok && lane > 0 && lok) {
+            if (lk == keys[j])
+                h = 0u;
+            else if (lk == keys[j] + dx)
+                h = 1u;            // the left block's x+0 half is this block's x+1 half
+            else if (lk + dx == keys[j])
+                h = 2u;            // the left block's x+1 half is this block's x+0 half
+        }
+        halves |= h << (2 * j);
+        nw += 4u * ((h & 1u) + (h >> 1));
+    }
+    // one scan for both counts: written candidates (<= 16384 per block) and valid pixels
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<kThreads>(nw | (c << 16), tmp, tot);
+    if (threadIdx.x == 0 && tot) {
+        base_s = atomicAdd(&cnt->n_cand, tot & 0xffffu);
+        atomicAdd(&cnt->n_keys, tot >> 16);
+    }
+    __syncthreads();
+    if (!nw)
+        return;
+    K *o = out + base_s + (ex & 0xffffu);
+#pragma unroll
+    for (int j = 0; j < kKgItems; ++j) {
+        const uint32_t h = halves >> (2 * j) & 3u;
         const K key = keys[j];
-        o[0] = key;           o[1] = key + dx;
-        o[2] = key + dy;      o[3] = key + dy + dx;
-        o[4] = key + dz;      o[5] = key + dz + dx;
-        o[6] = key + dz + dy; o[7] = key + dz + dy + dx;
-        o += 8;
+        if (h & 1u) {
+            o[0] = key;      o[1] = key + dy;
+            o[2] = key + dz; o[3] = key + dz + dy;
+            o += 4;
+        }
+        if (h & 2u) {
+            o[0] = key + dx;      o[1] = key + dy + dx;
+            o[2] = key + dz + dx; o[3] = key + dz + dy + dx;
+            o += 4;
+        }
     }
 }
 

@@ -171,7 +171,7 @@ struct SplatParams {
 // Reset list for per-frame device counters (zeroed by a fill kernel, never by a
 // memset node, so that the integration sequence can be captured in a graph).
 struct FrameCounters {
-    uint32_t n_cand;      // candidates written by keygen (8 per valid pixel)
+    uint32_t n_cand;      // candidates written by keygen (up to 8 per valid pixel)
     uint32_t n_keys;      // valid pixels (K)
     uint32_t n_unique;    // unique candidates
     uint32_t n_new;       // E = n_unique - 1 (the reference's x-pass drops the max)
