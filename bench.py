@@ -175,6 +175,58 @@ def learned_section(mf, ctx, cam, args):
         pipe.close()
         return dt, st, np.array(errs)
 
+    def downsample(d_mm):
+        """Network input channel: 4x4 block mean of the valid depths (m) / 4 m."""
+        b = (d_mm.astype(np.float32) * np.float32(0.001)).reshape(H // 4, 4, W // 4, 4).transpose(0, 2, 1, 3)
+        b = b.reshape(H // 4, W // 4, 16)
+        c = (b > 0).sum(-1)
+        return np.where(c > 0, b.sum(-1) / np.maximum(c, 1) * np.float32(0.25), 0.0).astype(np.float32)
+
+    def twist_pose(prev_e2w, tw):
+        """prev * [R(omega) | v] in column storage (the pipeline's guess)."""
+        w, v = np.asarray(tw[:3], np.float64), np.asarray(tw[3:], np.float64)
+        th = np.linalg.norm(w)
+        K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        R = np.eye(3) + (np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K if th > 1e-12 else K)
+        T = np.eye(4)
+        T[:3, :3], T[:3, 3] = R, v
+        return (np.asarray(prev_e2w, np.float64).reshape(4, 4).T @ T).T.reshape(16)
+
+    def open_loop(n):
+        """ICP of frame f against a model integrated at the true poses of frames
+        0..f-1, from the true previous pose and from the learned start: the
+        iterations and accuracy the initial pose buys, apart from how long
+        closed-loop tracking holds."""
+        host = [mf.render_device(ctx, scene, cam, poses[f], 5000 + f) for f in range(n)]
+        small = [downsample(h) for h in host]
+        x = np.stack([np.stack([small[f - 1], small[f]]) for f in range(1, n)])
+        raw = np.concatenate([net.forward(x[k:k + 256]) for k in range(0, len(x), 256)])
+        tw = raw * mf.TWIST_SCALE
+        prm = mf.IcpParams.default()
+        prm.level_gate_growth = 2.0
+        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=mf.STORAGE_F16U8, params=prm)
+        out = {"plain_f16": ([], []), "learned_f16": ([], [])}
+        for f in range(1, n):
+            pipe.integrate(host[f - 1], mf.invert_rigid(poses[f - 1]).astype(np.float32))
+            for name, guess in (("plain_f16", poses[f - 1]), ("learned_f16", twist_pose(poses[f - 1], tw[f - 1]))):
+                pose, st = pipe.model.icp_align(host[f], guess, prm)
+                out[name][0].append(st["iterations"])
+                out[name][1].append(mf.pose_error(pose, poses[f]))
+        pipe.close()
+        res_ol = {}
+        for name, (its, errs) in out.items():
+            its, errs = np.array(its), np.array(errs)
+            res_ol[name] = {
+                "icp_iters_per_frame": round(float(its.mean()), 3),
+                "frames_within_2cm_1deg": round(float(np.mean((errs[:, 0] < 0.02) & (errs[:, 1] < 1.0))), 4),
+                "median_error_cm_deg": [round(float(np.median(errs[:, 0])) * 100, 3),
+                                        round(float(np.median(errs[:, 1])), 3)],
+            }
+        res_ol["icp_iters_reduction"] = round(1.0 - res_ol["learned_f16"]["icp_iters_per_frame"] /
+                                              max(res_ol["plain_f16"]["icp_iters_per_frame"], 1e-9), 4)
+        res_ol["frames"] = n - 1
+        return res_ol
+
     res = {}
     for name, storage, use_net in (("plain_f32", mf.STORAGE_F32, False), ("plain_f16", mf.STORAGE_F16U8, False),
                                    ("learned_f16", mf.STORAGE_F16U8, True)):
@@ -188,6 +240,7 @@ def learned_section(mf, ctx, cam, args):
             "ate_m": round(float(np.sqrt(np.mean(e ** 2))), 4),
             "first_frame_error_over_10cm": lost,
         }
+    ol = open_loop(min(nf, 300))
     buf.free()
     net.close()
     base = res["plain_f32"]
@@ -210,6 +263,9 @@ def learned_section(mf, ctx, cam, args):
                                                                               axis=1).mean() * 100), 3),
         },
         "results": res,
+        "open_loop": dict(ol, note="ICP of each frame against a model integrated at the true poses, from the "
+                                   "true previous pose (plain) and from the learned start, no fallback: what the "
+                                   "initial pose buys, apart from how long closed-loop tracking holds"),
         "fps_ratio_learned_f16_vs_plain_f32": round(res["learned_f16"]["frames_per_s"] / base["frames_per_s"], 3),
         "icp_iters_reduction_vs_plain": round(1.0 - res["learned_f16"]["icp_iters_per_frame"] /
                                               max(base["icp_iters_per_frame"], 1e-9), 4),
