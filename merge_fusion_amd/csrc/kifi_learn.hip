@@ -95,13 +95,10 @@ __global__ __launch_bounds__(256) void k_gemm(Op op)
     for (int k = kb; k < ke; k += 16) {
         bf16x4 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int kk = k + kq + j;
-                a[i][j] = (short)(kk < ke ? op.a(m0 + 16 * i + r, kk) : 0);
-                b[i][j] = (short)(kk < ke ? op.b(kk, n0 + 16 * i + r) : 0);
-            }
+        for (int i = 0; i < 2; ++i) {
+            op.a4(m0 + 16 * i + r, k + kq, ke, a[i]);
+            op.b4(k + kq, n0 + 16 * i + r, ke, b[i]);
+        }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -128,6 +125,27 @@ void launch_gemm(const Op &op, int splits, hipStream_t st)
     hipLaunchKernelGGL(k_gemm<Op>, grid, dim3(256), 0, st, op);
 }
 
+// Four consecutive k (k0 .. k0+3) of an operand row or column, 0 at k >= ke.
+// The default takes them one element at a time; an operator whose element
+// index needs runtime divisions replaces it with one decomposition per call.
+template <class D>
+struct GemmElems {
+    __device__ void a4(int m, int k0, int ke, bf16x4 &o) const
+    {
+        const D &d = static_cast<const D &>(*this);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = (short)(k0 + j < ke ? d.a(m, k0 + j) : 0);
+    }
+    __device__ void b4(int k0, int n, int ke, bf16x4 &o) const
+    {
+        const D &d = static_cast<const D &>(*this);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = (short)(k0 + j < ke ? d.b(k0 + j, n) : 0);
+    }
+};
+
 // Split of a reduction dimension K into `splits` chunks of multiples of 16.
 inline int chunk_of(int K, int splits) { return ((K + splits - 1) / splits + 15) / 16 * 16; }
 
@@ -136,7 +154,7 @@ struct ConvGeom {
 };
 
 // forward: M = B*Ho*Wo, N = Co, K = Ci*9; epilogue bias + ReLU -> bf16
-struct OpConvFwd {
+struct OpConvFwd : GemmElems<OpConvFwd> {
     ConvGeom g;
     const uint16_t *X, *W;
     const float *bias;
@@ -164,7 +182,7 @@ struct OpConvFwd {
 
 // backward data: dX[b][ci][hi][wi] = sum over (co, ky, kx) with hi = 2 ho - 1 + ky
 // of dY[b][co][ho][wo] W[co][ci][ky][kx], then x ReLU'(X)
-struct OpConvBwdData {
+struct OpConvBwdData : GemmElems<OpConvBwdData> {
     ConvGeom g;
     const uint16_t *dY, *W, *X;
     uint16_t *dX;
@@ -199,7 +217,7 @@ struct OpConvBwdData {
 };
 
 // backward weights: dW[co][ci*9+t] = sum over (b, ho, wo) dY[b][co][ho][wo] X[b][ci][2ho-1+ky][2wo-1+kx]
-struct OpConvBwdW {
+struct OpConvBwdW : GemmElems<OpConvBwdW> {
     ConvGeom g;
     const uint16_t *dY, *X;
     float *part;     // [splits][M][N]
@@ -222,11 +240,57 @@ struct OpConvBwdW {
             return 0;
         return X[((size_t)(b * g.Ci + ci) * g.Hi + hi) * g.Wi + wi];
     }
+    // Four k of a row / column: k = (b, ho, wo) is decomposed once and stepped,
+    // and every load is issued (indices clamped into the tensors) before the
+    // masks apply.  Same values as a() / b().
+    __device__ void a4(int m, int k0, int ke, bf16x4 &o) const
+    {
+        const int hw = g.Ho * g.Wo, mm = min(m, M - 1);
+        int b = k0 / hw, p = k0 - b * hw;
+        uint16_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t idx = k0 + j < K ? ((size_t)(b * g.Co + mm)) * hw + p : 0;
+            v[j] = dY[idx];
+            if (++p == hw) {
+                p = 0;
+                ++b;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = (short)(m < M && k0 + j < ke ? v[j] : 0);
+    }
+    __device__ void b4(int k0, int n, int ke, bf16x4 &o) const
+    {
+        const int hw = g.Ho * g.Wo, nn = min(n, N - 1);
+        const int ci = nn / 9, t = nn - ci * 9, ky = t / 3, kx = t - ky * 3;
+        int b = k0 / hw, p = k0 - b * hw, ho = p / g.Wo, wo = p - ho * g.Wo;
+        uint16_t v[4];
+        bool ok[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int hi = 2 * ho - 1 + ky, wi = 2 * wo - 1 + kx;
+            ok[j] = n < N && k0 + j < ke && hi >= 0 && hi < g.Hi && wi >= 0 && wi < g.Wi;
+            const int bc = min(b, g.B - 1), hc = min(max(hi, 0), g.Hi - 1), wc = min(max(wi, 0), g.Wi - 1);
+            v[j] = X[((size_t)(bc * g.Ci + ci) * g.Hi + hc) * g.Wi + wc];
+            if (++wo == g.Wo) {
+                wo = 0;
+                if (++ho == g.Ho) {
+                    ho = 0;
+                    ++b;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = (short)(ok[j] ? v[j] : 0);
+    }
     __device__ void store(int m, int n, float v, int z) const { part[((size_t)z * M + m) * N + n] = v; }
 };
 
 // fully connected forward: Y[b][n] = sum_k X[b][k] W[n][k] + bias[n] (ReLU -> bf16, or f32 out)
-struct OpFcFwd {
+struct OpFcFwd : GemmElems<OpFcFwd> {
     const uint16_t *X, *W;
     const float *bias;
     uint16_t *Y;       // bf16 + ReLU when set
@@ -245,7 +309,7 @@ struct OpFcFwd {
 };
 
 // fully connected backward data: dX[b][k] = sum_n dY[b][n] W[n][k], x ReLU'(X)
-struct OpFcBwdData {
+struct OpFcBwdData : GemmElems<OpFcBwdData> {
     const uint16_t *dY, *W, *X;
     uint16_t *dX;
     int M, N, K, kchunk;    // M = B, N = in features, K = out features
@@ -259,7 +323,7 @@ struct OpFcBwdData {
 };
 
 // fully connected backward weights: dW[n][k] = sum_b dY[b][n] X[b][k]
-struct OpFcBwdW {
+struct OpFcBwdW : GemmElems<OpFcBwdW> {
     const uint16_t *dY, *X;
     float *part;
     int M, N, K, kchunk;    // M = out features, N = in features, K = B
