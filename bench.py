@@ -152,12 +152,14 @@ def learned_section(mf, ctx, cam, args):
         mf.render_device(ctx, scene, cam, poses[f], 5000 + f, out_ptr=buf.address(f * fbytes))
     ctx.synchronize()
 
-    def run(storage, use_net, track):
+    def run(storage, use_net, track, coarse=4):
         prm = mf.IcpParams.default()
         prm.level_gate_growth = 2.0       # 10/20/40 cm gates: the fast motion's large start errors
+        prm.iters[2] = coarse
         if use_net:
             prm.fallback_rms = 0.005      # 5 mm (1 m-scaled residual)
             prm.fallback_iters = 20       # inert with the 10/5/4 budget (19 iterations)
+            prm.first_iters[2] = 2        # from the prediction the coarse level stops after 2; the fallback runs 4
         pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=storage, params=prm)
         if use_net:
             pipe.set_twistnet(net)
@@ -204,12 +206,18 @@ def learned_section(mf, ctx, cam, args):
         tw = raw * mf.TWIST_SCALE
         prm = mf.IcpParams.default()
         prm.level_gate_growth = 2.0
+        prm2 = mf.IcpParams.default()
+        prm2.level_gate_growth = 2.0
+        prm2.iters[2] = 2
         pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=mf.STORAGE_F16U8, params=prm)
-        out = {"plain_f16": ([], []), "learned_f16": ([], [])}
+        runs = (("plain_f16", False, prm), ("learned_f16", True, prm),
+                ("plain_f16_coarse2", False, prm2), ("learned_f16_coarse2", True, prm2))
+        out = {name: ([], []) for name, _, _ in runs}
         for f in range(1, n):
             pipe.integrate(host[f - 1], mf.invert_rigid(poses[f - 1]).astype(np.float32))
-            for name, guess in (("plain_f16", poses[f - 1]), ("learned_f16", twist_pose(poses[f - 1], tw[f - 1]))):
-                pose, st = pipe.model.icp_align(host[f], guess, prm)
+            for name, learned, pr in runs:
+                guess = twist_pose(poses[f - 1], tw[f - 1]) if learned else poses[f - 1]
+                pose, st = pipe.model.icp_align(host[f], guess, pr)
                 out[name][0].append(st["iterations"])
                 out[name][1].append(mf.pose_error(pose, poses[f]))
         pipe.close()
@@ -224,14 +232,20 @@ def learned_section(mf, ctx, cam, args):
             }
         res_ol["icp_iters_reduction"] = round(1.0 - res_ol["learned_f16"]["icp_iters_per_frame"] /
                                               max(res_ol["plain_f16"]["icp_iters_per_frame"], 1e-9), 4)
+        res_ol["icp_iters_reduction_coarse2"] = round(1.0 - res_ol["learned_f16_coarse2"]["icp_iters_per_frame"] /
+                                                      max(res_ol["plain_f16_coarse2"]["icp_iters_per_frame"], 1e-9), 4)
         res_ol["frames"] = n - 1
         return res_ol
 
     res = {}
-    for name, storage, use_net in (("plain_f32", mf.STORAGE_F32, False), ("plain_f16", mf.STORAGE_F16U8, False),
-                                   ("learned_f16", mf.STORAGE_F16U8, True)):
-        dt, st, _ = run(storage, use_net, False)
-        _, _, e = run(storage, use_net, True)
+    # plain_f32_coarse2: plain ICP under the learned first pass's 10/5/2 schedule,
+    # to separate what the shorter coarse level gives from what the start gives
+    for name, storage, use_net, coarse in (("plain_f32", mf.STORAGE_F32, False, 4),
+                                           ("plain_f16", mf.STORAGE_F16U8, False, 4),
+                                           ("plain_f32_coarse2", mf.STORAGE_F32, False, 2),
+                                           ("learned_f16", mf.STORAGE_F16U8, True, 4)):
+        dt, st, _ = run(storage, use_net, False, coarse)
+        _, _, e = run(storage, use_net, True, coarse)
         lost = int(np.argmax(e > 0.10)) if (e > 0.10).any() else None
         res[name] = {
             "frames_per_s": round((nf - 1) / dt, 2),
@@ -246,9 +260,10 @@ def learned_section(mf, ctx, cam, args):
     base = res["plain_f32"]
     return {
         "workload": "fast trajectory (steps U[2,10] cm, U[2,15] deg), %d frames of the room scene, 512x424, "
-                    "4 mm voxels, 3-level point-to-plane ICP 10/5/4 with 10/20/40 cm gates (both variants); learned: CNN twist as ICP start, plain ICP "
+                    "4 mm voxels, 3-level point-to-plane ICP 10/5/4 with 10/20/40 cm gates (both variants); learned: CNN twist as ICP start "
+                    "with the coarse level stopped after 2 iterations (10/5/2), plain ICP 10/5/4 "
                     "from the previous pose as fallback when the 1 m-scaled residual > 5 mm or 20 iterations "
-                    "are reached (the 10/5/4 budget is 19)" % nf,
+                    "are reached (inert: the budgets are 17 and 19); plain_f32_coarse2 is plain ICP at 10/5/2" % nf,
         "training": {
             "pairs": args.train_pairs, "epochs": args.train_epochs, "batch": 256, "optimizer": "adam, cosine lr 1e-3",
             "pair_generation_s": round(t_gen, 2), "train_s": round(t_train, 2),
@@ -264,8 +279,9 @@ def learned_section(mf, ctx, cam, args):
         },
         "results": res,
         "open_loop": dict(ol, note="ICP of each frame against a model integrated at the true poses, from the "
-                                   "true previous pose (plain) and from the learned start, no fallback: what the "
-                                   "initial pose buys, apart from how long closed-loop tracking holds"),
+                                   "true previous pose (plain) and from the learned start, no fallback, at 10/5/4 "
+                                   "and at 10/5/2 (_coarse2): what the initial pose buys, apart from how long "
+                                   "closed-loop tracking holds"),
         "fps_ratio_learned_f16_vs_plain_f32": round(res["learned_f16"]["frames_per_s"] / base["frames_per_s"], 3),
         "icp_iters_reduction_vs_plain": round(1.0 - res["learned_f16"]["icp_iters_per_frame"] /
                                               max(base["icp_iters_per_frame"], 1e-9), 4),

@@ -167,11 +167,16 @@ typedef struct {
                            /* fallback pass gives up (0: 2^18, about 0.1 s).  A test  */
                            /* hook: a tiny limit makes the pass time out on purpose,  */
                            /* and the next state read returns KIFI_ERR_STATE          */
+    int   first_iters[3];  /* with a fallback configured (fallback_rms > 0 or          */
+                           /* fallback_iters > 0), the first pass starts from a        */
+                           /* predicted pose and stops level l after first_iters[l]    */
+                           /* iterations when that is > 0 (0: iters[l]); the fallback  */
+                           /* pass from the previous pose always runs iters            */
 } kifi_icp_params;
 
 typedef struct {
     int   iterations;      /* solves performed in the frame (both passes)              */
-    int   iters[3];        /* of the last pass, per level                              */
+    int   iters[3];        /* solves per level in the frame (both passes)              */
     float rms;             /* RMS residual of the last association (m)                 */
     int   inliers;         /* correspondences of the last association                  */
     int   failed;          /* too few inliers or a singular system                     */

@@ -73,7 +73,8 @@ class IcpParams(ctypes.Structure):
                 ("dist_thresh", ctypes.c_float), ("eps_rot", ctypes.c_float), ("eps_trans", ctypes.c_float),
                 ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int),
                 ("level_gate_growth", ctypes.c_float), ("naive_sums", ctypes.c_int),
-                ("eps_level_growth", ctypes.c_float), ("spin_limit", ctypes.c_int)]
+                ("eps_level_growth", ctypes.c_float), ("spin_limit", ctypes.c_int),
+                ("first_iters", ctypes.c_int * 3)]
 
     @classmethod
     def default(cls):

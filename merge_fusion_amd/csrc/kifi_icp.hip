@@ -1330,8 +1330,15 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrins
     else if (levels == 2)
         hipLaunchKernelGGL(k_depth_down, dim3((W / 2 * (H / 2) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
                            w->lv[1]);
-    icp_pass(m, w, cam, ip, levels);
-    if ((ip->fallback_rms > 0.0f || ip->fallback_iters > 0) && !icp_fallback_multi(m, w, cam, ip, levels)) {
+    // the first pass starts from the predicted pose when a fallback is configured,
+    // and may stop its levels earlier than the fallback from the previous pose
+    const bool with_fallback = ip->fallback_rms > 0.0f || ip->fallback_iters > 0;
+    kifi_icp_params first = *ip;
+    for (int l = 0; l < 3; ++l)
+        if (with_fallback && ip->first_iters[l] > 0)
+            first.iters[l] = ip->first_iters[l];
+    icp_pass(m, w, cam, &first, levels);
+    if (with_fallback && !icp_fallback_multi(m, w, cam, ip, levels)) {
         hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
                            ip->fallback_iters);
         icp_pass(m, w, cam, ip, levels);

@@ -67,6 +67,8 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->naive_sums = 0;
     p->eps_level_growth = 4.0f;   // half the resolution and a quarter of the samples per coarser level
     p->spin_limit = 0;
+    for (int l = 0; l < 3; ++l)
+        p->first_iters[l] = 0;    // the first pass runs iters
     return KIFI_OK;
 }
 

@@ -245,6 +245,31 @@ def test_fallback_pass_reruns_plain_icp_from_previous_pose(ctx):
             assert np.array_equal(poses[f], plain[f]), (gate, f)
 
 
+def test_first_pass_level_caps_keep_the_fallback_schedule(ctx):
+    """first_iters caps the levels of the first pass (the pass from a predicted
+    pose) when a fallback is configured; the fallback pass from the previous pose
+    keeps iters.  Frame 2 starts at the previous pose under a gate that always
+    holds, so both passes run (the per-level counts add both): uncapped, the
+    first pass is the fallback's twin; capped at one iteration per level, it
+    adds exactly one per level, and the fallback still runs plain tracking's."""
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    truth = [mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f) for f in range(3)]
+    frames = [mf.render(sc, CAM, p, noise_seed=900 + f) for f, p in enumerate(truth)]
+    w2e0 = mf.invert_rigid(truth[0]).astype(np.float32)
+    _, st_plain = _track(ctx, frames, w2e0, mf.IcpParams.default())
+    prm = mf.IcpParams.default()
+    prm.fallback_rms = 1e-9
+    prm.fallback_iters = 0
+    _, st_full = _track(ctx, frames, w2e0, prm, {2: lambda prev: prev})
+    assert st_full["fallbacks"] == 1 and st_full["iters"] == [2 * i for i in st_plain["iters"]], (st_full, st_plain)
+    assert st_full["iterations"] == 2 * sum(st_plain["iters"]), (st_full, st_plain)
+    for l in range(3):
+        prm.first_iters[l] = 1
+    _, st_cap = _track(ctx, frames, w2e0, prm, {2: lambda prev: prev})
+    assert st_cap["fallbacks"] == 1 and st_cap["iters"] == [i + 1 for i in st_plain["iters"]], (st_cap, st_plain)
+    assert st_cap["iterations"] == sum(st_plain["iters"]) + 3, (st_cap, st_plain)
+
+
 def test_fallback_pass_wait_timeout_is_reported(ctx):
     """Every wait of the one-launch fallback pass is bounded.  A spin limit of one
     poll makes the waits give up on purpose: every workgroup leaves, the frames
