@@ -85,7 +85,7 @@ __device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, int pi
     return valid;
 }
 
-constexpr int kKgItems = 8;   // pixels per thread: one counter atomic per 2048 pixels
+constexpr int kKgItems = 4;   // pixels per thread: one counter atomic per 1024 pixels
 
 // dm_out (uint16 frames): receives the frame in metres for the voxel update
 template <typename K, typename D>
