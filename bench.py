@@ -8,22 +8,26 @@ and 2 % invalid pixels; 4 mm voxels (res 1024, side 4.096 m, truncation 2 cm,
 binary32 sum + weight, 32-bit keys); per frame: 3-level point-to-plane ICP
 (10/5/4 iterations) against the model of the previous frame, integration at
 the tracked pose, model prediction; a mesh (Mesher::Mesh, vertices copied to
-the host) every 50 frames.  One step = one frame.  Frames are rendered on the
-host and copied to HBM before timing; the timed region covers frames
-[warmup, warmup + steps).
+page-locked host memory) at frames 49, 99, 149 and 199.
 
-    python bench.py [--gpus 1] [--steps 180] [--warmup 20]
+One step = the whole 200-frame trajectory from an empty volume (frame 0 at
+its true pose, frames 1..199 tracked, the four meshes), so every step is the
+workload as stated.  Frames are rendered on the host and copied to HBM before
+timing; `value` = 200 x steps / the timed wall time.
 
-Prints one JSON line.  The CPU baseline is the single-threaded oracle
-(integration, model prediction, FP64 ICP) on a bounded sample of the same
-trajectory, timed on this host.  "fps_with_h2d" repeats the timed frames handed
-over from host memory.  "learned_vs_plain" trains the twist-prediction network
-on this GPU (seeded pairs, bounded budget) and compares plain and learned
-initial poses on a 1000-frame fast trajectory rendered on the device.
-"precision_variant_2mm" times integration into a 2 mm, 64-bit-key volume of
-about 50 M active voxels (binary16 + u8 and binary32 storage).
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+
+Prints one JSON line.  Besides the headline it reports: the fused-frame
+algorithmic GB/s against HBM peak and against a measured copy kernel; the
+single-frame plain configuration (8 mm voxels, integrate + mesh); the host-frame
+(PCIe-inclusive) rate; the CPU leg (the single-threaded oracle timed on one
+pinned host core, which is also the checker of the GPU's errors against the
+FP64 oracle on sampled frames and voxels); the learned-vs-plain comparison on
+the 1000-frame fast trajectory (network trained here on this GPU); and the
+2 mm / 64-bit-key precision variant at about 50 M voxels.
 """
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -38,19 +42,21 @@ sys.path.insert(0, ROOT)
 METRIC = "fused frames/s at 512x424, 4 mm voxels; ICP iters/frame, learned vs plain init"
 HBM_PEAK_GBS = 8000.0
 W, H, F = 512, 424, 365.0
+P = W * H
 SEED = 2
 RES, SIDE, TRUNC = 1024, 4.096, 0.02
 S_K, S_V = 4, 8            # key and value bytes (32-bit keys, binary32 sum + weight)
+TRAJ_FRAMES, MESH_EVERY = 200, 50
+LEVEL_PIXELS = (P, P // 4, P // 16)   # ICP pyramid 512x424, 256x212, 128x106
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=180)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mesh-every", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=10, help="timed trajectories of 200 frames")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed trajectories before the timed ones")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline timing and FP64 checks)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-initialisation comparison")
     ap.add_argument("--fast-frames", type=int, default=1000, help="frames of the fast trajectory")
     ap.add_argument("--train-pairs", type=int, default=100000)   # the 100k pairs of the training workload
@@ -64,6 +70,34 @@ def parse():
     return ap.parse_args()
 
 
+class Roctx:
+    """roctxProfilerPause/Resume when the ROCTx library is present (no-ops
+    otherwise and when no profiler is attached).  Under `rocprofv3
+    --selected-regions` only the resumed region is traced: the timed window of
+    the roofline pass, so a kernel trace of this very command gives the
+    dominant kernel's launches of that window and nothing else."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so", "libroctx64.so.4"):
+            try:
+                lib = ctypes.CDLL(name)
+                lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+                lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+                self.lib = lib
+                break
+            except (OSError, AttributeError):
+                continue
+
+    def pause(self):
+        if self.lib:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib:
+            self.lib.roctxProfilerResume(0)
+
+
 def render_frames(mf, cam, n):
     scene = mf.make_scene(mf.SCENE_ROOM, SEED)
     poses = [mf.trajectory_pose(mf.TRAJ_CIRCLE, SEED, f) for f in range(n)]
@@ -71,26 +105,56 @@ def render_frames(mf, cam, n):
     return frames, poses
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+class PinnedCore:
+    """Runs the enclosed block on one host core (sched_setaffinity of this
+    process, restored afterwards): the reference path is single-threaded."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+        return False
+
+
 def cpu_baseline(mf, cam, frames, poses, budget_s):
-    """Oracle pipeline on the first frames of the trajectory until the time budget is spent."""
+    """The oracle pipeline (FP64 point-to-plane ICP 10/5/4, binary32 integration,
+    model prediction) on the first frames of the trajectory, on one pinned
+    host core, until the time budget is spent."""
     import oracle as orc
-    vol = orc.Volume(RES, SIDE, TRUNC)
-    w2e0 = mf.invert_rigid(poses[0]).astype(np.float32)
-    vol.integrate(frames[0], cam, w2e0)
-    pts, nrm = orc.model_predict(vol, w2e0, cam)
-    model_w2e, pose = w2e0, poses[0]
-    times = []
-    t_start = time.perf_counter()
-    f = 1
-    while f < len(frames) and time.perf_counter() - t_start < budget_s:
-        t0 = time.perf_counter()
-        pose, _, _ = orc.icp_f64(frames[f], cam, pts, nrm, model_w2e.astype(np.float64), pose)
-        w2e = mf.invert_rigid(pose).astype(np.float32)
-        vol.integrate(frames[f], cam, w2e)
-        pts, nrm = orc.model_predict(vol, w2e, cam)
-        model_w2e = w2e
-        times.append(time.perf_counter() - t0)
-        f += 1
+    with PinnedCore() as pin:
+        vol = orc.Volume(RES, SIDE, TRUNC)
+        w2e0 = mf.invert_rigid(poses[0]).astype(np.float32)
+        vol.integrate(frames[0], cam, w2e0)
+        pts, nrm = orc.model_predict(vol, w2e0, cam)
+        model_w2e, pose = w2e0, poses[0]
+        times = []
+        t_start = time.perf_counter()
+        f = 1
+        while f < len(frames) and time.perf_counter() - t_start < budget_s:
+            t0 = time.perf_counter()
+            pose, _, _ = orc.icp_f64(frames[f], cam, pts, nrm, model_w2e.astype(np.float64), pose)
+            w2e = mf.invert_rigid(pose).astype(np.float32)
+            vol.integrate(frames[f], cam, w2e)
+            pts, nrm = orc.model_predict(vol, w2e, cam)
+            model_w2e = w2e
+            times.append(time.perf_counter() - t0)
+            f += 1
+        core = pin.core
     ts = np.array(times)
     med = float(np.median(ts[3:] if len(ts) > 5 else ts))
     return {
@@ -99,10 +163,128 @@ def cpu_baseline(mf, cam, frames, poses, budget_s):
         "cores": 1,
         "kind": "port",
         "sample": "oracle pipeline (FP64 point-to-plane ICP 10/5/4, integration, model prediction) on frames "
-                  "1..%d of the same trajectory (volume 0 -> %d voxels), median frame time %.3f s; host %s, "
-                  "nproc %d" % (f - 1, vol.size(), med, platform.processor() or platform.machine(),
-                                os.cpu_count() or 0),
+                  "1..%d of the same trajectory (volume 0 -> %d voxels), median frame time %.3f s after 3 "
+                  "warm-up frames; pinned to host core %d of %d (%s)"
+                  % (f - 1, vol.size(), med, core, os.cpu_count() or 0, cpu_model()),
     }
+
+
+def copy_rate_gbs(mf, ctx, mib=1024, launches=20):
+    """Read + write GB/s of the library's plain 16-byte copy kernel on this GPU."""
+    nbytes = mib << 20
+    a = ctx.device_upload(np.zeros(nbytes // 4, np.uint32))
+    b = ctx.device_upload(np.zeros(nbytes // 4, np.uint32))
+    ctx.device_copy(b.address(), a.address(), nbytes)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        ctx.device_copy(b.address(), a.address(), nbytes)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    a.free()
+    b.free()
+    return 2.0 * nbytes * launches / dt / 1e9
+
+
+def int_bytes(d, s_k, s_v, frames):
+    """Algorithmic bytes of `frames` integrations from the running-total deltas d:
+    2P + 4 S_k K + 2 S_k E + (S_k + S_v)(V0 + 2 V1) + S_v A per frame."""
+    return (2 * P * frames + 4 * s_k * d["keys"] + 2 * s_k * d["new"] +
+            (s_k + s_v) * (d["v_old"] + 2 * d["v_new"]) + s_v * d["updated"])
+
+
+def icp_bytes(level_iters, frames):
+    """36 P_l per iteration at level l, plus 2P + sum_l 24 P_l per tracked frame
+    (pyramid and normals)."""
+    return (sum(36 * LEVEL_PIXELS[l] * level_iters[l] for l in range(3)) +
+            frames * (2 * P + sum(24 * p for p in LEVEL_PIXELS)))
+
+
+def headline(mf, ctx, cam, args, frames, dframes, poses, roctx):
+    """Config-2 trajectories.  Passes: the headline (no events), the roofline
+    kernel's launch durations (only its events; the profiler window), every
+    stage timer, no meshing, and host frames (PCIe copy in each frame)."""
+    fbytes = P * 2
+    mesh_buf = [ctx.host_empty((1 << 22, 3), np.float32)]
+    w2e0 = mf.invert_rigid(poses[0]).astype(np.float32)
+
+    def trajectory(pipe, mesh=True, host=None, record=None):
+        pipe.reset()
+        meshes = []
+        for f in range(TRAJ_FRAMES):
+            w2e = w2e0 if f == 0 else None
+            if host is not None:
+                pipe.integrate(host[f], w2e)
+            else:
+                pipe.integrate_device(dframes.address(f * fbytes), w2e)
+            if record is not None:
+                record(f, pipe)
+            if mesh and (f + 1) % MESH_EVERY == 0:
+                nv = pipe.volume.mesh_into(mesh_buf[0])
+                if nv > len(mesh_buf[0]):
+                    mesh_buf[0] = ctx.host_empty((int(nv * 1.5), 3), np.float32)
+                    nv = pipe.volume.mesh_into(mesh_buf[0])
+                meshes.append(int(nv))
+        return meshes
+
+    def run_pass(stages, mesh=True, host=None, window=False):
+        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
+        for _ in range(args.warmup):
+            trajectory(pipe, mesh, host)
+        ctx.synchronize()
+        st0, tot0 = pipe.stats(), pipe.volume.totals()
+        if stages != []:
+            ctx.timing(True, stages)
+        if window:
+            roctx.resume()
+        t0 = time.perf_counter()
+        meshes = []
+        for _ in range(args.steps):
+            meshes = trajectory(pipe, mesh, host)
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        if window:
+            roctx.pause()
+        ctx.timing(False)
+        st1, tot1 = pipe.stats(), pipe.volume.totals()
+        stage = {name: ctx.timing_read(s) for name, s in (("merge_update", mf.STAGE_MERGE_UPDATE),
+                                                          ("sort", mf.STAGE_SORT), ("icp", mf.STAGE_ICP),
+                                                          ("model", mf.STAGE_MODEL), ("mesh", mf.STAGE_MESH))}
+        pipe.close()
+        d = {k: tot1[k] - tot0[k] for k in tot1}
+        icp = {"frames": st1["frames"] - st0["frames"], "sum_iters": st1["sum_iters"] - st0["sum_iters"],
+               "fallbacks": st1["fallbacks"] - st0["fallbacks"],
+               "level_iters": [st1["sum_level_iters"][l] - st0["sum_level_iters"][l] for l in range(3)]}
+        return dict(elapsed=elapsed, tot=d, icp=icp, stage=stage, meshes=meshes)
+
+    main = run_pass([])
+    roof = run_pass([mf.STAGE_MERGE_UPDATE], window=True)
+    diag = run_pass(None)
+    nomesh = run_pass([], mesh=False)
+    hframes = ctx.host_empty((TRAJ_FRAMES, H, W), np.uint16)
+    hframes[:] = frames
+    h2d = run_pass([], mesh=False, host=hframes)
+
+    # one more trajectory, untimed: every frame's pose, the volume and the
+    # model at sampled frames (for the FP64 checks of the CPU leg)
+    quality = {"poses": [], "volume_at_mesh": [], "samples": {}}
+    sample_frames = (50, 100, 150, 199)
+
+    def record(f, pipe):
+        quality["poses"].append(pipe.pose())
+        if (f + 1) % MESH_EVERY == 0:
+            quality["volume_at_mesh"].append(int(pipe.volume.size()))
+        if f + 1 in sample_frames:
+            pts, nrm = pipe.model.download()       # the model the next frame is tracked against
+            quality["samples"][f + 1] = {"pts": pts, "nrm": nrm, "prev_pose": pipe.pose()}
+        if f in sample_frames:
+            quality["samples"][f]["pose"] = pipe.pose()
+            quality["samples"][f]["start_pose"] = pipe.stats()["start_pose"]
+
+    qpipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
+    qmeshes = trajectory(qpipe, True, None, record)
+    qpipe.close()
+    return dict(main=main, roof=roof, diag=diag, nomesh=nomesh, h2d=h2d, quality=quality, qmeshes=qmeshes)
 
 
 def twistnet_flops_per_sample():
@@ -358,141 +540,199 @@ def calibrate_copy(mf, mib):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 20}))
 
 
+C1_RES, C1_SIDE, C1_TRUNC = 1024, 8.192, 0.04
+
+
+def config1_frame(mf, cam):
+    """The plain single-frame configuration: plane at 2.5 m + 3 spheres, identity pose."""
+    scene = mf.make_scene(mf.SCENE_PLANE_SPHERES, 1)
+    pose = mf.trajectory_pose(mf.TRAJ_STATIC, 1, 0)
+    return mf.render(scene, cam, pose, noise_seed=7), mf.invert_rigid(pose).astype(np.float32)
+
+
+def config1_section(mf, ctx, cam, steps=50):
+    """One frame integrated into an empty 8 mm volume (res 1024, side 8.192 m,
+    truncation 4 cm, binary32) and meshed (Mesher::Mesh, vertices to page-locked
+    host memory): Integrator.cpp:26-71 + Mesher.cpp:14-17.  A step clears the
+    volume, integrates the HBM-resident frame and meshes."""
+    depth, w2e = config1_frame(mf, cam)
+    dframe = ctx.device_upload(depth)
+    vol = mf.Volume(ctx, C1_RES, C1_SIDE, C1_TRUNC)
+    buf = ctx.host_empty((1 << 21, 3), np.float32)
+
+    def step(mesh):
+        vol.clear()
+        vol.integrate_device(dframe.address(), cam, w2e)
+        return vol.mesh_into(buf) if mesh else 0
+
+    for _ in range(3):
+        nv = step(True)
+    res = {}
+    for name, mesh in (("integrate_mesh", True), ("integrate", False)):
+        ctx.synchronize()
+        tot0 = vol.totals()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(mesh)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        d = {k: v - tot0[k] for k, v in vol.totals().items()}
+        nvox = int(vol.size())
+        b = int_bytes(d, S_K, S_V, steps) / steps + ((S_K + S_V) * nvox + 12 * nv if mesh else 0)
+        res[name] = {"frames_per_s": round(1.0 / dt, 2), "ms_per_frame": round(1e3 * dt, 4),
+                     "alg_bytes": int(b), "GBs": round(b / dt / 1e9, 2),
+                     "frac_hbm_peak": round(b / dt / 1e9 / HBM_PEAK_GBS, 5)}
+    keys, vals = vol.download()
+    verts = buf[:nv].copy()
+    vol.close()
+    dframe.free()
+    return {"workload": "one 512x424 frame of plane + 3 spheres at the identity pose, 8 mm voxels (res 1024, "
+                        "side 8.192 m, trunc 4 cm), binary32 sum + weight, integrate into an empty volume + "
+                        "Mesher::Mesh", "voxels": len(keys), "mesh_vertices": int(nv), **res,
+            "_result": (depth, w2e, keys, vals, verts)}
+
+
+def tsdf_error(gk, gd, gw, ok, ov):
+    """Distance error of a GPU volume (keys gk, distances gd, weights gw) against
+    an FP64 oracle volume (keys ok, values ov = [sum, weight] doubles), on the
+    voxels where both have the same positive weight.  A voxel whose centre
+    projects within rounding of a pixel edge reads the neighbouring pixel in one
+    precision ("pixel choice", an error above 1 mm); the others give the
+    arithmetic error."""
+    if not np.array_equal(gk.astype(np.uint64), ok):
+        return {"keys_equal": False}
+    w64 = ov[:, 1]
+    m = (w64 > 0) & (gw == w64)
+    e = np.abs(gd[m] - ov[m, 0] / w64[m])
+    choice = e > 1e-3
+    a = e[~choice]
+    return {"keys_equal": True, "voxels": int(len(gk)), "weight_equal_frac": round(float(np.mean(gw == w64)), 6),
+            "pixel_choice_frac": round(float(choice.mean()), 6), "max_m": float(a.max()),
+            "p99_m": float(np.quantile(a, 0.99)), "mean_m": float(a.mean())}
+
+
+def cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision):
+    """The CPU side of the bench, on one pinned host core: the oracle timed
+    (the CPU baseline), and the oracle as the checker of the GPU's results:
+    config 1 bit-exactness, and errors against FP64 (poses of sampled tracked
+    frames against FP64 ICP from the same start and model; TSDF distances
+    against an FP64 volume fed the same frames and poses)."""
+    import oracle as orc
+    out = {"cpu_baseline": cpu_baseline(mf, cam, frames, poses, args.cpu_seconds)}
+    with PinnedCore():
+        # config 1: the oracle's integrate + mesh, timed and compared bit for bit
+        depth, w2e, gk, gv, gverts = c1["_result"]
+        t0 = time.perf_counter()
+        ov = orc.Volume(C1_RES, C1_SIDE, C1_TRUNC)
+        ov.integrate(depth, cam, w2e)
+        om = ov.mesh()
+        dt = time.perf_counter() - t0
+        ok, ovals = ov.read()
+        g = np.stack([gv["sum"], gv["weight"]], 1)
+        c1["cpu_oracle"] = {"frames_per_s": round(1.0 / dt, 3), "cores": 1}
+        c1["bit_exact_vs_oracle"] = {"keys": bool(np.array_equal(gk.astype(np.uint64), ok)),
+                                     "values": bool(np.array_equal(g.view(np.uint32), ovals.view(np.uint32))),
+                                     "mesh": bool(np.array_equal(gverts.view(np.uint32), om.view(np.uint32)))}
+        # config 2: sampled tracked frames against FP64 ICP from the same start and model
+        q = head["quality"]
+        pose_checks = []
+        for f, smp in sorted(q["samples"].items()):
+            if "pose" not in smp:
+                continue
+            model_w2e = orc.invert_transform(np.asarray(smp["prev_pose"], np.float64).astype(np.float32))
+            opose, oit, _ = orc.icp_f64(frames[f], cam, smp["pts"], smp["nrm"], model_w2e.astype(np.float64),
+                                        smp["start_pose"])
+            dt_m, dr = mf.pose_error(smp["pose"], opose)
+            pose_checks.append([f, round(dt_m * 1e3, 5), round(dr, 6), int(oit)])
+        # config 2: TSDF of the first frames at the GPU-tracked poses, binary32 GPU vs FP64 oracle
+        n_t = 20
+        gvol = mf.Volume(ctx, RES, SIDE, TRUNC)
+        o64 = orc.Volume(RES, SIDE, TRUNC, 10, orc.STORE_F64)
+        for f in range(n_t):
+            w = (mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else
+                 orc.invert_transform(np.asarray(q["poses"][f], np.float64).astype(np.float32)))
+            gvol.integrate(frames[f], cam, w)
+            o64.integrate(frames[f], cam, w)
+        gk2, gv2 = gvol.download()
+        gvol.close()
+        ok2, ov2 = o64.read()
+        del o64
+        err2 = tsdf_error(gk2, gv2["sum"].astype(np.float64) / gv2["weight"], gv2["weight"].astype(np.float64),
+                          ok2, ov2)
+    out["error_vs_fp64"] = {
+        "config2_pose_mm_deg": {"frames": pose_checks,
+                                "note": "[frame, mm, deg, FP64 iterations]: GPU pose of the tracked frame against "
+                                        "FP64 point-to-plane ICP (oracle) from the GPU's start pose against the "
+                                        "GPU's model maps; tolerance 1 mm / 0.1 deg"},
+        "config2_tsdf_f32": dict(err2, frames=n_t, note="frames 0..%d at the GPU-tracked poses into a binary32 "
+                                                         "GPU volume and an FP64 oracle volume" % (n_t - 1)),
+    }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: the fusion path runs on one GPU (it is not sharded across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    roctx = Roctx()
+    roctx.pause()      # a profiler attached with --selected-regions records only the roofline window
     import merge_fusion_amd as mf
     if args.calibrate_copy_mib:
         calibrate_copy(mf, args.calibrate_copy_mib)
         return
     cam = mf.Intrinsics(W, H, F, F)
-    n = args.warmup + args.steps
-    frames, poses = render_frames(mf, cam, n)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(mf, cam, frames, poses, args.cpu_seconds)
+    frames, poses = render_frames(mf, cam, TRAJ_FRAMES)
 
     ctx = mf.Context(0)
     dframes = ctx.device_upload(frames)
-    fbytes = W * H * 2
-    # meshes arrive in page-locked host memory (a pageable buffer halves the copy rate)
-    mesh_buf = [ctx.host_empty((1 << 22, 3), np.float32)]
+    head = headline(mf, ctx, cam, args, frames, dframes, poses, roctx)
+    copy_gbs = copy_rate_gbs(mf, ctx)
+    c1 = config1_section(mf, ctx, cam)
+    learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
+    precision = None if args.no_precision else precision_section(mf, ctx, cam, args)
+    cpu = None if args.no_cpu else cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision)
+    c1.pop("_result")
 
-    def run_pass(stages):
-        """One pass over all frames; the timed region is frames warmup..n-1 with
-        the HIP-event timers of `stages` on (none: no events at all; each event
-        record costs stream time)."""
-        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
-        mesh_counts, pose_err = [], []
-
-        def step(f):
-            if f == 0:
-                pipe.integrate_device(dframes.address(0), mf.invert_rigid(poses[0]).astype(np.float32))
-            else:
-                pipe.integrate_device(dframes.address(f * fbytes))
-            if (f + 1) % args.mesh_every == 0:
-                nv = pipe.volume.mesh_into(mesh_buf[0])
-                if nv > len(mesh_buf[0]):
-                    mesh_buf[0] = ctx.host_empty((int(nv * 1.5), 3), np.float32)
-                    nv = pipe.volume.mesh_into(mesh_buf[0])
-                mesh_counts.append(int(nv))
-                pose_err.append((f, mf.pose_error(pipe.pose(), poses[f])))
-
-        for f in range(args.warmup):
-            step(f)
-        ctx.synchronize()
-        st0, tot0 = pipe.stats(), pipe.volume.totals()
-        if stages != []:
-            ctx.timing(True, stages)
-        t0 = time.perf_counter()
-        for f in range(args.warmup, n):
-            step(f)
-        ctx.synchronize()
-        elapsed = time.perf_counter() - t0
-        ctx.timing(False)
-        st1, tot1 = pipe.stats(), pipe.volume.totals()
-        final_err = mf.pose_error(pipe.pose(), poses[n - 1])
-        stage = {name: ctx.timing_read(s) for name, s in (("merge_update", mf.STAGE_MERGE_UPDATE),
-                                                          ("sort", mf.STAGE_SORT), ("icp", mf.STAGE_ICP),
-                                                          ("model", mf.STAGE_MODEL), ("mesh", mf.STAGE_MESH))}
-        vol = int(pipe.volume.size())
-        pipe.close()
-        return dict(elapsed=elapsed, st0=st0, st1=st1, tot0=tot0, tot1=tot1, final_err=final_err, stage=stage,
-                    mesh_counts=mesh_counts, pose_err=pose_err, vol=vol)
-
-    # headline: no events at all; then the same frames with only the roofline
-    # kernel's events (its launch durations), then with every stage timer
-    main = run_pass([])
-    roof = run_pass([mf.STAGE_MERGE_UPDATE])
-    diag = run_pass(None)
-    elapsed, st0, st1 = main["elapsed"], main["st0"], main["st1"]
-    tot0, tot1 = roof["tot0"], roof["tot1"]
-    final_err, mesh_counts, pose_err, vol_final = main["final_err"], main["mesh_counts"], main["pose_err"], main["vol"]
+    main_p, roof, diag = head["main"], head["roof"], head["diag"]
+    frames_timed = TRAJ_FRAMES * args.steps
+    fps = frames_timed / main_p["elapsed"]
     stage = dict(diag["stage"])
     stage["merge_update"] = roof["stage"]["merge_update"]
-    fps = args.steps / elapsed
 
     # roofline of the dominant kernel: fused merge + TSDF update.  Algorithmic
     # bytes per launch = read V0 keys+values + E new keys, write V1 keys+values,
     # read the depth frame once.
-    launches = tot1["frames"] - tot0["frames"]
-    dv0, de, dv1 = (tot1[k] - tot0[k] for k in ("v_old", "new", "v_new"))
-    alg_bytes = (S_K + S_V) * (dv0 + dv1) + S_K * de + 2 * W * H * launches
+    d = roof["tot"]
+    launches = d["frames"]
+    alg_bytes = (S_K + S_V) * (d["v_old"] + d["v_new"]) + S_K * d["new"] + 2 * P * launches
     mu_ms, mu_n = stage["merge_update"]
     achieved = alg_bytes / launches / (mu_ms / mu_n * 1e-3) / 1e9 if mu_n else 0.0
-    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # summary (copy-kernel calibrated).  When it was measured on these very
-    # launches (same warm-up and step count) it is used as is; otherwise its
-    # measured traffic/algorithmic ratio is applied and the note says so.
+    # HBM traffic per launch: the measured traffic / algorithmic ratio of this
+    # kernel on this workload (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel
+    # calibrated) x this run's algorithmic bytes
     traffic, traffic_note = None, None
     pmc_path = os.path.join(ROOT, "profiles", "merge_update_pmc.json")
     if os.path.exists(pmc_path) and launches:
         with open(pmc_path) as fh:
             pmc = json.load(fh)
-        if pmc.get("warmup") == args.warmup and pmc.get("steps") == args.steps:
-            traffic = int(pmc["traffic_bytes_per_launch"])
-            traffic_note = "measured on these launches (%s)" % pmc.get("source")
-        elif pmc.get("traffic_over_alg"):
+        if pmc.get("traffic_over_alg"):
             traffic = int(pmc["traffic_over_alg"] * alg_bytes / launches)
-            traffic_note = "measured traffic/algorithmic ratio %.3f (%s, %s) x this run's algorithmic bytes" % (
-                pmc["traffic_over_alg"], pmc.get("launches"), pmc.get("source"))
+            traffic_note = "measured traffic/algorithmic ratio %.3f (%s) x this run's algorithmic bytes" % (
+                pmc["traffic_over_alg"], pmc.get("source"))
 
-    # the same frames from HBM without meshing: the reference point of the host-frame pass
-    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
-    for f in range(args.warmup):
-        pipe.integrate_device(dframes.address(f * fbytes),
-                              mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for f in range(args.warmup, n):
-        pipe.integrate_device(dframes.address(f * fbytes))
-    ctx.synchronize()
-    fps_no_mesh = args.steps / (time.perf_counter() - t0)
-    pipe.close()
+    # fused-frame algorithmic bytes of the headline pass: integration + ICP + meshes
+    dm, im = main_p["tot"], main_p["icp"]
+    b_int = int_bytes(dm, S_K, S_V, dm["frames"])
+    b_icp = icp_bytes(im["level_iters"], im["frames"])
+    q = head["quality"]
+    b_mesh = args.steps * sum((S_K + S_V) * v + 12 * nv for v, nv in zip(q["volume_at_mesh"], head["qmeshes"]))
+    fused_gbs = (b_int + b_icp + b_mesh) / main_p["elapsed"] / 1e9
 
-    # the same frames handed over from page-locked host memory (PCIe copy inside each step)
-    hframes = ctx.host_empty(frames.shape, np.uint16)
-    hframes[:] = frames
-    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
-    for f in range(args.warmup):
-        pipe.integrate(hframes[f], mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for f in range(args.warmup, n):
-        pipe.integrate(hframes[f])
-    ctx.synchronize()
-    fps_h2d = args.steps / (time.perf_counter() - t0)
-    pipe.close()
-
-    learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
-    precision = None if args.no_precision else precision_section(mf, ctx, cam, args)
-
-    icp_frames = st1["frames"] - st0["frames"]
-    icp_iters = (st1["sum_iters"] - st0["sum_iters"]) / max(icp_frames, 1)
+    poses_gpu = q["poses"]
+    errs = np.array([mf.pose_error(poses_gpu[f], poses[f]) for f in range(TRAJ_FRAMES)])
+    icp_frames = max(im["frames"], 1)
     out = {
         "metric": METRIC,
         "value": round(fps, 3),
@@ -500,7 +740,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "ms_per_step": round(1000.0 * main_p["elapsed"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -508,9 +748,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "room trajectory, 512x424, 4 mm voxels (res 1024, side 4.096 m, trunc 2 cm), "
-                        "3-level point-to-plane ICP 10/5/4, mesh every %d frames, plain init" % args.mesh_every,
-            "frames": n,
-            "timed_frames": "%d..%d" % (args.warmup, n - 1),
+                        "3-level point-to-plane ICP 10/5/4, mesh every 50 frames, plain init; one step = the "
+                        "whole 200-frame trajectory from an empty volume",
+            "frames": TRAJ_FRAMES,
+            "frames_per_step": TRAJ_FRAMES,
+            "meshes_per_step": TRAJ_FRAMES // MESH_EVERY,
             "key_bits": 32,
             "storage": "f32 sum + f32 weight",
         },
@@ -527,20 +769,36 @@ def main():
             "avg_launch_ms": round(mu_ms / mu_n, 5) if mu_n else None,
             "launches": mu_n,
         },
-        "cpu_baseline": cpu,
-        "icp_iters_per_frame": round(icp_iters, 3),
-        "icp_fallback_rate": round((st1["fallbacks"] - st0["fallbacks"]) / max(icp_frames, 1), 4),
-        "pose_error_m_deg": [[f, round(e[0], 5), round(e[1], 4)] for f, e in pose_err] +
-                            [[n - 1, round(final_err[0], 5), round(final_err[1], 4)]],
-        "volume_voxels_final": vol_final,
-        "mesh_vertices": mesh_counts,
-        "stage_ms_per_frame": {k: round(v[0] / max(args.steps, 1), 4) for k, v in stage.items()},
+        "cpu_baseline": cpu["cpu_baseline"] if cpu else None,
+        "fused_frame": {
+            "alg_bytes_per_frame": int((b_int + b_icp + b_mesh) / frames_timed),
+            "alg_bytes_split": {"integrate": int(b_int / frames_timed), "icp": int(b_icp / frames_timed),
+                                "mesh": int(b_mesh / frames_timed)},
+            "GBs": round(fused_gbs, 2),
+            "frac_hbm_peak": round(fused_gbs / HBM_PEAK_GBS, 4),
+            "copy_kernel_GBs": round(copy_gbs, 1),
+            "frac_copy_kernel": round(fused_gbs / copy_gbs, 4),
+        },
+        "icp_iters_per_frame": round(im["sum_iters"] / icp_frames, 3),
+        "icp_level_iters_per_frame": [round(x / icp_frames, 3) for x in im["level_iters"]],
+        "icp_fallback_rate": round(im["fallbacks"] / icp_frames, 4),
+        "pose_error_vs_truth": {
+            "ate_m": round(float(np.sqrt(np.mean(errs[:, 0] ** 2))), 5),
+            "at_mesh_frames_m_deg": [[f, round(float(errs[f, 0]), 5), round(float(errs[f, 1]), 4)]
+                                     for f in range(MESH_EVERY - 1, TRAJ_FRAMES, MESH_EVERY)],
+        },
+        "error_vs_fp64": cpu["error_vs_fp64"] if cpu else None,
+        "volume_voxels_at_mesh": q["volume_at_mesh"],
+        "mesh_vertices": head["qmeshes"],
+        "deterministic": head["qmeshes"] == main_p["meshes"] == roof["meshes"] == diag["meshes"],
+        "stage_ms_per_frame": {k: round(v[0] / frames_timed, 4) for k, v in stage.items()},
         "stage_timing_note": "value from a pass with no events; merge_update from a second pass over the same "
-                             "frames with only its events (%.1f frames/s), the other stages from a third pass "
-                             "with every stage timer on (%.1f frames/s)"
-                             % (args.steps / roof["elapsed"], args.steps / diag["elapsed"]),
-        "fps_with_h2d": round(fps_h2d, 3),
-        "fps_no_mesh": round(fps_no_mesh, 3),
+                             "trajectories with only its events (%.1f frames/s), the other stages from a third "
+                             "pass with every stage timer on (%.1f frames/s)"
+                             % (frames_timed / roof["elapsed"], frames_timed / diag["elapsed"]),
+        "fps_no_mesh": round(frames_timed / head["nomesh"]["elapsed"], 3),
+        "fps_with_h2d_no_mesh": round(frames_timed / head["h2d"]["elapsed"], 3),
+        "config1_plain_8mm": c1,
         "learned_vs_plain": learned,
         "precision_variant_2mm": precision,
     }

@@ -92,7 +92,13 @@ kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out)
 
 /* ---- integration: kifi::Integrator::Integrate (Integrator.h:19-26, Integrator.cpp:26-71) ----
  * depth_mm: host pointer (kifi_integrate) or device pointer already in HBM
- * (kifi_integrate_device).  worldToEye: column storage.  Asynchronous. */
+ * (kifi_integrate_device).  worldToEye: column storage.  Asynchronous on the
+ * device: a host frame is copied into one of two device staging buffers on a
+ * second stream, and the call returns once that copy has read the host buffer,
+ * so the caller may refill or free it at once (the reference's Integrate takes
+ * the frame by const reference and returns when done with it).  The copy of a
+ * frame waits for the kernels of the frame two before it (the same staging
+ * buffer), so the host runs at most two frames ahead of the device. */
 kifi_status kifi_integrate(kifi_volume *vol, const uint16_t *depth_mm, const kifi_intrinsics *cam,
                            const float worldToEye[16]);
 kifi_status kifi_integrate_device(kifi_volume *vol, const uint16_t *depth_mm_dev,
@@ -109,8 +115,9 @@ kifi_status kifi_integrate_device_f32(kifi_volume *vol, const float *depth_m_dev
  * [2] E keys handed to the merge, [3] volume size after the merge.  Synchronous. */
 kifi_status kifi_integrate_stats(kifi_volume *vol, size_t stats[4]);
 /* Running totals since the volume was created: [0] integrations, [1] sum of V0,
- * [2] sum of E, [3] sum of V1, [4] sum of K.  Synchronous. */
-kifi_status kifi_integrate_totals(kifi_volume *vol, uint64_t totals[5]);
+ * [2] sum of E, [3] sum of V1, [4] sum of K, [5] sum of A (voxels whose TSDF the
+ * update changed).  Synchronous. */
+kifi_status kifi_integrate_totals(kifi_volume *vol, uint64_t totals[6]);
 
 /* ---- meshing: kifi::Mesher (Mesher.h:17-18, Mesher.cpp:14-45) ----
  * kifi_mesh: Mesh(volume, verts) -- float3 vertices in generation order.
@@ -186,6 +193,9 @@ typedef struct {
     long long fallbacks;
     float rms_1m;          /* RMS of the point-to-plane residual scaled to 1 m depth   */
                            /* by the sensor noise model (r / d^2 x 1 m^2), metres      */
+    long long sum_level_iters[3]; /* running totals of solves per pyramid level        */
+    double start_pose[16]; /* eye->world start of the last frame's first pass (the     */
+                           /* caller's or the learned guess, else the previous pose)   */
 } kifi_icp_stats;
 
 kifi_status kifi_icp_params_default(kifi_icp_params *p);
@@ -257,6 +267,12 @@ kifi_status kifi_pipeline_integrate_f32(kifi_pipeline *p, const float *depth_m, 
  * pose, e.g. a learned prediction; plain ICP from the previous pose remains the
  * fallback per kifi_icp_params. */
 kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double eyeToWorld[16]);
+/* Back to the state after kifi_pipeline_create, keeping every allocation: the
+ * volume is emptied (capacity kept), the pose is the identity, the next frame
+ * is the first, the caller's guess and the network's frame history are
+ * dropped.  Running totals (kifi_pipeline_stats, kifi_integrate_totals) go on.
+ * Synchronous.  No counterpart in the reference (a new Pipeline there). */
+kifi_status kifi_pipeline_reset(kifi_pipeline *p);
 kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double eyeToWorld[16]);
 kifi_status kifi_pipeline_pose(kifi_pipeline *p, double eyeToWorld[16]);     /* EyeToWorld(), sync */
 kifi_status kifi_pipeline_stats(kifi_pipeline *p, kifi_icp_stats *stats);     /* sync */

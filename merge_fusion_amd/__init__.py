@@ -37,6 +37,7 @@ HIP_EXPORTS = [
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
+    "kifi_pipeline_reset",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_host_alloc", "kifi_host_free", "kifi_memcpy_to_device",
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
     "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
@@ -87,10 +88,16 @@ class IcpStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("iters", ctypes.c_int * 3), ("rms", ctypes.c_float),
                 ("inliers", ctypes.c_int), ("failed", ctypes.c_int), ("fallback", ctypes.c_int),
                 ("sum_iters", ctypes.c_longlong), ("frames", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong),
-                ("rms_1m", ctypes.c_float)]
+                ("rms_1m", ctypes.c_float), ("sum_level_iters", ctypes.c_longlong * 3),
+                ("start_pose", ctypes.c_double * 16)]
 
     def as_dict(self):
-        return {k: (list(getattr(self, k)) if k == "iters" else getattr(self, k)) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if k in ("iters", "sum_level_iters") else (
+                np.array(v, np.float64) if k == "start_pose" else v)
+        return out
 
 
 class Scene(ctypes.Structure):
@@ -162,6 +169,7 @@ def load_hip():
             "kifi_pipeline_integrate": ([vp, vp, i32, f32p], i32),
             "kifi_pipeline_integrate_f32": ([vp, vp, i32, f32p], i32),
             "kifi_pipeline_set_guess": ([vp, f64p], i32),
+            "kifi_pipeline_reset": ([vp], i32),
             "kifi_pipeline_set_pose": ([vp, f64p], i32),
             "kifi_pipeline_pose": ([vp, f64p], i32),
             "kifi_pipeline_stats": ([vp, ctypes.POINTER(IcpStats)], i32),
@@ -490,10 +498,10 @@ class Volume:
         self.ctx.check(fn(self._h, ctypes.c_void_p(depth_ptr), ctypes.byref(cam), _f32p(m)), "kifi_integrate_device")
 
     def totals(self):
-        """Running sums since creation: integrations, V0, E, V1, K."""
-        t = (ctypes.c_uint64 * 5)()
+        """Running sums since creation: integrations, V0, E, V1, K, A (updated voxels)."""
+        t = (ctypes.c_uint64 * 6)()
         self.ctx.check(self.ctx._lib.kifi_integrate_totals(self._h, t), "kifi_integrate_totals")
-        return {"frames": t[0], "v_old": t[1], "new": t[2], "v_new": t[3], "keys": t[4]}
+        return {"frames": t[0], "v_old": t[1], "new": t[2], "v_new": t[3], "keys": t[4], "updated": t[5]}
 
     def stats(self):
         s = (ctypes.c_size_t * 4)()
@@ -691,6 +699,10 @@ class Pipeline:
                        "kifi_pipeline_set_twistnet")
         self._net = net
 
+    def reset(self):
+        """Empty volume, identity pose, next frame is the first; allocations and running totals kept."""
+        self.ctx.check(self.ctx._lib.kifi_pipeline_reset(self._h), "kifi_pipeline_reset")
+
     def set_guess(self, e2w):
         g = np.ascontiguousarray(e2w, np.float64).reshape(16)
         self.ctx.check(self.ctx._lib.kifi_pipeline_set_guess(self._h, _f64p(g)), "kifi_pipeline_set_guess")
@@ -716,7 +728,12 @@ def pose_error(a, b):
     B = np.asarray(b, np.float64).reshape(4, 4).T
     dt = np.linalg.norm(A[:3, 3] - B[:3, 3])
     R = A[:3, :3].T @ B[:3, :3]
-    ang = np.degrees(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1)))
+    # angle from the skew part (sin) and the trace (cos): a rotation that began as
+    # a binary32 matrix is orthonormal only to ~1e-8, which moves the trace of
+    # R (a symmetric error) but not its skew part; arccos of the trace alone
+    # would read that as ~0.01 deg
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    ang = np.degrees(np.arctan2(s, (np.trace(R) - 1) / 2))
     return dt, ang
 
 

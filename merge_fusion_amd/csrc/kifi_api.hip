@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 
 #include "kifi_internal.hpp"
@@ -216,7 +217,7 @@ kifi_status kifi_device_copy(kifi_ctx *ctx, void *dst, const void *src, size_t b
     return KIFI_OK;
 }
 
-kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[5])
+kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[6])
 {
     if (!v || !totals)
         return KIFI_ERR_ARG;
@@ -229,6 +230,7 @@ kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[5])
     totals[2] = c.tot_new;
     totals[3] = c.tot_v_new;
     totals[4] = c.tot_keys;
+    totals[5] = c.tot_upd;
     return KIFI_OK;
 }
 
@@ -492,6 +494,29 @@ kifi_status kifi_volume_save(kifi_volume *v, const char *path)
     return ok ? KIFI_OK : set_error(ctx, KIFI_ERR_ARG, "kifi_volume_save: write failed");
 }
 
+} // extern "C"
+
+// Keys of a loaded file: strictly increasing (the merge relies on it), no bits
+// above the three axes, every axis below res.
+template <typename K>
+static bool valid_keys(const K *k, uint64_t n, int res)
+{
+    constexpr int kb = KeyTraits<K>::kb;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (i > 0 && !(k[i - 1] < k[i]))
+            return false;
+        if (k[i] >> (3 * kb))
+            return false;
+        uint32_t x, y, z;
+        unpack_key<K>(k[i], x, y, z);
+        if (x >= (uint32_t)res || y >= (uint32_t)res || z >= (uint32_t)res)
+            return false;
+    }
+    return true;
+}
+
+extern "C" {
+
 kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out)
 {
     if (!ctx || !path || !out)
@@ -512,26 +537,59 @@ kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out)
     d.res = res;
     d.key_bits = kb;
     d.storage = storage;
-    kifi_volume *v = nullptr;
-    kifi_status st = ok ? kifi_volume_create(ctx, &d, &v) : KIFI_ERR_ARG;
-    if (st == KIFI_OK) {
-        std::vector<char> keys(n * v->key_bytes), vals(n * v->val_bytes);
-        ok = std::fread(keys.data(), 1, keys.size(), f) == keys.size() &&
-             std::fread(vals.data(), 1, vals.size(), f) == vals.size();
-        // keys must be strictly increasing: the merge relies on it
-        for (uint64_t i = 1; ok && i < n; ++i) {
-            if (v->key_bytes == 4)
-                ok = ((const uint32_t *)keys.data())[i - 1] < ((const uint32_t *)keys.data())[i];
-            else
-                ok = ((const uint64_t *)keys.data())[i - 1] < ((const uint64_t *)keys.data())[i];
+    // The header is checked before anything is allocated: a valid description,
+    // n at most res^3 voxels, and exactly n keys and n values left in the file.
+    const char *why = ok ? nullptr : "not a volume file (header)";
+    const int maxres = kb == 32 ? 1024 : (1 << 20);
+    if (!why && ((kb != 32 && kb != 64) || res < 2 || res > maxres || (res & (res - 1)) ||
+                 (storage != KIFI_STORAGE_F32 && storage != KIFI_STORAGE_F16U8) || !(d.side > 0.0f) ||
+                 !(d.trunc > 0.0f)))
+        why = "invalid volume description";
+    const uint64_t rec = (uint64_t)(kb / 8) + (storage == KIFI_STORAGE_F32 ? 8u : 4u);
+    if (!why && n > (uint64_t)res * (uint64_t)res * (uint64_t)res)   // res <= 2^20: res^3 <= 2^60
+        why = "voxel count above res^3";
+    if (!why) {
+        const long here = std::ftell(f);
+        if (here < 0 || std::fseek(f, 0, SEEK_END) != 0)
+            why = "cannot size the file";
+        const long end = why ? -1 : std::ftell(f);
+        if (!why && (end < here || std::fseek(f, here, SEEK_SET) != 0))
+            why = "cannot size the file";
+        if (!why) {
+            const uint64_t left = (uint64_t)(end - here);
+            if (left % rec != 0 || left / rec != n)
+                why = "file size does not match its voxel count";
         }
-        st = ok ? kifi_volume_upload(v, keys.data(), vals.data(), n) : KIFI_ERR_ARG;
+    }
+    if (why) {
+        std::fclose(f);
+        return set_error(ctx, KIFI_ERR_ARG, std::string("kifi_volume_load: ") + why);
+    }
+    kifi_volume *v = nullptr;
+    kifi_status st = kifi_volume_create(ctx, &d, &v);
+    if (st == KIFI_OK) {
+        std::vector<char> keys, vals;
+        try {
+            keys.resize(n * v->key_bytes);
+            vals.resize(n * v->val_bytes);
+        } catch (const std::exception &) {
+            st = set_error(ctx, KIFI_ERR_NOMEM, "kifi_volume_load: host memory for the voxels");
+        }
+        if (st == KIFI_OK) {
+            ok = std::fread(keys.data(), 1, keys.size(), f) == keys.size() &&
+                 std::fread(vals.data(), 1, vals.size(), f) == vals.size();
+            ok = ok && (v->key_bytes == 4 ? valid_keys((const uint32_t *)keys.data(), n, res)
+                                          : valid_keys((const uint64_t *)keys.data(), n, res));
+            st = ok ? kifi_volume_upload(v, keys.data(), vals.data(), n)
+                    : set_error(ctx, KIFI_ERR_ARG, "kifi_volume_load: keys not strictly increasing or "
+                                                   "outside the volume");
+        }
     }
     std::fclose(f);
     if (st != KIFI_OK) {
         if (v)
             kifi_volume_destroy(v);
-        return st == KIFI_ERR_ARG ? set_error(ctx, KIFI_ERR_ARG, "kifi_volume_load: not a valid volume file") : st;
+        return st;
     }
     *out = v;
     return KIFI_OK;
@@ -594,6 +652,9 @@ kifi_status stage_frame(kifi_ctx *ctx, const void *host, size_t bytes, const voi
     KIFI_HIP(ctx, hipMemcpyAsync(ctx->stage[s], host, bytes, hipMemcpyHostToDevice, ctx->copy_stream));
     KIFI_HIP(ctx, hipEventRecord(ctx->stage_copied[s], ctx->copy_stream));
     KIFI_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->stage_copied[s], 0));
+    // the caller may reuse its host buffer when the call returns: wait until the
+    // copy has read it (the kernels of this frame still run asynchronously)
+    KIFI_HIP(ctx, hipEventSynchronize(ctx->stage_copied[s]));
     *dev = ctx->stage[s];
     *slot = s;
     return KIFI_OK;
@@ -676,7 +737,7 @@ kifi_status kifi_mesh(kifi_volume *v, float *verts, size_t vcap, size_t *nverts)
     if (!v || !nverts || (vcap && !verts))
         return KIFI_ERR_ARG;
     kifi_status st = mesh_dispatch(v, false, verts, vcap, nullptr, 0, nverts, nullptr, nullptr);
-    if (st == KIFI_ERR_CAPACITY)
+    if (st == KIFI_ERR_CAPACITY && *nverts <= 0xffffffffull)   // else the 32-bit limit's message stands
         set_error(v->ctx, st, "kifi_mesh: vertex buffer too small");
     return st;
 }
@@ -687,7 +748,7 @@ kifi_status kifi_mesh_indexed(kifi_volume *v, float *verts, size_t vcap, uint32_
     if (!v || !nverts || !nidx || (vcap && !verts) || (icap && !idx))
         return KIFI_ERR_ARG;
     kifi_status st = mesh_dispatch(v, true, verts, vcap, idx, icap, nverts, nidx, stalled);
-    if (st == KIFI_ERR_CAPACITY)
+    if (st == KIFI_ERR_CAPACITY && *nverts <= 0xffffffffull && *nidx <= 0xffffffffull)
         set_error(v->ctx, st, "kifi_mesh_indexed: buffer too small");
     return st;
 }

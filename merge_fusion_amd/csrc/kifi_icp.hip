@@ -1063,6 +1063,8 @@ __global__ void k_icp_frame_end(IcpState *__restrict__ st, Mat4 *__restrict__ w2
     st->sum_iters += st->total_iters;
     st->frames_tracked += 1;
     st->fallbacks += st->fallback;
+    for (int l = 0; l < 3; ++l)
+        st->sum_level_iters[l] += st->iters[l];
     if (w2e) {
         Mat4 m = st->pose_f;
         invert_transform(m.m);
@@ -1274,22 +1276,35 @@ void model_free(kifi_model *m)
     if (m->d_pose) (void)hipFree(m->d_pose);
 }
 
-kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H)
+// Work buffer of a W x H frame: the three pyramid levels, then the block partials.
+size_t icp_work_bytes(int W, int H)
 {
     const size_t P = (size_t)W * H;
-    size_t bytes = al256(P * 4) + al256(P / 4 * 4 + 4) + al256(P / 16 * 4 + 4);
     const size_t nb = (P + kIT - 1) / kIT;
-    bytes += al256(nb * IcpNV<KIFI_ICP_POINT_TO_PLANE>::nv * sizeof(CSum));
-    KIFI_HIP(ctx, hipMalloc(&w->buf, bytes));
-    char *p = (char *)w->buf;
+    return al256(P * 4) + al256(P / 4 * 4 + 4) + al256(P / 16 * 4 + 4) +
+           al256(nb * IcpNV<KIFI_ICP_POINT_TO_PLANE>::nv * sizeof(CSum));
+}
+
+void icp_work_carve(IcpWork *w, void *buf, int W, int H)
+{
+    const size_t P = (size_t)W * H;
+    w->buf = buf;
+    char *p = (char *)buf;
     w->lv[0] = (float *)p; p += al256(P * 4);
     w->lv[1] = (float *)p; p += al256(P / 4 * 4 + 4);
     w->lv[2] = (float *)p; p += al256(P / 16 * 4 + 4);
     w->partials = p;
-    KIFI_HIP(ctx, hipMalloc(&w->d_state, sizeof(IcpState)));
-    KIFI_HIP(ctx, hipMemset(w->d_state, 0, sizeof(IcpState)));
     w->W = W;
     w->H = H;
+}
+
+kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H)
+{
+    void *buf = nullptr;
+    KIFI_HIP(ctx, hipMalloc(&buf, icp_work_bytes(W, H)));
+    icp_work_carve(w, buf, W, H);
+    KIFI_HIP(ctx, hipMalloc(&w->d_state, sizeof(IcpState)));
+    KIFI_HIP(ctx, hipMemset(w->d_state, 0, sizeof(IcpState)));
     return KIFI_OK;
 }
 

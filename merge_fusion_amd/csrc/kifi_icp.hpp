@@ -62,6 +62,7 @@ struct IcpState {
     long long sum_iters;
     long long frames_tracked;
     long long fallbacks;
+    long long sum_level_iters[3];   // solves per pyramid level
 };
 
 struct IcpWork {
@@ -95,6 +96,8 @@ void        model_end(kifi_model *m, kifi_volume *vol, bool splat);
 kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, const kifi_intrinsics *proj_cam);
 kifi_status model_from_depth(kifi_model *m, const uint16_t *d_mm, const float e2w[16], const float w2e[16]);
 kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
+size_t      icp_work_bytes(int W, int H);                        // work buffer bytes (256-aligned parts)
+void        icp_work_carve(IcpWork *w, void *buf, int W, int H); // the levels and partials in buf
 void        icp_free(IcpWork *w);
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,

@@ -31,7 +31,7 @@ __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, 
     if (blockIdx.x != 0 || threadIdx.x != 0)
         return;
     c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
-    c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
+    c->v_new = 0; c->n_tiles = 0; c->overflow = 0; c->n_upd = 0;
     FrameMats f;
     f.w2e = w2e_dev ? *w2e_dev : w2e_val;
     mat_mul(clip.m, f.w2e.m, f.w2c.m);
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B,
     const uint64_t *__restrict__ part_a, const uint32_t *__restrict__ tile_base_dups,
     const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, UpdateParams up,
-    SplatParams sp)
+    SplatParams sp, FrameCounters *__restrict__ upd_cnt)
 {
     __shared__ K sA[kMgTile];
     __shared__ uint32_t h[kHistWords];   // per A position: kept new keys sorting just below it
@@ -785,7 +785,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     // New keys, kThreads per round: a key already in the volume is dropped (A wins
     // ties; the A element just before the tile can match the first new key).  A
     // kept key goes to obase + (kept keys before it) + (A keys below it).
-    uint32_t lbq[PER], keepm = 0, nbk = 0;
+    uint32_t lbq[PER], keepm = 0, nbk = 0, nupd = 0;   // nupd: voxels this lane updated (A)
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         lbq[q] = 0;
@@ -807,8 +807,10 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
                 const uint32_t ol = pos + lbq[q];
                 V val{};
                 float s;
-                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, s))
+                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, s)) {
                     voxel_apply(val, s);
+                    ++nupd;
+                }
                 Ot[ol] = bk[q];
                 Ovt[ol] = val;
                 if (SPLAT)
@@ -869,8 +871,10 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             const uint32_t ol = i + (nbk ? h[hpad(i)] : 0u);
             V val = av[q];
             float s;
-            if (pix[q] >= 0 && voxel_update<K>(k, dmm[q], up, w2c, s))
+            if (pix[q] >= 0 && voxel_update<K>(k, dmm[q], up, w2c, s)) {
                 voxel_apply(val, s);
+                ++nupd;
+            }
             Ot[ol] = k;
             Ovt[ol] = val;
             if (SPLAT)
@@ -900,6 +904,12 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             const uint32_t e = h[t];
             splat_point<K>(sA[e & 0xffffu], obase + (e >> 16), sp, w2e);
         }
+    }
+    // A, the frame's updated voxels: one atomic per tile
+    nupd = block_reduce_sum<kThreads>(nupd, tmp);
+    if (threadIdx.x == 0 && nupd) {
+        atomicAdd(&upd_cnt->n_upd, nupd);
+        atomicAdd((unsigned long long *)&upd_cnt->tot_upd, (unsigned long long)nupd);
     }
 }
 
@@ -1048,11 +1058,11 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     if (splat)
         hipLaunchKernelGGL((k_merge_update<K, V, true>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                            (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
-                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, *splat);
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, *splat, cnt);
     else
         hipLaunchKernelGGL((k_merge_update<K, V, false>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                            (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
-                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, SplatParams{});
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, SplatParams{}, cnt);
     timer_end(ctx, KIFI_STAGE_MERGE_UPDATE, t_mu);
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;

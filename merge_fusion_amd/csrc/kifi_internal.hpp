@@ -179,8 +179,10 @@ struct FrameCounters {
     uint64_t v_new;       // V1
     uint32_t n_tiles;     // merge tiles this frame
     uint32_t overflow;    // set when a device-side capacity check fails
+    uint32_t n_upd;       // A: voxels whose TSDF the update changed
+    uint32_t pad_;
     // running totals over all integrations (never reset by a frame)
-    uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys;
+    uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys, tot_upd;
 };
 
 } // namespace kifi
@@ -214,7 +216,8 @@ struct kifi_ctx {
 
 namespace kifi {
 // Uploads a host frame of `bytes` into the next staging buffer on the copy stream
-// and makes the context's stream wait for it; *dev receives the device copy.
+// and makes the context's stream wait for it; returns once the copy has read the
+// host buffer.  *dev receives the device copy.
 // stage_done(ctx, slot) after the frame's last kernel releases the buffer.
 kifi_status stage_frame(kifi_ctx *ctx, const void *host, size_t bytes, const void **dev, int *slot);
 void        stage_done(kifi_ctx *ctx, int slot);

@@ -550,6 +550,10 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     *nverts = h_tot[0];
     if (nidx)
         *nidx = tris ? h_tot[1] : 0;
+    // per-voxel output offsets and the indices themselves are 32-bit
+    if (h_tot[0] > 0xffffffffull || (tris && h_tot[1] > 0xffffffffull))
+        return set_error(ctx, KIFI_ERR_CAPACITY, "mesh: more than 2^32 - 1 vertices or indices (32-bit offsets "
+                                                 "and indices)");
     if (h_tot[0] > vcap || (tris && h_tot[1] > icap))
         return KIFI_ERR_CAPACITY;
 

@@ -155,6 +155,10 @@ static void fill_stats(const IcpState &s, kifi_icp_stats *st)
     st->sum_iters = s.sum_iters;
     st->frames = s.frames_tracked;
     st->fallbacks = s.fallbacks;
+    for (int l = 0; l < 3; ++l)
+        st->sum_level_iters[l] = s.sum_level_iters[l];
+    for (int i = 0; i < 16; ++i)
+        st->start_pose[i] = s.guess[i];
 }
 
 // Host frame -> one ICP run from `guess`; the final device state in *hs.
@@ -427,6 +431,23 @@ kifi_status kifi_pipeline_set_guess(kifi_pipeline *p, const double e2w[16])
     return KIFI_OK;
 }
 
+kifi_status kifi_pipeline_reset(kifi_pipeline *p)
+{
+    if (!p)
+        return KIFI_ERR_ARG;
+    kifi_status s = kifi_volume_clear(p->vol);   // synchronises the stream first
+    if (s != KIFI_OK)
+        return s;
+    const double I[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+    icp_set_pose(&p->work, I, nullptr, p->ctx->stream);
+    KIFI_HIP(p->ctx, hipGetLastError());
+    p->frame = 0;
+    p->have_guess = false;
+    if (p->net)
+        twistnet_reset_frames(p->net);
+    return KIFI_OK;
+}
+
 kifi_status kifi_pipeline_set_twistnet(kifi_pipeline *p, kifi_twistnet *net)
 {
     if (!p)
@@ -476,6 +497,10 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
     } release{ ctx, slot };
     const uint16_t *d_depth = (const uint16_t *)din.p;
     kifi_twistnet *net = metres ? nullptr : p->net;
+    // the network predicts the motion between consecutive mm frames: a metre
+    // frame breaks that sequence, so the next mm frame starts a new one
+    if (metres && p->net)
+        twistnet_reset_frames(p->net);
     kifi_status s = KIFI_OK;
     bool tracked = false;     // icp_run wrote this frame's worldToEye
     if (w2e) {
@@ -498,6 +523,10 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
             ip.fallback_rms = 0.0f;   // the guess is the previous pose: nothing to fall back to
             ip.fallback_iters = 0;
         }
+        // a frame started from the caller's guess still becomes the network's
+        // previous frame: the next prediction is then the motion from this one
+        if (p->have_guess && net)
+            twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
         s = icp_run(p->model, &p->work, din, &p->cam, p->have_guess ? p->guess : nullptr,
                     p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e);
         tracked = true;
