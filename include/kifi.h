@@ -212,6 +212,19 @@ kifi_status kifi_model_download(kifi_model *m, float *points, float *normals);
 kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
                            const double guess_eyeToWorld[16], const kifi_icp_params *params,
                            double out_eyeToWorld[16], kifi_icp_stats *stats);
+/* kifi_icp_align with the frame already in device memory, allocating nothing:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its work memory (depth pyramid, block partials, tracking
+ * state) is a fixed 2 MiB array in the library, one per GPU, held until the
+ * process ends; frames above 640 x 480 pixels return KIFI_ERR_CAPACITY.  The
+ * result equals kifi_icp_align's for the same inputs.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from two contexts or
+ * threads) race; calls in sequence are independent (the state is cleared each
+ * call).  Stage timers are off during the call.  Synchronous. */
+kifi_status kifi_icp_align_device_noalloc(kifi_model *m, const uint16_t *depth_mm_dev, const kifi_intrinsics *cam,
+                                         const double guess_eyeToWorld[16], const kifi_icp_params *params,
+                                         double out_eyeToWorld[16], kifi_icp_stats *stats);
 /* ICP::Align itself (ICP.h:18-27, ICP.cpp:9-265): `iterations` window-search
  * Horn steps (the reference runs 4) of the raw frame (host, mm) against a host
  * point buffer synth_xyz (synth_width x synth_height float3 world points, all

@@ -37,7 +37,7 @@ HIP_EXPORTS = [
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
     "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
-    "kifi_pipeline_reset",
+    "kifi_pipeline_reset", "kifi_icp_align_device_noalloc",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_host_alloc", "kifi_host_free", "kifi_memcpy_to_device",
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
     "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
@@ -160,6 +160,8 @@ def load_hip():
                                 ctypes.POINTER(IcpStats)], i32),
             "kifi_icp_normal_equations": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, i32, ctypes.c_float, i32,
                                            f64p], i32),
+            "kifi_icp_align_device_noalloc": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, ctypes.POINTER(IcpParams),
+                                               f64p, ctypes.POINTER(IcpStats)], i32),
             "kifi_horn_align": ([vp, f32p, f32p, sz, f32p], i32),
             "kifi_pipeline_create": ([vp, ctypes.POINTER(VolumeDesc), ctypes.POINTER(Intrinsics),
                                       ctypes.POINTER(IcpParams), ctypes.POINTER(vp)], i32),
@@ -593,6 +595,19 @@ class Model:
         self.ctx.check(self.ctx._lib.kifi_icp_align(self._h, d.ctypes.data, ctypes.byref(self.cam), _f64p(g),
                                                     ctypes.byref(p), _f64p(out), ctypes.byref(st)),
                        "kifi_icp_align")
+        return out, st.as_dict()
+
+    def icp_align_device_noalloc(self, depth_ptr, guess_e2w, params=None):
+        """kifi_icp_align_device_noalloc: the same alignment of a frame already in
+        device memory (uint16 mm at depth_ptr), allocating nothing."""
+        p = params if params is not None else IcpParams.default()
+        g = np.ascontiguousarray(guess_e2w, np.float64).reshape(16)
+        out = np.zeros(16, np.float64)
+        st = IcpStats()
+        self.ctx.check(self.ctx._lib.kifi_icp_align_device_noalloc(self._h, ctypes.c_void_p(depth_ptr),
+                                                                   ctypes.byref(self.cam), _f64p(g), ctypes.byref(p),
+                                                                   _f64p(out), ctypes.byref(st)),
+                       "kifi_icp_align_device_noalloc")
         return out, st.as_dict()
 
     def normal_equations(self, depth_mm, e2w, level=0, dist_thresh=0.1, naive=False):

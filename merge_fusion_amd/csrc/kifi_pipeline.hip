@@ -1,6 +1,7 @@
 // kifi_pipeline.hip -- C ABI of model prediction, ICP, Horn and the frame loop.
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "kifi_icp.hpp"
@@ -17,6 +18,58 @@ void twistnet_reset_frames(kifi_twistnet *n);
 } // namespace kifi
 
 using namespace kifi;
+
+// ---- allocation-free ICP: its work memory is this file-scope array ----
+// One copy per GPU, held until the process ends.  In host code `scratch` is a
+// host-side stand-in for the symbol, never a device address: its device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current, and only that address is used afterwards.
+namespace {
+constexpr size_t kNoAllocBytes = size_t(2) << 20;              // 2 MiB
+constexpr size_t kNoAllocMaxPixels = size_t(640) * 480;        // frames up to 640 x 480
+constexpr int kNoAllocDevices = 64;
+struct NoAllocSlot {
+    std::once_flag once;
+    void          *ptr = nullptr;
+    hipError_t     err = hipSuccess;
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The device address of this device's `scratch` (looked up on the first call).
+static kifi_status noalloc_scratch(kifi_ctx *ctx, char **out)
+{
+    if (ctx->device < 0 || ctx->device >= kNoAllocDevices)
+        return set_error(ctx, KIFI_ERR_ARG, "allocation-free entry point: device id above 63");
+    NoAllocSlot &slot = g_noalloc[ctx->device];
+    std::call_once(slot.once, [&] {
+        int prev = -1;
+        hipError_t e = hipGetDevice(&prev);
+        if (e == hipSuccess)
+            e = hipSetDevice(ctx->device);
+        void *p = nullptr;
+        if (e == hipSuccess)
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+        slot.err = e;
+        slot.ptr = e == hipSuccess ? p : nullptr;
+    });
+    if (slot.err != hipSuccess || !slot.ptr)
+        return hip_error(ctx, slot.err != hipSuccess ? slot.err : hipErrorInvalidSymbol,
+                         "hipGetSymbolAddress(scratch)");
+    *out = (char *)slot.ptr;
+    return KIFI_OK;
+}
+
+// Fill kernel (no memset node): the tracking state starts at zero on every call.
+__global__ void k_zero_words(uint32_t *__restrict__ p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
 
 struct kifi_pipeline {
     kifi_ctx        *ctx = nullptr;
@@ -218,6 +271,55 @@ kifi_status kifi_icp_align(kifi_model *m, const uint16_t *depth_mm, const kifi_i
             fill_stats(hs, stats);
     }
     return s;
+}
+
+kifi_status kifi_icp_align_device_noalloc(kifi_model *m, const uint16_t *depth_mm_dev, const kifi_intrinsics *cam,
+                                         const double guess[16], const kifi_icp_params *params, double out[16],
+                                         kifi_icp_stats *stats)
+{
+    if (!m || !depth_mm_dev || !guess || !params || !out)
+        return KIFI_ERR_ARG;
+    kifi_ctx *ctx = m->ctx;
+    kifi_status s = bad_cam(ctx, cam);
+    if (s != KIFI_OK)
+        return s;
+    if (cam->width != m->cam.width || cam->height != m->cam.height)
+        return set_error(ctx, KIFI_ERR_ARG, "kifi_icp_align_device_noalloc: frame size differs from the model's");
+    const size_t P = (size_t)cam->width * cam->height;
+    const size_t work = icp_work_bytes(cam->width, cam->height);
+    const size_t state_off = (work + 255) & ~size_t(255);
+    if (P > kNoAllocMaxPixels || state_off + sizeof(IcpState) > kNoAllocBytes)
+        return set_error(ctx, KIFI_ERR_CAPACITY, "kifi_icp_align_device_noalloc: frames above 640 x 480 pixels "
+                                                 "do not fit its 2 MiB work array");
+    char *base = nullptr;
+    s = noalloc_scratch(ctx, &base);
+    if (s != KIFI_OK)
+        return s;
+    hipStream_t st = ctx->stream;
+    IcpWork w;
+    icp_work_carve(&w, base, cam->width, cam->height);
+    w.d_state = (IcpState *)(base + state_off);
+    // stage timers create their events on demand: off for this call
+    const unsigned mask = ctx->timing_mask;
+    ctx->timing_mask = 0;
+    static_assert(sizeof(IcpState) % 4 == 0, "whole words");
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, (uint32_t *)w.d_state, sizeof(IcpState) / 4);
+    icp_set_pose(&w, guess, nullptr, st);   // previous pose for a fallback = the guess
+    s = icp_run(m, &w, depth_mm_dev, cam, guess, nullptr, params);
+    ctx->timing_mask = mask;
+    if (s != KIFI_OK)
+        return s;
+    IcpState hs;
+    KIFI_HIP(ctx, hipMemcpyAsync(&hs, w.d_state, sizeof hs, hipMemcpyDeviceToHost, st));
+    KIFI_HIP(ctx, hipStreamSynchronize(st));
+    s = state_ok(ctx, hs);
+    if (s != KIFI_OK)
+        return s;
+    for (int i = 0; i < 16; ++i)
+        out[i] = hs.pose[i];
+    if (stats)
+        fill_stats(hs, stats);
+    return KIFI_OK;
 }
 
 kifi_status kifi_icp_normal_equations(kifi_model *m, const uint16_t *depth_mm, const kifi_intrinsics *cam,
