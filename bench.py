@@ -27,7 +27,7 @@ the 1000-frame fast trajectory (network trained here on this GPU); and the
 2 mm / 64-bit-key precision variant at about 50 M voxels.
 """
 import argparse
-import ctypes
+import csv
 import json
 import os
 import platform
@@ -67,35 +67,10 @@ def parse():
     ap.add_argument("--precision-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain copy kernel of this size 20 times (HBM counter calibration)")
+    ap.add_argument("--trace-window", metavar="KERNEL_TRACE_CSV",
+                    help="no GPU work: from a rocprofv3 kernel trace of a run with the same --steps and --warmup, "
+                         "the roofline kernel's launches of that run's roofline pass, timed window only")
     return ap.parse_args()
-
-
-class Roctx:
-    """roctxProfilerPause/Resume when the ROCTx library is present (no-ops
-    otherwise and when no profiler is attached).  Under `rocprofv3
-    --selected-regions` only the resumed region is traced: the timed window of
-    the roofline pass, so a kernel trace of this very command gives the
-    dominant kernel's launches of that window and nothing else."""
-
-    def __init__(self):
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so", "libroctx64.so.4"):
-            try:
-                lib = ctypes.CDLL(name)
-                lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-                lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-                self.lib = lib
-                break
-            except (OSError, AttributeError):
-                continue
-
-    def pause(self):
-        if self.lib:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib:
-            self.lib.roctxProfilerResume(0)
 
 
 def render_frames(mf, cam, n):
@@ -200,7 +175,7 @@ def icp_bytes(level_iters, frames):
             frames * (2 * P + sum(24 * p for p in LEVEL_PIXELS)))
 
 
-def headline(mf, ctx, cam, args, frames, dframes, poses, roctx):
+def headline(mf, ctx, cam, args, frames, dframes, poses):
     """Config-2 trajectories.  Passes: the headline (no events), the roofline
     kernel's launch durations (only its events; the profiler window), every
     stage timer, no meshing, and host frames (PCIe copy in each frame)."""
@@ -227,7 +202,7 @@ def headline(mf, ctx, cam, args, frames, dframes, poses, roctx):
                 meshes.append(int(nv))
         return meshes
 
-    def run_pass(stages, mesh=True, host=None, window=False):
+    def run_pass(stages, mesh=True, host=None):
         pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC)
         for _ in range(args.warmup):
             trajectory(pipe, mesh, host)
@@ -235,16 +210,12 @@ def headline(mf, ctx, cam, args, frames, dframes, poses, roctx):
         st0, tot0 = pipe.stats(), pipe.volume.totals()
         if stages != []:
             ctx.timing(True, stages)
-        if window:
-            roctx.resume()
         t0 = time.perf_counter()
         meshes = []
         for _ in range(args.steps):
             meshes = trajectory(pipe, mesh, host)
         ctx.synchronize()
         elapsed = time.perf_counter() - t0
-        if window:
-            roctx.pause()
         ctx.timing(False)
         st1, tot1 = pipe.stats(), pipe.volume.totals()
         stage = {name: ctx.timing_read(s) for name, s in (("merge_update", mf.STAGE_MERGE_UPDATE),
@@ -258,7 +229,7 @@ def headline(mf, ctx, cam, args, frames, dframes, poses, roctx):
         return dict(elapsed=elapsed, tot=d, icp=icp, stage=stage, meshes=meshes)
 
     main = run_pass([])
-    roof = run_pass([mf.STAGE_MERGE_UPDATE], window=True)
+    roof = run_pass([mf.STAGE_MERGE_UPDATE])
     diag = run_pass(None)
     nomesh = run_pass([], mesh=False)
     hframes = ctx.host_empty((TRAJ_FRAMES, H, W), np.uint16)
@@ -529,6 +500,32 @@ def precision_section(mf, ctx, cam, args):
     return out
 
 
+ROOF_KERNEL = "k_merge_update<unsigned int, kifi::VoxF32, true>"
+
+
+def trace_window(args):
+    """The headline's roofline-kernel launches in a rocprofv3 kernel trace of a
+    bench run with the same --steps and --warmup.  Every frame of the headline
+    passes launches the roofline kernel's tracked binary32 instance once, and
+    nothing before them does: pass 1 (no events) is launches [0, n), pass 2
+    (the roofline pass) is [n, 2n) with n = 200 (warmup + steps), and its timed
+    window is the last 200 x steps of those.  Prints their count and mean
+    duration, the figure the bench's avg_launch_ms measures with HIP events."""
+    rows = []
+    with open(args.trace_window, newline="") as fh:
+        for r in csv.DictReader(fh):
+            if r["Kernel_Name"].startswith("void kifi::" + ROOF_KERNEL) or r["Kernel_Name"].startswith(ROOF_KERNEL):
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    rows.sort()
+    n = TRAJ_FRAMES * (args.warmup + args.steps)
+    win = rows[n + TRAJ_FRAMES * args.warmup: 2 * n]
+    dur = np.array([e - s for s, e in win], np.float64) * 1e-6
+    print(json.dumps({"kernel": ROOF_KERNEL, "launches_in_trace": len(rows), "window_launches": len(win),
+                      "window": "[%d, %d)" % (n + TRAJ_FRAMES * args.warmup, 2 * n),
+                      "mean_ms": round(float(dur.mean()), 5) if len(dur) else None,
+                      "median_ms": round(float(np.median(dur)), 5) if len(dur) else None}))
+
+
 def calibrate_copy(mf, mib):
     ctx = mf.Context(0)
     nbytes = mib << 20
@@ -676,8 +673,9 @@ def main():
         print("bench.py: the fusion path runs on one GPU (it is not sharded across GPUs); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    roctx = Roctx()
-    roctx.pause()      # a profiler attached with --selected-regions records only the roofline window
+    if args.trace_window:
+        trace_window(args)
+        return
     import merge_fusion_amd as mf
     if args.calibrate_copy_mib:
         calibrate_copy(mf, args.calibrate_copy_mib)
@@ -687,7 +685,7 @@ def main():
 
     ctx = mf.Context(0)
     dframes = ctx.device_upload(frames)
-    head = headline(mf, ctx, cam, args, frames, dframes, poses, roctx)
+    head = headline(mf, ctx, cam, args, frames, dframes, poses)
     copy_gbs = copy_rate_gbs(mf, ctx)
     c1 = config1_section(mf, ctx, cam)
     learned = None if args.no_learned else learned_section(mf, ctx, cam, args)

@@ -18,7 +18,7 @@ namespace kifi {
 constexpr int kThreads = 256;
 constexpr int kRsItems = 8;
 constexpr int kRsTile = kThreads * kRsItems;   // 2048 keys per radix tile
-constexpr int kMgTile = 4096;                  // merged positions per merge tile
+constexpr int kMgTile = 2048;                  // merged positions per merge tile
 
 // ===================================================================== reset
 // Frame start: per-frame counters and radix totals cleared (a fill kernel, never
