@@ -21,10 +21,36 @@ constexpr int kRsTile = kThreads * kRsItems;   // 2048 keys per radix tile
 constexpr int kMgTile = 2048;                  // merged positions per merge tile
 
 // ===================================================================== reset
+// The update's cone pre-test (ClipPre).  Row r of clip space at the centre of
+// voxel X is sum_j m_rj (X_j vl + c0) + m_r3.  The reference evaluates it in
+// binary32 in source order; the pre-test as three fused multiply-adds of
+// a_j = m_rj vl and a_3 = m_r3 + sum_j m_rj c0.  With u = 2^-24,
+// S = sum_j |m_rj| C + |m_r3| and C = side + |c0| bounding every centre, each
+// lies within about 8 u S of the exact value, so within 16 u S of the other;
+// eps = 64 u S, maximised over the three rows, keeps a margin of 4.
+__device__ void clip_pre(const Mat4 &w2c, const VolumeConsts &vc, ClipPre &p)
+{
+    const float C = (float)vc.res * vc.vl + fabsf(vc.vlHalfPlusMin);
+    const int rows[3] = { 0, 1, 3 };
+    float *dst[3] = { p.x, p.y, p.w };
+    float smax = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const int r = rows[k];
+        const float m0 = w2c.m[r], m1 = w2c.m[r + 4], m2 = w2c.m[r + 8], m3 = w2c.m[r + 12];
+        dst[k][0] = m0 * vc.vl;
+        dst[k][1] = m1 * vc.vl;
+        dst[k][2] = m2 * vc.vl;
+        dst[k][3] = ((m0 * vc.vlHalfPlusMin + m1 * vc.vlHalfPlusMin) + m2 * vc.vlHalfPlusMin) + m3;
+        const float S = (fabsf(m0) + fabsf(m1) + fabsf(m2)) * C + fabsf(m3);
+        smax = fmaxf(smax, S);
+    }
+    p.eps = 64.0f * 0x1p-24f * smax;
+}
+
 // Frame start: per-frame counters and radix totals cleared (a fill kernel, never
 // a memset node), and the frame's matrices from worldToEye (Integrator.cpp:41-42).
 __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, Mat4 clip, Mat4 w2e_val,
-                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out)
+                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
         totals[i] = 0;
@@ -37,6 +63,7 @@ __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, 
     mat_mul(clip.m, f.w2e.m, f.w2c.m);
     f.e2w = f.w2e;
     invert_transform(f.e2w.m);
+    clip_pre(f.w2c, vc, f.pre);
     *out = f;
 }
 
@@ -557,31 +584,51 @@ __device__ __forceinline__ float voxel_centre(uint32_t c, const UpdateParams &p)
     return (float)c * p.vl + p.c0;                                 // Volume.cpp:75-78
 }
 
+// Outside the update's cone for sure: the pre-test's |x| or |y| exceeds
+// 1.02 |w| + eps, so the reference-order binary32 values have |x/w| or |y/w|
+// above 1.01 (their distance from the pre-test is at most eps / 2), which lands
+// outside [0,W) x [0,H) whatever the rounding.  NaN is never outside.
 template <typename K>
-__device__ __forceinline__ int voxel_pixel(K key, const UpdateParams &p, const Mat4 &w2c)
+__device__ __forceinline__ bool voxel_outside(K key, const ClipPre &c)
 {
+    uint32_t x, y, z;
+    unpack_key<K>(key, x, y, z);
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    const float px = fmaf(c.x[0], fx, fmaf(c.x[1], fy, fmaf(c.x[2], fz, c.x[3])));
+    const float py = fmaf(c.y[0], fx, fmaf(c.y[1], fy, fmaf(c.y[2], fz, c.y[3])));
+    const float pw = fmaf(c.w[0], fx, fmaf(c.w[1], fy, fmaf(c.w[2], fz, c.w[3])));
+    const float lim = fmaf(1.02f, fabsf(pw), c.eps);
+    return fabsf(px) > lim || fabsf(py) > lim;
+}
+
+template <typename K>
+__device__ __forceinline__ int voxel_pixel(K key, const UpdateParams &p, const Mat4 &w2c, const ClipPre &pre)
+{
+    if (voxel_outside<K>(key, pre))   // most of the volume: no projection in the reference's order
+        return -1;
     uint32_t x, y, z;
     unpack_key<K>(key, x, y, z);
     float k[4] = { voxel_centre<K>(x, p), voxel_centre<K>(y, p), voxel_centre<K>(z, p), 1.0f };
     mat_vec(w2c, k, k);                                            // :290-293
     const float kw = k[3];
-    // |x/w| or |y/w| above 1.01 lands outside [0,W) x [0,H) whatever the rounding,
-    // so such voxels (most of the volume) skip the divisions; NaN passes on
-    const float lim = 1.01f * fabsf(kw);
-    if (fabsf(k[0]) > lim || fabsf(k[1]) > lim)
-        return -1;
     // :305-315 divides all four lanes by w and keeps the voxel only if
     // (z/w)*0+0 and (w/w)*0+0 are in range, i.e. z/w is finite and w is finite
     // and nonzero.  With w finite and nonzero, RN(z/w) is finite exactly when
     // |z| < |w| (2^128 - 2^103) (the rounding boundary above FLT_MAX; a tie
-    // rounds to the even infinity); the product is exact in double (24 x 25
-    // bits), so the test needs no division.  NaN z fails the comparison.
+    // rounds to the even infinity).  Binary32, no division in the common case:
+    // |z| < |w| 2^127 is finite (the scaling by a power of two is exact, or
+    // overflows to infinity, which only accepts what is finite anyway);
+    // |z| >= |w| 2^128 is not; between the two the division decides.  NaN z
+    // fails every comparison and the division's finiteness.
     const bool wok = kw != 0.0f && fabsf(kw) <= FLT_MAX;
-    const bool zok = (double)fabsf(k[2]) < (double)fabsf(kw) * 0x1.ffffffp127;
+    const float za = fabsf(k[2]), wa = fabsf(kw);
+    bool zok = za < wa * 0x1p127f;
+    if (!zok && !(za >= wa * 0x1p127f * 2.0f))   // |w| 2^128, or infinity when that overflows
+        zok = __builtin_isfinite(k[2] / kw);
     float u = k[0] / kw, v = k[1] / kw;
     u = u * p.ndcU + p.ndcU;                                       // :310-313
     v = v * p.ndcV + p.ndcV;
-    if (u >= 0.0f && u < p.maxU && v >= 0.0f && v < p.maxV && zok && wok)
+    if ((u >= 0.0f) & (u < p.maxU) & (v >= 0.0f) & (v < p.maxV) & zok & wok)
         return (int)((p.H - (unsigned)v - 1) * (unsigned)p.W + (unsigned)u);   // :319
     return -1;
 }
@@ -606,27 +653,33 @@ __device__ __forceinline__ bool voxel_update(K key, float d, const UpdateParams 
 }
 
 template <typename K>
-__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const Mat4 &w2c, float &sd_out)
+__device__ __forceinline__ bool voxel_sample(K key, const UpdateParams &p, const Mat4 &w2c, const ClipPre &pre,
+                                             float &sd_out)
 {
-    const int pix = voxel_pixel<K>(key, p, w2c);
+    const int pix = voxel_pixel<K>(key, p, w2c, pre);
     return pix >= 0 && voxel_update<K>(key, p.depth[pix], p, w2c, sd_out);
 }
 
 struct VoxF32 { float sum, weight; };
 
-__device__ __forceinline__ void voxel_apply(VoxF32 &v, float s)
+__device__ __forceinline__ void voxel_apply(VoxF32 &v, float s, const float *)
 {
     v.sum += s;                                                    // Voxel.h:53-57
     v.weight++;
 }
 
-// binary16 distance + uint8 weight; binary32 update d' = (d*w + s)/(w + 1)
-__device__ __forceinline__ void voxel_apply(uint32_t &v, float s)
+// binary16 distance + uint8 weight; binary32 update d' = (d*w + s) * R[w] with
+// R[w] = 1/(w + 1) correctly rounded (the table `recip`, in LDS)
+__device__ __forceinline__ void voxel_apply(uint32_t &v, float s, const float *recip)
 {
     const uint32_t w = (v >> 16) & 0xffu;
     const float d = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
     const float wf = (float)w;
-    const float dn = (d * wf + s) / (wf + 1.0f);
+    float dn = (d * wf + s) * recip[w];
+    // the product is rounded to binary32, then to binary16: without this the
+    // compiler fuses multiply and conversion into one mixed-precision FMA,
+    // which rounds once, straight to binary16
+    asm volatile("" : "+v"(dn));
     const uint32_t wn = w < 255u ? w + 1u : 255u;
     v = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)dn) | (wn << 16);
 }
@@ -672,22 +725,33 @@ __device__ __forceinline__ bool splat_candidate(const V &val, const SplatParams 
     return vox_weight(val) > 0.0f && near_surface(val, sp.dmax);
 }
 
-// Splat of a candidate voxel with merged index o into the model z-buffer.
+// The voxel's eye coordinates and whether it can reach the splat's image: in
+// front of the near plane and not rejected by the multiply-only test well
+// outside the image (2 % + 1 px slack, rounding-safe).
 template <typename K>
-__device__ __forceinline__ void splat_point(K key, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+__device__ __forceinline__ bool splat_visible(K key, const SplatParams &sp, const Mat4 &w2e, float &z, float &fxe,
+                                              float &fye)
 {
     uint32_t x, y, zz;
     unpack_key<K>(key, x, y, zz);
     const float p[4] = { (float)x * sp.vl + sp.c0, (float)y * sp.vl + sp.c0, (float)zz * sp.vl + sp.c0, 1.0f };
     float e[4];
     mat_vec(w2e, p, e);
-    const float z = -e[2];
+    z = -e[2];
     if (!(z > sp.znear))
-        return;
-    // multiply-only rejection well outside the image (2 % + 1 px slack, rounding-safe)
-    const float fxe = sp.fx * e[0], fye = sp.fy * e[1];
-    if (fxe > ((float)sp.W + 8.0f - sp.cx) * z * 1.02f + 1.0f || fxe < (-8.0f - sp.cx) * z * 1.02f - 1.0f ||
-        -fye > ((float)sp.H + 8.0f - sp.cy) * z * 1.02f + 1.0f || -fye < (-8.0f - sp.cy) * z * 1.02f - 1.0f)
+        return false;
+    fxe = sp.fx * e[0];
+    fye = sp.fy * e[1];
+    return !(fxe > ((float)sp.W + 8.0f - sp.cx) * z * 1.02f + 1.0f || fxe < (-8.0f - sp.cx) * z * 1.02f - 1.0f ||
+             -fye > ((float)sp.H + 8.0f - sp.cy) * z * 1.02f + 1.0f || -fye < (-8.0f - sp.cy) * z * 1.02f - 1.0f);
+}
+
+// Splat of a candidate voxel with merged index o into the model z-buffer.
+template <typename K>
+__device__ __forceinline__ void splat_point(K key, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
+{
+    float z, fxe, fye;
+    if (!splat_visible<K>(key, sp, w2e, z, fxe, fye))
         return;
     const float u = fxe / z + sp.cx;
     const float v = sp.cy - fye / z;
@@ -732,13 +796,18 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     __shared__ K sA[kMgTile];
     __shared__ uint32_t h[kHistWords];   // per A position: kept new keys sorting just below it
     __shared__ uint32_t tmp[kThreads / kWave + 1];
+    constexpr bool kF16 = sizeof(V) == 4;
+    __shared__ float recip[kF16 ? 256 : 1];   // binary16 storage: R[w] = 1/(w + 1)
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
         return;
+    if (kF16)
+        recip[threadIdx.x] = 1.0f / (float)(threadIdx.x + 1);   // correctly rounded; visible after the barrier below
     // the frame matrix into registers once (the output stores would otherwise
     // force a reload per voxel); the splat's matrix is loaded where it is used,
     // so the two are not live together
     const Mat4 w2c = up.mats->w2c;
+    const ClipPre pre = up.mats->pre;
     const uint64_t tot = cnt->v_old + cnt->n_new;
     const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
     const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
@@ -807,8 +876,8 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
                 const uint32_t ol = pos + lbq[q];
                 V val{};
                 float s;
-                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, s)) {
-                    voxel_apply(val, s);
+                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, pre, s)) {
+                    voxel_apply(val, s, recip);
                     ++nupd;
                 }
                 Ot[ol] = bk[q];
@@ -854,7 +923,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
         const uint32_t i = q * kThreads + threadIdx.x;
         pix[q] = -1;
         if (i < mt.na && i + (nbk ? h[hpad(i)] : 0u) < ulim)
-            pix[q] = voxel_pixel<K>(sA[i], up, w2c);
+            pix[q] = voxel_pixel<K>(sA[i], up, w2c, pre);
     }
     // unconditional loads (pixel 0 stands in for "none"): a load under a branch
     // would be waited for before its value merges, one gather at a time
@@ -872,7 +941,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             V val = av[q];
             float s;
             if (pix[q] >= 0 && voxel_update<K>(k, dmm[q], up, w2c, s)) {
-                voxel_apply(val, s);
+                voxel_apply(val, s, recip);
                 ++nupd;
             }
             Ot[ol] = k;
@@ -992,7 +1061,7 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     for (int i = 0; i < 16; ++i)
         wval.m[i] = w2e ? w2e[i] : 0.0f;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256, clip, wval,
-                       w2e_dev, vol->d_mats);
+                       w2e_dev, vol->d_mats, vol->consts);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;

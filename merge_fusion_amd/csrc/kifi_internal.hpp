@@ -144,10 +144,20 @@ __host__ __device__ __forceinline__ void eye_to_clip_rh(int Wi, int Hi, float fx
 
 // Per-frame matrices of an integration, computed on the device from worldToEye
 // in the reference's order (Integrator.cpp:41-42).
+// Clip rows x, y and w of a voxel centre as affine functions of its integer
+// coordinates (w2c * (vl * (x, y, z) + c0)), evaluated with fused multiply-adds,
+// and a bound `eps` that exceeds twice their distance from the reference-order
+// binary32 values: a cheap conservative test of the update's cone.
+struct ClipPre {
+    float x[4], y[4], w[4];
+    float eps;
+};
+
 struct FrameMats {
     Mat4 w2e;   // worldToEye as given
     Mat4 w2c;   // EyeToClipRH() * worldToEye
     Mat4 e2w;   // invert_transform(worldToEye)
+    ClipPre pre;
 };
 
 // A depth frame on the device: uint16 millimetres (converted as DepthStream.cpp:89-90,

@@ -163,13 +163,21 @@ float orc_f16_to_f32(uint16_t h)
 
 /* FP16 distance + uint8 weight storage (no reference counterpart; the unbuilt
  * cuda/Voxel.h:38-64 packs unorm16 + 8-bit weight with a running average).
- * Update in binary32: d' = (d*w + s) / (w + 1), weight saturates at 255. */
+ * Update in binary32: d' = (d*w + s) * R[w] with R[w] = 1/(w + 1) correctly
+ * rounded to binary32 (a 256-entry table; one rounding more than a division,
+ * far below the binary16 rounding of the stored distance); the weight
+ * saturates at 255, where R[255] = 1/256 exactly. */
+float orc_f16u8_recip(uint32_t w)
+{
+    return 1.0f / (float)(w + 1u);
+}
+
 void orc_voxel_update_f16u8(uint32_t *p, float s)
 {
     uint32_t w = (*p >> 16) & 0xffu;
     float d = orc_f16_to_f32((uint16_t)(*p & 0xffffu));
     float wf = (float)w;
-    float dn = (d * wf + s) / (wf + 1.0f);
+    float dn = (d * wf + s) * orc_f16u8_recip(w);
     uint32_t wn = w < 255u ? w + 1u : 255u;
     *p = (uint32_t)orc_f32_to_f16(dn) | (wn << 16);
 }

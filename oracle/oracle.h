@@ -109,6 +109,7 @@ size_t orc_integrate_f32(orc_volume *v, const float *depth_m, const orc_camera *
 /* ---- voxel helpers ---- */
 void     orc_voxel_update_f32(float *sum_weight, float d);      /* Voxel.h:53-57 */
 float    orc_voxel_distance_f32(const float *sum_weight);       /* Voxel.h:39-44 */
+float    orc_f16u8_recip(uint32_t w);
 void     orc_voxel_update_f16u8(uint32_t *packed, float d);     /* FP16 storage, no reference counterpart */
 float    orc_voxel_distance_f16u8(uint32_t packed);
 uint16_t orc_f32_to_f16(float f);                                /* IEEE binary16, round to nearest even */

@@ -53,6 +53,49 @@ def test_voxel_f16u8_matches_same_cases():
         assert abs(L.orc_voxel_distance_f16u8(p.value) - want) <= tol * max(abs(want), 1e-3)
 
 
+def test_f16u8_reciprocal_table_is_correctly_rounded():
+    """R[w] = 1/(w+1) rounded to nearest binary32, checked exactly with rationals:
+    |R - 1/(w+1)| <= half a binary32 ulp of R, and R[255] = 1/256 exactly."""
+    from fractions import Fraction
+    L.orc_f16u8_recip.argtypes = [ctypes.c_uint32]
+    L.orc_f16u8_recip.restype = ctypes.c_float
+    for w in range(256):
+        r = L.orc_f16u8_recip(w)
+        half_ulp = Fraction(float(np.spacing(np.float32(r)))) / 2
+        assert abs(Fraction(r) - Fraction(1, w + 1)) <= half_ulp, w
+    assert L.orc_f16u8_recip(255) == 1.0 / 256
+
+
+def test_f16u8_update_vs_fp64():
+    """The binary32 update (d w + s) R[w] before the binary16 store is within
+    4 binary32 ulp of the exact (d w + s)/(w + 1); the stored distance is the
+    binary16 rounding of the exact value except where that value lies within
+    those few ulp of a binary16 rounding boundary."""
+    rng = np.random.default_rng(5)
+    tau = 0.04
+    bad = 0
+    for _ in range(4000):
+        w = int(rng.integers(0, 256))
+        d = float(np.float16(rng.uniform(-tau, tau)))
+        s = float(np.float32(rng.uniform(-tau, tau)))
+        p = ctypes.c_uint32((int(np.float16(d).view(np.uint16))) | (w << 16))
+        L.orc_voxel_update_f16u8(ctypes.byref(p), s)
+        exact = (d * w + s) / (w + 1)
+        got = float(np.uint16(p.value & 0xFFFF).view(np.float16))
+        assert (p.value >> 16) & 0xFF == min(w + 1, 255)
+        lo, hi = np.float16(exact), np.float16(exact)
+        if float(lo) > exact:
+            lo = np.nextafter(lo, np.float16(-1))
+        if float(hi) < exact:
+            hi = np.nextafter(hi, np.float16(1))
+        assert got in (float(lo), float(hi))        # one of the two binary16 neighbours
+        if got != float(np.float16(exact)):
+            mid = (float(lo) + float(hi)) / 2
+            assert abs(exact - mid) <= 4 * float(np.spacing(np.float32(exact))) + 1e-12
+            bad += 1
+    assert bad <= 4
+
+
 def test_voxel_f16u8_weight_saturates():
     p = ctypes.c_uint32(0)
     for _ in range(300):
