@@ -322,6 +322,10 @@ kifi_status kifi_twistnet_loss_grad(kifi_twistnet *net, const float *x, const fl
  * replacement from the valid pairs of `set` (seeded); losses[steps] may be NULL */
 kifi_status kifi_twistnet_train(kifi_twistnet *net, kifi_pairset *set, int steps, int batch, float lr,
                                 uint64_t seed, float *losses);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8; the same step as training) with a
+ * caller's gradient in parameter order, e.g. one all-reduced over several
+ * trainers.  The step count continues the network's (set_params resets it). */
+kifi_status kifi_twistnet_apply_gradient(kifi_twistnet *net, const float *grad, float lr);
 /* twists (physical units) predicted for pairs [first, first + n) of a set */
 kifi_status kifi_twistnet_predict_set(kifi_twistnet *net, kifi_pairset *set, int first, int n, float *twists);
 
@@ -347,6 +351,21 @@ kifi_status kifi_render(kifi_ctx *ctx, const struct kifi_scene_s *scene, const k
 /* Attach a network to a pipeline (NULL detaches); tracked frames then start ICP
  * from its prediction, with the fallback of the pipeline's kifi_icp_params. */
 kifi_status kifi_pipeline_set_twistnet(kifi_pipeline *p, kifi_twistnet *net);
+
+/* ---- the wave64 workgroup primitives: cuda::segmented_inclusive_scan,
+ * segmented_exclusive_scan (cuda/scan.h:8-21) and segmented_reduce
+ * (cuda/reduce.h:8-14) of the reference's unbuilt CUDA part, with the same
+ * argument meaning: `size` unsigned values, segments of `segment` values (64,
+ * 256 or 1024: one workgroup each), scanned in place per segment or summed
+ * into sums[ceil(size / segment)].  Here data and out are host buffers and the
+ * call is synchronous (the reference takes device pointers).  They run the
+ * scan and reduction every kernel of the library uses (kifi_prims.hpp). */
+kifi_status kifi_segmented_inclusive_scan(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                          uint32_t *out);
+kifi_status kifi_segmented_exclusive_scan(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                          uint32_t *out);
+kifi_status kifi_segmented_reduce(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                  uint32_t *sums);
 
 /* ---- measurement and staging (no reference counterpart; the reference times
  * stages with a CPU stop_watch, util/stop_watch.cpp:23-58) ----

@@ -43,7 +43,8 @@ HIP_EXPORTS = [
     "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
     "kifi_twistnet_set_params", "kifi_twistnet_forward", "kifi_twistnet_loss_grad", "kifi_twistnet_train",
     "kifi_twistnet_predict_set", "kifi_pairset_generate", "kifi_pairset_destroy", "kifi_pairset_size",
-    "kifi_pairset_download", "kifi_render", "kifi_pipeline_set_twistnet",
+    "kifi_pairset_download", "kifi_render", "kifi_pipeline_set_twistnet", "kifi_twistnet_apply_gradient",
+    "kifi_segmented_inclusive_scan", "kifi_segmented_exclusive_scan", "kifi_segmented_reduce",
 ]
 STAGE_MERGE_UPDATE, STAGE_SORT, STAGE_ICP, STAGE_MODEL, STAGE_MESH = range(5)
 ICP_POINT_TO_PLANE, ICP_HORN = 0, 1
@@ -196,6 +197,10 @@ def load_hip():
             "kifi_twistnet_forward": ([vp, f32p, i32, f32p], i32),
             "kifi_twistnet_loss_grad": ([vp, f32p, f32p, i32, f32p, f32p], i32),
             "kifi_twistnet_train": ([vp, vp, i32, i32, ctypes.c_float, ctypes.c_uint64, f32p], i32),
+            "kifi_twistnet_apply_gradient": ([vp, f32p, ctypes.c_float], i32),
+            "kifi_segmented_inclusive_scan": ([vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp], i32),
+            "kifi_segmented_exclusive_scan": ([vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp], i32),
+            "kifi_segmented_reduce": ([vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp], i32),
             "kifi_twistnet_predict_set": ([vp, vp, i32, i32, f32p], i32),
             "kifi_pairset_generate": ([vp, ctypes.POINTER(Intrinsics), vp, i32, i32, ctypes.c_uint64,
                                        ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp)], i32),
@@ -366,6 +371,17 @@ class Context:
     def device_copy(self, dst, src, nbytes):
         self.check(self._lib.kifi_device_copy(self._h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes),
                    "kifi_device_copy")
+
+    def segmented(self, op, data, segment):
+        """op: "inclusive" / "exclusive" scan per segment, or "reduce" (sum per
+        segment) of uint32 data, on the device's workgroup primitives."""
+        d = np.ascontiguousarray(data, np.uint32)
+        n = d.size
+        out = np.zeros(n if op != "reduce" else (n + segment - 1) // segment, np.uint32)
+        fn = {"inclusive": self._lib.kifi_segmented_inclusive_scan, "exclusive": self._lib.kifi_segmented_exclusive_scan,
+              "reduce": self._lib.kifi_segmented_reduce}[op]
+        self.check(fn(self._h, d.ctypes.data, n, segment, out.ctypes.data), "kifi_segmented_" + op)
+        return out
 
     def bin(self, points, world_to_clip, width, height):
         """Renderer::Bin: z-buffered splat into a (H, W, 3) float32 map."""
@@ -811,6 +827,14 @@ class TwistNet:
         self.ctx.check(self.ctx._lib.kifi_twistnet_train(self._h, pairs._h, steps, batch, lr, seed, _f32p(losses)),
                        "kifi_twistnet_train")
         return losses[:steps]
+
+    def apply_gradient(self, grad, lr):
+        """One Adam step with a caller's gradient (parameter order)."""
+        g = np.ascontiguousarray(grad, np.float32).reshape(-1)
+        if g.size != self.n_params:
+            raise KifiError("gradient size does not match the network")
+        self.ctx.check(self.ctx._lib.kifi_twistnet_apply_gradient(self._h, _f32p(g), lr),
+                       "kifi_twistnet_apply_gradient")
 
     def predict(self, pairs, first=0, n=None):
         n = pairs.size - first if n is None else n

@@ -6,6 +6,7 @@
 #include <new>
 
 #include "kifi_internal.hpp"
+#include "kifi_prims.hpp"
 
 namespace kifi {
 
@@ -90,6 +91,56 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed)
     vol->cur = 0;
     vol->cap = cap;
     vol->size_bound = n;
+    return KIFI_OK;
+}
+
+// Segmented scans and reduction with the workgroup primitives every kernel
+// uses (kifi_prims.hpp): one segment per workgroup of NT = segment threads.
+// OP 0: inclusive scan, 1: exclusive scan, 2: sum per segment.
+template <int NT, int OP>
+__global__ __launch_bounds__(NT) void k_segmented(const uint32_t *__restrict__ in, uint32_t n,
+                                                  uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t tmp[NT / kWave + 1];
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    if (OP == 2) {
+        const uint32_t s = block_reduce_sum<NT>(v, tmp);
+        if (threadIdx.x == 0)
+            out[blockIdx.x] = s;
+    } else {
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<NT>(v, tmp, tot);
+        if (i < n)
+            out[i] = OP == 0 ? ex + v : ex;
+    }
+}
+
+template <int OP>
+static kifi_status segmented(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment, uint32_t *out)
+{
+    if (!ctx || (size && (!data || !out)) || (segment != 64 && segment != 256 && segment != 1024))
+        return ctx ? set_error(ctx, KIFI_ERR_ARG, "segmented scan/reduce: segment must be 64, 256 or 1024")
+                   : KIFI_ERR_ARG;
+    if (!size)
+        return KIFI_OK;
+    const uint32_t segs = (size + segment - 1) / segment;
+    const size_t outn = OP == 2 ? segs : size;
+    const size_t inb = ((size_t)size * 4 + 255) & ~size_t(255);
+    kifi_status st = ensure_scratch(ctx, inb + outn * 4);
+    if (st != KIFI_OK)
+        return st;
+    uint32_t *din = (uint32_t *)ctx->scratch, *dout = (uint32_t *)((char *)ctx->scratch + inb);
+    KIFI_HIP(ctx, hipMemcpyAsync(din, data, (size_t)size * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (segment == 64)
+        hipLaunchKernelGGL((k_segmented<64, OP>), dim3(segs), dim3(64), 0, ctx->stream, din, size, dout);
+    else if (segment == 256)
+        hipLaunchKernelGGL((k_segmented<256, OP>), dim3(segs), dim3(256), 0, ctx->stream, din, size, dout);
+    else
+        hipLaunchKernelGGL((k_segmented<1024, OP>), dim3(segs), dim3(1024), 0, ctx->stream, din, size, dout);
+    KIFI_HIP(ctx, hipGetLastError());
+    KIFI_HIP(ctx, hipMemcpyAsync(out, dout, outn * 4, hipMemcpyDeviceToHost, ctx->stream));
+    KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KIFI_OK;
 }
 
@@ -215,6 +266,24 @@ kifi_status kifi_device_copy(kifi_ctx *ctx, void *dst, const void *src, size_t b
         KIFI_HIP(ctx, hipGetLastError());
     }
     return KIFI_OK;
+}
+
+kifi_status kifi_segmented_inclusive_scan(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                          uint32_t *out)
+{
+    return segmented<0>(ctx, data, size, segment, out);
+}
+
+kifi_status kifi_segmented_exclusive_scan(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                          uint32_t *out)
+{
+    return segmented<1>(ctx, data, size, segment, out);
+}
+
+kifi_status kifi_segmented_reduce(kifi_ctx *ctx, const uint32_t *data, uint32_t size, uint32_t segment,
+                                  uint32_t *sums)
+{
+    return segmented<2>(ctx, data, size, segment, sums);
 }
 
 kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[6])

@@ -1361,6 +1361,18 @@ kifi_status kifi_twistnet_loss_grad(kifi_twistnet *n, const float *x, const floa
     return KIFI_OK;
 }
 
+kifi_status kifi_twistnet_apply_gradient(kifi_twistnet *n, const float *grad, float lr)
+{
+    if (!n || !grad || !(lr >= 0.0f))
+        return KIFI_ERR_ARG;
+    hipStream_t st = n->ctx->stream;
+    KIFI_HIP(n->ctx, hipMemcpyAsync(n->g, grad, n->L.total * 4, hipMemcpyHostToDevice, st));
+    adam_step(n, lr, st);
+    KIFI_HIP(n->ctx, hipGetLastError());
+    KIFI_HIP(n->ctx, hipStreamSynchronize(st));
+    return KIFI_OK;
+}
+
 kifi_status kifi_twistnet_train(kifi_twistnet *n, kifi_pairset *set, int steps, int batch, float lr, uint64_t seed,
                                 float *losses)
 {

@@ -150,3 +150,17 @@ def bf16_round(a):
 def bf16_store(a):
     """Float64 values rounded through binary32 to bfloat16, as the kernels store them."""
     return _bf16_64(a)
+
+
+def adam_step(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) in float64: returns the
+    new parameters and moments for step t >= 1.
+    m_t = b1 m + (1 - b1) g; v_t = b2 v + (1 - b2) g^2;
+    p_t = p - lr (m_t / (1 - b1^t)) / (sqrt(v_t / (1 - b2^t)) + eps)."""
+    p, g, m, v = (np.asarray(a, np.float64) for a in (p, g, m, v))
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v
+

@@ -75,7 +75,70 @@ def record(name, spec):
     }
 
 
+# Full-size cases, at the sizes the bench runs (minutes of oracle time; written
+# to fullsize_golden.json by `make_golden.py --full`):
+#  - the room trajectory (200 frames of 512x424, circle, seeds as the bench's
+#    headline), integrated at its true poses into the 4 mm binary32 volume,
+#    and the frame-199 mesh (Mesher::Mesh, generation order);
+#  - the precision scene (plane + spheres) at 2 mm, 64-bit keys, binary16 + u8,
+#    grown by seeded views about the identity (the bench's growth law) until it
+#    holds at least 20 M voxels, and its mesh.
+FULL_CASES = [
+    # name, scene, scene seed, traj, traj seed, frames, noise seed base, (res, side, trunc), kb, storage, min voxels
+    ("room_trajectory_512x424_4mm_200f", mf.SCENE_ROOM, 2, mf.TRAJ_CIRCLE, 2, 200, 1000, (1024, 4.096, 0.02), 10,
+     orc.STORE_F32, 0),
+    ("precision_512x424_2mm_64bit_f16u8_20M", mf.SCENE_PLANE_SPHERES, 1, mf.TRAJ_SWAY, 1, 400, 100,
+     (4096, 8.192, 0.04), 21, orc.STORE_F16U8, 20_000_000),
+]
+FULL_CAM = (512, 424, 365.0)
+
+
+def full_frames(spec):
+    """(depth, worldToEye) of each frame of a full-size case, in order."""
+    scene_kind, sseed, traj, tseed, n, nseed = spec[:6]
+    W, H, f = FULL_CAM
+    cam = mf.Intrinsics(W, H, f, f)
+    sc = mf.make_scene(scene_kind, sseed)
+    for fr in range(n):
+        pose = mf.trajectory_pose(traj, tseed, fr)
+        yield cam, mf.render(sc, cam, pose, noise_seed=nseed + fr), mf.invert_rigid(pose).astype(np.float32)
+
+
+def full_stop(spec, frame, voxels):
+    """A case ends after its frame count, or at a multiple of 8 frames once it holds min voxels."""
+    n, minv = spec[4], spec[-1]
+    return frame + 1 >= n or (minv and (frame + 1) % 8 == 0 and voxels >= minv)
+
+
+def fingerprint(keys, vals, verts, frames):
+    pick = np.linspace(0, len(keys) - 1, 8).astype(int)
+    return {"frames": int(frames), "voxels": int(len(keys)), "keys_sha256": sha(keys.astype(np.uint64)),
+            "values_sha256": sha(vals),
+            "samples": [[int(keys[i]), [int(x) for x in np.atleast_1d(vals[i].view(np.uint32))]] for i in pick],
+            "mesh_vertices": int(len(verts)), "mesh_sha256": sha(verts)}
+
+
+def record_full(name, spec):
+    res_side_trunc, kb, storage = spec[6], spec[7], spec[8]
+    v = orc.Volume(*res_side_trunc, kb=kb, storage=storage)
+    fr = -1
+    for fr, (cam, d, w2e) in enumerate(full_frames(spec)):
+        v.integrate(d, cam, w2e)
+        if full_stop(spec, fr, v.size()):
+            break
+    keys, vals = v.read()
+    return dict(name=name, **fingerprint(keys, vals, v.mesh(), fr + 1))
+
+
 def main():
+    if "--full" in sys.argv:
+        out = {"generator": "tests/golden/make_golden.py --full", "cases": []}
+        for name, *spec in FULL_CASES:
+            out["cases"].append(record_full(name, spec))
+            print(name, out["cases"][-1]["frames"], "frames", out["cases"][-1]["voxels"], "voxels", flush=True)
+        with open(os.path.join(HERE, "fullsize_golden.json"), "w") as fh:
+            json.dump(out, fh, indent=1)
+        return
     out = {"generator": "tests/golden/make_golden.py", "cases": []}
     for name, *spec in CASES:
         out["cases"].append(record(name, spec))

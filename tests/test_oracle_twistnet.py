@@ -86,3 +86,39 @@ def test_bf16_round_nearest_even():
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 0.0], np.float32)
     r = tn.bf16_round(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2 ** -7) and r[3] == -2.5 and r[4] == 0.0
+
+
+# ---- Adam (Kingma & Ba 2015, Algorithm 1) ----------------------------------
+def test_adam_first_step_is_lr_sign():
+    """m = v = 0: the bias corrections make m^ = g and v^ = g^2, so the first
+    step is -lr g / (|g| + eps): -lr sign(g) for |g| >> eps."""
+    g = np.array([3.0, -2e-3, 1e-6, -7.5, 0.0])
+    p0 = np.array([0.5, -1.0, 2.0, 0.0, 1.0])
+    p1, m, v = tn.adam_step(p0, g, np.zeros(5), np.zeros(5), 1, 1e-3)
+    assert np.allclose(p1 - p0, -1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=0)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+def test_adam_constant_gradient_keeps_the_step():
+    """A constant gradient gives m^ = g and v^ = g^2 at every step (the bias
+    corrections exactly undo the moments' warm-up): every step equals the first."""
+    g = np.array([0.3, -4.0, 2e-5])
+    p, m, v = np.zeros(3), np.zeros(3), np.zeros(3)
+    for t in range(1, 8):
+        q, m, v = tn.adam_step(p, g, m, v, t, 2e-3)
+        assert np.allclose(q - p, -2e-3 * g / (np.abs(g) + 1e-8), rtol=1e-10, atol=0)
+        p = q
+
+
+def test_adam_moments_follow_their_recurrences():
+    """Alternating gradients: the first moment decays geometrically toward the
+    alternating mean while the second tracks g^2 (closed forms of the two
+    linear recurrences)."""
+    p, m, v = np.zeros(1), np.zeros(1), np.zeros(1)
+    for t in range(1, 11):
+        g = np.array([1.0 if t % 2 else -1.0])
+        p, m, v = tn.adam_step(p, g, m, v, t, 1e-3)
+        # m_t = (1 - b1) sum_k b1^(t-k) g_k, v_t = (1 - b2) sum_k b2^(t-k)
+        mt = 0.1 * sum(0.9 ** (t - k) * (1.0 if k % 2 else -1.0) for k in range(1, t + 1))
+        vt = 0.001 * sum(0.999 ** (t - k) for k in range(1, t + 1))
+        assert np.isclose(m[0], mt, rtol=1e-12) and np.isclose(v[0], vt, rtol=1e-12)
