@@ -59,8 +59,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline timing and FP64 checks)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-initialisation comparison")
     ap.add_argument("--fast-frames", type=int, default=1000, help="frames of the fast trajectory")
-    ap.add_argument("--train-pairs", type=int, default=100000)   # the 100k pairs of the training workload
-    ap.add_argument("--train-epochs", type=int, default=16)
+    ap.add_argument("--train-pairs", type=int, default=200000)
+    ap.add_argument("--train-epochs", type=int, default=32)
+    ap.add_argument("--open-loop-frames", type=int, default=300, help="fast-trajectory frames of the open loop")
     ap.add_argument("--no-precision", action="store_true", help="skip the 2 mm / 64-bit-key variant at scale")
     ap.add_argument("--precision-voxels", type=int, default=50_000_000,
                     help="active voxels the 2 mm volume is grown to before its timed frames")
@@ -267,16 +268,96 @@ def twistnet_flops_per_sample():
     return fl + 2 * (2048 * 256 + 256 * 6)
 
 
+# The fast trajectory's tracker settings, the same for the learned and the plain
+# start: pyramid 10/6/4 iterations (20 in all, so the fallback's iteration
+# trigger is the cap), gates 10/20/40 cm, the motion prior, and for the learned
+# start the fallback to plain ICP from the previous pose.
+C5_ITERS = (10, 6, 4)
+C5_GATE_GROWTH = 2.0
+C5_PRIOR = 1e-4
+C5_FALLBACK_RMS = 0.005
+C5_SCENE_SEED, C5_TRAJ_SEED = SEED, 3
+
+
+def c5_params(mf, learned):
+    prm = mf.IcpParams.default()
+    prm.iters[0], prm.iters[1], prm.iters[2] = C5_ITERS
+    prm.level_gate_growth = C5_GATE_GROWTH
+    prm.motion_prior = C5_PRIOR
+    if learned:
+        prm.fallback_rms = C5_FALLBACK_RMS
+        prm.fallback_iters = sum(C5_ITERS)
+    return prm
+
+
+def trajectory_errors(est, truth):
+    """Per-frame errors of eye->world poses against the truth: absolute
+    (m, deg), the absolute trajectory error after the rigid alignment of the
+    positions (Horn / Umeyama without scale, the TUM RGB-D definition) and the
+    relative pose error of consecutive frames."""
+    E = [np.asarray(p, np.float64).reshape(4, 4).T for p in est]
+    G = [np.asarray(p, np.float64).reshape(4, 4).T for p in truth]
+    ab = np.array([pose_error_mats(a, g) for a, g in zip(E, G)])
+    pe, pg = np.array([a[:3, 3] for a in E]), np.array([g[:3, 3] for g in G])
+    me, mg = pe.mean(0), pg.mean(0)
+    U, _, Vt = np.linalg.svd((pg - mg).T @ (pe - me))
+    D = np.eye(3)
+    D[2, 2] = np.sign(np.linalg.det(U @ Vt))
+    R = U @ D @ Vt
+    al = (R @ (pe - me).T).T + mg
+    rel = np.array([pose_error_mats(np.linalg.inv(G[f - 1]) @ G[f], np.linalg.inv(E[f - 1]) @ E[f])
+                    for f in range(1, len(E))])
+    return ab, float(np.sqrt(np.mean(np.sum((al - pg) ** 2, 1)))), rel
+
+
+def pose_error_mats(a, b):
+    d = np.linalg.inv(b) @ a
+    c = np.clip((np.trace(d[:3, :3]) - 1.0) * 0.5, -1.0, 1.0)
+    return np.linalg.norm(d[:3, 3]), float(np.degrees(np.arccos(c)))
+
+
+def net_input(d_mm):
+    """The network's input channel of a frame: 4x4 block mean of the valid
+    depths (m) / 4 m, as the pipeline's downsampling kernel forms it."""
+    b = (d_mm.astype(np.float32) * np.float32(0.001)).reshape(H // 4, 4, W // 4, 4).transpose(0, 2, 1, 3)
+    b = b.reshape(H // 4, W // 4, 16)
+    c = (b > 0).sum(-1)
+    return np.where(c > 0, b.sum(-1) / np.maximum(c, 1) * np.float32(0.25), 0.0).astype(np.float32)
+
+
+def compose_twist(prev_e2w, tw):
+    """prev * [R(omega) | v] in column storage: the start the pipeline derives
+    from a predicted twist."""
+    w, v = np.asarray(tw[:3], np.float64), np.asarray(tw[3:], np.float64)
+    th = np.linalg.norm(w)
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    R = np.eye(3) + (np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K if th > 1e-12 else K)
+    T = np.eye(4)
+    T[:3, :3], T[:3, 3] = R, v
+    return (np.asarray(prev_e2w, np.float64).reshape(4, 4).T @ T).T.reshape(16)
+
+
 def learned_section(mf, ctx, cam, args):
-    """Learned vs plain initial pose on a fast trajectory (steps U[2,10] cm and
-    U[2,15] deg): the network is trained here, on this GPU, by the library's own
-    forward/backward kernels (seeded, bounded budget), then both pipelines run
-    the same device-resident frames."""
+    """Learned vs plain initial pose on the fast trajectory (per-frame steps of
+    U[2,10] cm horizontal translation and U[2,15] deg yaw, 1000 frames): the
+    network is trained here, on this GPU, by the library's own forward/backward
+    kernels on pairs drawn from the same motion law (seeded, bounded budget);
+    then every comparison runs both starts under the same ICP schedule, gates
+    and motion prior, on the same device-resident frames:
+    - closed loop: the pipelines as they are (plain loses the camera);
+    - re-anchored: before each frame the previous pose is set to the true one,
+      so that both trackers start each frame from the same anchor and fuse
+      volumes of comparable quality (the frames/s ratio on equal volumes);
+    - open loop: ICP of frame f against a model integrated at the true poses
+      of frames 0..f-1, also from the true pose (the floor of any start)."""
     scenes = [mf.make_scene(mf.SCENE_ROOM, 100 + k) for k in range(64)]
+    law = dict(min_trans=0.02, max_trans=0.10, min_rot_deg=2.0, max_rot_deg=15.0, walk=True,
+               max_pitch_step_deg=2.0)
     t0 = time.perf_counter()
-    pairs = mf.PairSet(ctx, cam, scenes, args.train_pairs, seed=7)
+    pairs = mf.PairSet(ctx, cam, scenes, args.train_pairs, seed=7, **law)
     ctx.synchronize()
     t_gen = time.perf_counter() - t0
+    _, _, _, pvalid = pairs.download(inputs=False)
     net = mf.TwistNet(ctx, 1)
     spe = args.train_pairs // 256
     losses = []
@@ -287,7 +368,7 @@ def learned_section(mf, ctx, cam, args):
     ctx.synchronize()
     t_train = time.perf_counter() - t0
     pairs.close()
-    hold = mf.PairSet(ctx, cam, [mf.make_scene(mf.SCENE_ROOM, 500 + k) for k in range(8)], 1024, seed=2)
+    hold = mf.PairSet(ctx, cam, [mf.make_scene(mf.SCENE_ROOM, 500 + k) for k in range(8)], 1024, seed=2, **law)
     _, _, truth, valid = hold.download(inputs=False)
     pred = net.predict(hold)
     hold.close()
@@ -297,147 +378,158 @@ def learned_section(mf, ctx, cam, args):
     train_tflops = 3.0 * twistnet_flops_per_sample() * samples / t_train / 1e12
 
     nf = args.fast_frames
-    scene = mf.make_scene(mf.SCENE_ROOM, SEED)
-    poses = [mf.trajectory_pose(mf.TRAJ_FAST, 3, f) for f in range(nf)]
+    scene = mf.make_scene(mf.SCENE_ROOM, C5_SCENE_SEED)
+    poses = [mf.trajectory_pose(mf.TRAJ_FAST, C5_TRAJ_SEED, f) for f in range(nf)]
     fbytes = W * H * 2
     buf = ctx.device_upload(np.zeros(nf * W * H, np.uint16))
     for f in range(nf):
         mf.render_device(ctx, scene, cam, poses[f], 5000 + f, out_ptr=buf.address(f * fbytes))
     ctx.synchronize()
+    w2e0 = mf.invert_rigid(poses[0]).astype(np.float32)
+    samp_frames = set(range(100, nf, 200))
+    c5_samples = {}
 
-    def run(storage, use_net, track, coarse=4):
-        prm = mf.IcpParams.default()
-        prm.level_gate_growth = 2.0       # 10/20/40 cm gates: the fast motion's large start errors
-        prm.iters[2] = coarse
-        if use_net:
-            prm.fallback_rms = 0.005      # 5 mm (1 m-scaled residual)
-            prm.fallback_iters = 20       # inert with the 10/5/4 budget (19 iterations)
-            prm.first_iters[2] = 2        # from the prediction the coarse level stops after 2; the fallback runs 4
-        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=storage, params=prm)
-        if use_net:
+    def make(storage, learned):
+        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=storage, params=c5_params(mf, learned))
+        if learned:
             pipe.set_twistnet(net)
-        errs = []
+        return pipe
+
+    def closed(storage, learned, anchored, track):
+        pipe = make(storage, learned)
+        est, fb_frames = [], []
         ctx.synchronize()
         t0 = time.perf_counter()
         for f in range(nf):
-            w2e = mf.invert_rigid(poses[0]).astype(np.float32) if f == 0 else None
-            pipe.integrate_device(buf.address(f * fbytes), w2e)
+            if anchored and f > 0:
+                pipe.set_pose(poses[f - 1])
+            sample = track and learned and not anchored and f in samp_frames
+            if sample:
+                pts, nrm = pipe.model.download()
+                prev = pipe.pose()
+            pipe.integrate_device(buf.address(f * fbytes), w2e0 if f == 0 else None)
             if track:
-                errs.append(mf.pose_error(pipe.pose(), poses[f])[0])
+                est.append(pipe.pose())
+                if f > 0:
+                    st = pipe.stats()
+                    if st["fallback"]:
+                        fb_frames.append(f)
+                    if sample:
+                        c5_samples[f] = {"pts": pts, "nrm": nrm, "prev_pose": prev, "start_pose": st["start_pose"],
+                                         "pose": est[-1], "fallback": st["fallback"]}
         ctx.synchronize()
         dt = time.perf_counter() - t0
         st = pipe.stats()
         pipe.close()
-        return dt, st, np.array(errs)
+        r = {"icp_iters_per_frame": round(st["sum_iters"] / max(st["frames"], 1), 3),
+             "icp_level_iters_per_frame": [round(x / max(st["frames"], 1), 3) for x in st["sum_level_iters"]],
+             "fallback_rate": round(st["fallbacks"] / max(st["frames"], 1), 4)}
+        if not track:
+            r["frames_per_s"] = round((nf - 1) / dt, 2)
+            return r
+        ab, ate_al, rel = trajectory_errors(est, poses)
+        over = np.nonzero(ab[:, 0] > 0.10)[0]
+        r.update({
+            "ate_m": round(float(np.sqrt(np.mean(ab[:, 0] ** 2))), 4),
+            "ate_aligned_m": round(ate_al, 4),
+            "max_error_m_deg": [round(float(ab[:, 0].max()), 4), round(float(ab[:, 1].max()), 3)],
+            "first_frame_error_over_10cm": int(over[0]) if len(over) else None,
+            "frames_error_over_10cm": int(len(over)),
+            "rpe_rmse_mm_deg": [round(float(np.sqrt(np.mean(rel[:, 0] ** 2))) * 1e3, 3),
+                                round(float(np.sqrt(np.mean(rel[:, 1] ** 2))), 4)],
+            "frames_rpe_over_1cm_or_1deg": int(((rel[:, 0] > 0.01) | (rel[:, 1] > 1.0)).sum()),
+        })
+        if anchored:
+            r["frames_within_2cm_1deg"] = round(float(np.mean((ab[1:, 0] < 0.02) & (ab[1:, 1] < 1.0))), 4)
+        if learned and not anchored:
+            r["_est"] = est
+        return r
 
-    def downsample(d_mm):
-        """Network input channel: 4x4 block mean of the valid depths (m) / 4 m."""
-        b = (d_mm.astype(np.float32) * np.float32(0.001)).reshape(H // 4, 4, W // 4, 4).transpose(0, 2, 1, 3)
-        b = b.reshape(H // 4, W // 4, 16)
-        c = (b > 0).sum(-1)
-        return np.where(c > 0, b.sum(-1) / np.maximum(c, 1) * np.float32(0.25), 0.0).astype(np.float32)
+    res = {"closed_loop": {}, "re_anchored": {}}
+    for name, storage, learned in (("plain_f32", mf.STORAGE_F32, False), ("plain_f16", mf.STORAGE_F16U8, False),
+                                   ("learned_f16", mf.STORAGE_F16U8, True)):
+        for mode, anchored in (("closed_loop", False), ("re_anchored", True)):
+            timed = closed(storage, learned, anchored, False)
+            tracked = closed(storage, learned, anchored, True)
+            res[mode][name] = dict(tracked, frames_per_s=timed["frames_per_s"],
+                                   deterministic=timed["icp_iters_per_frame"] == tracked["icp_iters_per_frame"])
+    est_learned = res["closed_loop"]["learned_f16"].pop("_est")
 
-    def twist_pose(prev_e2w, tw):
-        """prev * [R(omega) | v] in column storage (the pipeline's guess)."""
-        w, v = np.asarray(tw[:3], np.float64), np.asarray(tw[3:], np.float64)
-        th = np.linalg.norm(w)
-        K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
-        R = np.eye(3) + (np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K if th > 1e-12 else K)
-        T = np.eye(4)
-        T[:3, :3], T[:3, 3] = R, v
-        return (np.asarray(prev_e2w, np.float64).reshape(4, 4).T @ T).T.reshape(16)
-
-    def open_loop(n):
-        """ICP of frame f against a model integrated at the true poses of frames
-        0..f-1, from the true previous pose and from the learned start: the
-        iterations and accuracy the initial pose buys, apart from how long
-        closed-loop tracking holds."""
-        host = [mf.render_device(ctx, scene, cam, poses[f], 5000 + f) for f in range(n)]
-        small = [downsample(h) for h in host]
-        x = np.stack([np.stack([small[f - 1], small[f]]) for f in range(1, n)])
-        raw = np.concatenate([net.forward(x[k:k + 256]) for k in range(0, len(x), 256)])
-        tw = raw * mf.TWIST_SCALE
-        prm = mf.IcpParams.default()
-        prm.level_gate_growth = 2.0
-        prm2 = mf.IcpParams.default()
-        prm2.level_gate_growth = 2.0
-        prm2.iters[2] = 2
-        pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=mf.STORAGE_F16U8, params=prm)
-        runs = (("plain_f16", False, prm), ("learned_f16", True, prm),
-                ("plain_f16_coarse2", False, prm2), ("learned_f16_coarse2", True, prm2))
-        out = {name: ([], []) for name, _, _ in runs}
-        for f in range(1, n):
-            pipe.integrate(host[f - 1], mf.invert_rigid(poses[f - 1]).astype(np.float32))
-            for name, learned, pr in runs:
-                guess = twist_pose(poses[f - 1], tw[f - 1]) if learned else poses[f - 1]
-                pose, st = pipe.model.icp_align(host[f], guess, pr)
-                out[name][0].append(st["iterations"])
-                out[name][1].append(mf.pose_error(pose, poses[f]))
-        pipe.close()
-        res_ol = {}
-        for name, (its, errs) in out.items():
-            its, errs = np.array(its), np.array(errs)
-            res_ol[name] = {
-                "icp_iters_per_frame": round(float(its.mean()), 3),
-                "frames_within_2cm_1deg": round(float(np.mean((errs[:, 0] < 0.02) & (errs[:, 1] < 1.0))), 4),
-                "median_error_cm_deg": [round(float(np.median(errs[:, 0])) * 100, 3),
-                                        round(float(np.median(errs[:, 1])), 3)],
-            }
-        res_ol["icp_iters_reduction"] = round(1.0 - res_ol["learned_f16"]["icp_iters_per_frame"] /
-                                              max(res_ol["plain_f16"]["icp_iters_per_frame"], 1e-9), 4)
-        res_ol["icp_iters_reduction_coarse2"] = round(1.0 - res_ol["learned_f16_coarse2"]["icp_iters_per_frame"] /
-                                                      max(res_ol["plain_f16_coarse2"]["icp_iters_per_frame"], 1e-9), 4)
-        res_ol["frames"] = n - 1
-        return res_ol
-
-    res = {}
-    # plain_f32_coarse2: plain ICP under the learned first pass's 10/5/2 schedule,
-    # to separate what the shorter coarse level gives from what the start gives
-    for name, storage, use_net, coarse in (("plain_f32", mf.STORAGE_F32, False, 4),
-                                           ("plain_f16", mf.STORAGE_F16U8, False, 4),
-                                           ("plain_f32_coarse2", mf.STORAGE_F32, False, 2),
-                                           ("learned_f16", mf.STORAGE_F16U8, True, 4)):
-        dt, st, _ = run(storage, use_net, False, coarse)
-        _, _, e = run(storage, use_net, True, coarse)
-        lost = int(np.argmax(e > 0.10)) if (e > 0.10).any() else None
-        res[name] = {
-            "frames_per_s": round((nf - 1) / dt, 2),
-            "icp_iters_per_frame": round(st["sum_iters"] / max(st["frames"], 1), 3),
-            "fallback_rate": round(st["fallbacks"] / max(st["frames"], 1), 4),
-            "ate_m": round(float(np.sqrt(np.mean(e ** 2))), 4),
-            "first_frame_error_over_10cm": lost,
-        }
-    ol = open_loop(min(nf, 300))
+    # open loop: the model integrated at the true poses of frames 0..f-1
+    n_ol = min(nf, args.open_loop_frames)
+    host = [mf.render_device(ctx, scene, cam, poses[f], 5000 + f) for f in range(n_ol)]
+    prm = c5_params(mf, False)
+    small = [net_input(h) for h in host]
+    x = np.stack([np.stack([small[f - 1], small[f]]) for f in range(1, n_ol)])
+    tw = np.concatenate([net.forward(x[k:k + 256]) for k in range(0, len(x), 256)]) * mf.TWIST_SCALE
+    pipe = mf.Pipeline(ctx, cam, RES, SIDE, TRUNC, storage=mf.STORAGE_F16U8, params=prm)
+    out = {k: ([], []) for k in ("true_pose", "learned", "previous_pose")}
+    pred_err = []
+    for f in range(1, n_ol):
+        pipe.integrate(host[f - 1], mf.invert_rigid(poses[f - 1]).astype(np.float32))
+        lg = compose_twist(poses[f - 1], tw[f - 1])
+        pred_err.append(mf.pose_error(lg, poses[f]))
+        for name, guess in (("true_pose", poses[f]), ("learned", lg), ("previous_pose", poses[f - 1])):
+            pose, st = pipe.model.icp_align(host[f], guess, prm)
+            out[name][0].append(st["iterations"])
+            out[name][1].append(mf.pose_error(pose, poses[f]))
+    pipe.close()
+    ol = {"frames": n_ol - 1}
+    for name, (its, errs) in out.items():
+        its, errs = np.array(its), np.array(errs)
+        ol[name] = {"icp_iters_per_frame": round(float(its.mean()), 3),
+                    "frames_within_2cm_1deg": round(float(np.mean((errs[:, 0] < 0.02) & (errs[:, 1] < 1.0))), 4),
+                    "median_error_cm_deg": [round(float(np.median(errs[:, 0])) * 100, 3),
+                                            round(float(np.median(errs[:, 1])), 3)]}
+    pe = np.array(pred_err)
+    ol["prediction_error"] = {"mean_cm_deg": [round(float(pe[:, 0].mean()) * 100, 3), round(float(pe[:, 1].mean()), 3)],
+                              "p90_cm_deg": [round(float(np.percentile(pe[:, 0], 90)) * 100, 3),
+                                             round(float(np.percentile(pe[:, 1], 90)), 3)]}
+    ol["icp_iters_reduction"] = round(1.0 - ol["learned"]["icp_iters_per_frame"] /
+                                      ol["previous_pose"]["icp_iters_per_frame"], 4)
+    ol["icp_iters_reduction_bound"] = round(1.0 - ol["true_pose"]["icp_iters_per_frame"] /
+                                            ol["previous_pose"]["icp_iters_per_frame"], 4)
     buf.free()
     net.close()
-    base = res["plain_f32"]
+    cl, ra = res["closed_loop"], res["re_anchored"]
+
+    def ratio(d, a, b, k):
+        return round(d[a][k] / d[b][k], 3)
+
     return {
-        "workload": "fast trajectory (steps U[2,10] cm, U[2,15] deg), %d frames of the room scene, 512x424, "
-                    "4 mm voxels, 3-level point-to-plane ICP 10/5/4 with 10/20/40 cm gates (both variants); learned: CNN twist as ICP start "
-                    "with the coarse level stopped after 2 iterations (10/5/2), plain ICP 10/5/4 "
-                    "from the previous pose as fallback when the 1 m-scaled residual > 5 mm or 20 iterations "
-                    "are reached (inert: the budgets are 17 and 19); plain_f32_coarse2 is plain ICP at 10/5/2" % nf,
+        "workload": "fast trajectory (per-frame steps U[2,10] cm horizontal, U[2,15] deg yaw, pitch +-2 deg), "
+                    "%d frames of the room scene, 512x424, 4 mm voxels; both starts: point-to-plane ICP "
+                    "%d/%d/%d (20 iterations) with %d/%d/%d cm gates and motion prior %g per inlier; learned: CNN "
+                    "twist as the start, fallback to plain ICP from the previous pose when the 1 m-scaled "
+                    "residual > %g mm or the first pass used all 20 iterations" % (
+                        (nf,) + C5_ITERS + tuple(int(10 * C5_GATE_GROWTH ** l) for l in range(3)) +
+                        (C5_PRIOR, C5_FALLBACK_RMS * 1e3)),
         "training": {
-            "pairs": args.train_pairs, "epochs": args.train_epochs, "batch": 256, "optimizer": "adam, cosine lr 1e-3",
+            "pairs": args.train_pairs, "pairs_valid_frac": round(float(pvalid.mean()), 4),
+            "motion_law": "walk steps: translation U[2,10] cm horizontal, yaw +-U[2,15] deg, pitch +-U[0,2] deg",
+            "epochs": args.train_epochs, "batch": 256, "optimizer": "adam, cosine lr 1e-3",
             "pair_generation_s": round(t_gen, 2), "train_s": round(t_train, 2),
             "samples_per_s": round(samples / t_train, 1), "tflops": round(train_tflops, 3),
             "mfma_bf16_peak_tflops": 2500.0, "mfma_frac": round(train_tflops / 2500.0, 6),
             "loss_first_last": [round(float(losses[0][:20].mean()), 4), round(float(losses[-1][-20:].mean()), 4)],
-            "heldout_rot_err_deg_mean": round(float(rot_err.mean()), 3),
-            "heldout_trans_err_cm_mean": round(float(tr_err.mean() * 100), 3),
+            "heldout_rot_err_deg_mean_p90": [round(float(rot_err.mean()), 3), round(float(np.percentile(rot_err, 90)), 3)],
+            "heldout_trans_err_cm_mean_p90": [round(float(tr_err.mean() * 100), 3),
+                                              round(float(np.percentile(tr_err, 90) * 100), 3)],
             "heldout_prev_pose_rot_err_deg_mean": round(float(np.degrees(np.linalg.norm(truth[valid, :3],
                                                                                           axis=1)).mean()), 3),
             "heldout_prev_pose_trans_err_cm_mean": round(float(np.linalg.norm(truth[valid, 3:],
                                                                               axis=1).mean() * 100), 3),
         },
-        "results": res,
-        "open_loop": dict(ol, note="ICP of each frame against a model integrated at the true poses, from the "
-                                   "true previous pose (plain) and from the learned start, no fallback, at 10/5/4 "
-                                   "and at 10/5/2 (_coarse2): what the initial pose buys, apart from how long "
-                                   "closed-loop tracking holds"),
-        "fps_ratio_learned_f16_vs_plain_f32": round(res["learned_f16"]["frames_per_s"] / base["frames_per_s"], 3),
-        "icp_iters_reduction_vs_plain": round(1.0 - res["learned_f16"]["icp_iters_per_frame"] /
-                                              max(base["icp_iters_per_frame"], 1e-9), 4),
+        "closed_loop": cl,
+        "re_anchored": ra,
+        "open_loop": ol,
+        "fps_ratio_learned_f16_vs_plain_f32": {"closed_loop": ratio(cl, "learned_f16", "plain_f32", "frames_per_s"),
+                                               "re_anchored": ratio(ra, "learned_f16", "plain_f32", "frames_per_s")},
+        "icp_iters_reduction_vs_plain_f32": {
+            "closed_loop": round(1.0 - ratio(cl, "learned_f16", "plain_f32", "icp_iters_per_frame"), 4),
+            "re_anchored": round(1.0 - ratio(ra, "learned_f16", "plain_f32", "icp_iters_per_frame"), 4),
+            "open_loop": ol["icp_iters_reduction"]},
+        "_samples": c5_samples, "_est": est_learned, "_scene_traj": (C5_SCENE_SEED, C5_TRAJ_SEED),
     }
 
 
@@ -594,18 +686,20 @@ def tsdf_error(gk, gd, gw, ok, ov):
     an FP64 oracle volume (keys ok, values ov = [sum, weight] doubles), on the
     voxels where both have the same positive weight.  A voxel whose centre
     projects within rounding of a pixel edge reads the neighbouring pixel in one
-    precision ("pixel choice", an error above 1 mm); the others give the
-    arithmetic error."""
+    precision ("pixel choice": the depth difference of two pixels, an error
+    above 0.1 mm, ten binary16 ulps at the truncation distance); the others
+    give the arithmetic error."""
     if not np.array_equal(gk.astype(np.uint64), ok):
         return {"keys_equal": False}
     w64 = ov[:, 1]
     m = (w64 > 0) & (gw == w64)
     e = np.abs(gd[m] - ov[m, 0] / w64[m])
-    choice = e > 1e-3
+    choice = e > 1e-4
     a = e[~choice]
     return {"keys_equal": True, "voxels": int(len(gk)), "weight_equal_frac": round(float(np.mean(gw == w64)), 6),
             "pixel_choice_frac": round(float(choice.mean()), 6), "max_m": float(a.max()),
-            "p99_m": float(np.quantile(a, 0.99)), "mean_m": float(a.mean())}
+            "p99_m": float(np.quantile(a, 0.99)), "mean_m": float(a.mean()),
+            "all_voxels_p9999_m": float(np.quantile(e, 0.9999))}
 
 
 def cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision):
@@ -654,8 +748,14 @@ def cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision):
         gvol.close()
         ok2, ov2 = o64.read()
         del o64
-        err2 = tsdf_error(gk2, gv2["sum"].astype(np.float64) / gv2["weight"], gv2["weight"].astype(np.float64),
-                          ok2, ov2)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            gd2 = gv2["sum"].astype(np.float64) / gv2["weight"]
+        err2 = tsdf_error(gk2, gd2, gv2["weight"].astype(np.float64), ok2, ov2)
+        c5 = c3 = None
+        if learned is not None:
+            c5 = config5_fp64(mf, ctx, cam, orc, learned)
+        if precision is not None:
+            c3 = config3_fp64(mf, ctx, cam, orc)
     out["error_vs_fp64"] = {
         "config2_pose_mm_deg": {"frames": pose_checks,
                                 "note": "[frame, mm, deg, FP64 iterations]: GPU pose of the tracked frame against "
@@ -664,6 +764,94 @@ def cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision):
         "config2_tsdf_f32": dict(err2, frames=n_t, note="frames 0..%d at the GPU-tracked poses into a binary32 "
                                                          "GPU volume and an FP64 oracle volume" % (n_t - 1)),
     }
+    if c5 is not None:
+        out["error_vs_fp64"].update(c5)
+    if c3 is not None:
+        out["error_vs_fp64"].update(c3)
+    return out
+
+
+def f16u8_split(v):
+    """(distance, weight) of binary16 + u8 voxels packed as half | weight << 16."""
+    v = np.asarray(v, np.uint32)
+    return (v & 0xffff).astype(np.uint16).view(np.float16).astype(np.float64), ((v >> 16) & 0xff).astype(np.float64)
+
+
+def config5_fp64(mf, ctx, cam, orc, learned):
+    """Fast trajectory, learned start: sampled tracked frames against FP64 ICP
+    from the same start, model and settings (prior included); the binary16 + u8
+    TSDF of the first frames at the tracked poses against an FP64 volume."""
+    scene_seed, traj_seed = learned["_scene_traj"]
+    scene = mf.make_scene(mf.SCENE_ROOM, scene_seed)
+    checks = []
+    for f, smp in sorted(learned["_samples"].items()):
+        if smp["fallback"]:
+            continue
+        depth = mf.render_device(ctx, scene, cam, mf.trajectory_pose(mf.TRAJ_FAST, traj_seed, f), 5000 + f)
+        model_w2e = orc.invert_transform(np.asarray(smp["prev_pose"], np.float64).astype(np.float32))
+        opose, oit, _ = orc.icp_f64(depth, cam, smp["pts"], smp["nrm"], model_w2e.astype(np.float64),
+                                    smp["start_pose"], iters=C5_ITERS, gate_growth=C5_GATE_GROWTH,
+                                    motion_prior=C5_PRIOR)
+        dt_m, dr = mf.pose_error(smp["pose"], opose)
+        checks.append([f, round(dt_m * 1e3, 5), round(dr, 6), int(oit)])
+    n_t = 20
+    est = learned["_est"]
+    gvol = mf.Volume(ctx, RES, SIDE, TRUNC, 32, mf.STORAGE_F16U8)
+    o64 = orc.Volume(RES, SIDE, TRUNC, 10, orc.STORE_F64)
+    for f in range(n_t):
+        pose = mf.trajectory_pose(mf.TRAJ_FAST, traj_seed, f) if f == 0 else est[f]
+        depth = mf.render_device(ctx, scene, cam, mf.trajectory_pose(mf.TRAJ_FAST, traj_seed, f), 5000 + f)
+        w = orc.invert_transform(np.asarray(pose, np.float64).astype(np.float32))
+        gvol.integrate(depth, cam, w)
+        o64.integrate(depth, cam, w)
+    gk, gv = gvol.download()
+    gvol.close()
+    ok, ov = o64.read()
+    del o64
+    gd, gw = f16u8_split(gv)
+    return {
+        "config5_pose_mm_deg": {"frames": checks,
+                                "note": "[frame, mm, deg, FP64 iterations]: learned-start GPU pose against FP64 "
+                                        "point-to-plane ICP (oracle, same motion prior, schedule and gates) from "
+                                        "the GPU's start pose against the GPU's model maps; tolerance 1 mm / "
+                                        "0.1 deg; frames that fell back are skipped"},
+        "config5_tsdf_f16u8": dict(tsdf_error(gk, gd, gw, ok, ov), frames=n_t,
+                                   note="fast-trajectory frames 0..%d at the learned pipeline's tracked poses into "
+                                        "a binary16 + u8 GPU volume and an FP64 oracle volume" % (n_t - 1)),
+    }
+
+
+def config3_fp64(mf, ctx, cam, orc, n_t=4):
+    """Precision variant (2 mm voxels, 64-bit keys): the first seeded views of
+    its growth phase into binary32 and binary16 + u8 GPU volumes and an FP64
+    oracle volume."""
+    sc = mf.make_scene(mf.SCENE_PLANE_SPHERES, 1)
+    res, side, trunc = 4096, 8.192, 0.04
+    frames = []
+    for f in range(n_t):
+        pose = mf.trajectory_pose(mf.TRAJ_SWAY, 1, f)
+        frames.append((mf.render_device(ctx, sc, cam, pose, 100 + f), mf.invert_rigid(pose).astype(np.float32)))
+    o64 = orc.Volume(res, side, trunc, 21, orc.STORE_F64)
+    for d, w in frames:
+        o64.integrate(d, cam, w)
+    ok, ov = o64.read()
+    del o64
+    out = {}
+    for name, storage in (("f32", mf.STORAGE_F32), ("f16u8", mf.STORAGE_F16U8)):
+        vol = mf.Volume(ctx, res, side, trunc, 64, storage)
+        for d, w in frames:
+            vol.integrate(d, cam, w)
+        gk, gv = vol.download()
+        vol.close()
+        if storage == mf.STORAGE_F32:
+            with np.errstate(invalid="ignore", divide="ignore"):
+                gd = gv["sum"].astype(np.float64) / gv["weight"]
+            gw = gv["weight"].astype(np.float64)
+        else:
+            gd, gw = f16u8_split(gv)
+        out["config3_tsdf_" + name] = dict(tsdf_error(gk, gd, gw, ok, ov), frames=n_t,
+                                           note="plane + spheres, 2 mm voxels, 64-bit keys: the first %d seeded "
+                                                "views of the growth phase, GPU vs FP64 oracle volume" % n_t)
     return out
 
 
@@ -691,6 +879,8 @@ def main():
     learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
     precision = None if args.no_precision else precision_section(mf, ctx, cam, args)
     cpu = None if args.no_cpu else cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision)
+    for k in [k for k in (learned or {}) if k.startswith("_")]:
+        del learned[k]
     c1.pop("_result")
 
     main_p, roof, diag = head["main"], head["roof"], head["diag"]

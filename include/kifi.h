@@ -179,6 +179,14 @@ typedef struct {
                            /* predicted pose and stops level l after first_iters[l]    */
                            /* iterations when that is > 0 (0: iters[l]); the fallback  */
                            /* pass from the previous pose always runs iters            */
+    float motion_prior;    /* point-to-plane only; > 0: Gaussian prior on the motion   */
+                           /* from the pass's start pose, weight motion_prior per      */
+                           /* inlier: each solve minimises sum r^2 + motion_prior x    */
+                           /* inliers x (|w|^2 + |dc|^2) over the accumulated rotation */
+                           /* w (rad) and camera-centre displacement dc (m) since the  */
+                           /* start.  Keeps directions the scene does not constrain    */
+                           /* (a plane, a sphere on a floor) at the start instead of   */
+                           /* letting them drift; 0 (default): plain Gauss-Newton      */
 } kifi_icp_params;
 
 typedef struct {
@@ -340,6 +348,21 @@ kifi_status kifi_twistnet_predict_set(kifi_twistnet *net, kifi_pairset *set, int
 kifi_status kifi_pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, const struct kifi_scene_s *scenes,
                                   int n_scenes, int n, uint64_t seed, float max_trans, float max_rot_deg,
                                   kifi_pairset **out);
+/* The same with motion ranges: angle U[min_rot_deg, max_rot_deg], length
+ * U[min_trans, max_trans].  walk = 0: random axis and direction as above.
+ * walk = 1: the step of a walking camera (the fast synthetic trajectory's law):
+ * yaw by +-U[min_rot_deg, max_rot_deg], pitch by +-U[0, max_pitch_step_deg]
+ * (kept in [-20, 0] deg), the same roll, a horizontal translation of
+ * U[min_trans, max_trans] in a random direction. */
+typedef struct {
+    float min_trans, max_trans;          /* metres  */
+    float min_rot_deg, max_rot_deg;      /* degrees */
+    float max_pitch_step_deg;            /* walk only */
+    int   walk;
+} kifi_motion_steps;
+kifi_status kifi_pairset_generate_steps(kifi_ctx *ctx, const kifi_intrinsics *cam, const struct kifi_scene_s *scenes,
+                                        int n_scenes, int n, uint64_t seed, const kifi_motion_steps *steps,
+                                        kifi_pairset **out);
 void        kifi_pairset_destroy(kifi_pairset *set);
 int         kifi_pairset_size(const kifi_pairset *set);
 /* inputs n x 2 x 106 x 128, raw targets n x 6, true twists n x 6, valid n; any may be NULL */

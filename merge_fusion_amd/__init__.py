@@ -42,7 +42,7 @@ HIP_EXPORTS = [
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
     "kifi_twistnet_param_count", "kifi_twistnet_create", "kifi_twistnet_destroy", "kifi_twistnet_get_params",
     "kifi_twistnet_set_params", "kifi_twistnet_forward", "kifi_twistnet_loss_grad", "kifi_twistnet_train",
-    "kifi_twistnet_predict_set", "kifi_pairset_generate", "kifi_pairset_destroy", "kifi_pairset_size",
+    "kifi_twistnet_predict_set", "kifi_pairset_generate", "kifi_pairset_generate_steps", "kifi_pairset_destroy", "kifi_pairset_size",
     "kifi_pairset_download", "kifi_render", "kifi_pipeline_set_twistnet", "kifi_twistnet_apply_gradient",
     "kifi_segmented_inclusive_scan", "kifi_segmented_exclusive_scan", "kifi_segmented_reduce",
 ]
@@ -76,7 +76,7 @@ class IcpParams(ctypes.Structure):
                 ("min_inliers", ctypes.c_int), ("fallback_rms", ctypes.c_float), ("fallback_iters", ctypes.c_int),
                 ("level_gate_growth", ctypes.c_float), ("naive_sums", ctypes.c_int),
                 ("eps_level_growth", ctypes.c_float), ("spin_limit", ctypes.c_int),
-                ("first_iters", ctypes.c_int * 3)]
+                ("first_iters", ctypes.c_int * 3), ("motion_prior", ctypes.c_float)]
 
     @classmethod
     def default(cls):
@@ -204,6 +204,8 @@ def load_hip():
             "kifi_twistnet_predict_set": ([vp, vp, i32, i32, f32p], i32),
             "kifi_pairset_generate": ([vp, ctypes.POINTER(Intrinsics), vp, i32, i32, ctypes.c_uint64,
                                        ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp)], i32),
+            "kifi_pairset_generate_steps": ([vp, ctypes.POINTER(Intrinsics), vp, i32, i32, ctypes.c_uint64,
+                                             ctypes.POINTER(MotionSteps), ctypes.POINTER(vp)], i32),
             "kifi_pairset_destroy": ([vp], None),
             "kifi_pairset_size": ([vp], i32),
             "kifi_pairset_download": ([vp, f32p, f32p, f32p, ctypes.POINTER(ctypes.c_int)], i32),
@@ -844,16 +846,36 @@ class TwistNet:
         return out
 
 
-class PairSet:
-    """Device-resident training pairs (kifi_pairset) with ICP-converged targets."""
+class MotionSteps(ctypes.Structure):
+    """kifi_motion_steps: the ranges of a training pair's relative motion."""
+    _fields_ = [("min_trans", ctypes.c_float), ("max_trans", ctypes.c_float),
+                ("min_rot_deg", ctypes.c_float), ("max_rot_deg", ctypes.c_float),
+                ("max_pitch_step_deg", ctypes.c_float), ("walk", ctypes.c_int)]
 
-    def __init__(self, ctx, cam, scenes, n, seed=1, max_trans=0.10, max_rot_deg=15.0):
+
+class PairSet:
+    """Device-resident training pairs (kifi_pairset) with ICP-converged targets.
+
+    The relative motion has a random axis and direction, angle
+    U[min_rot_deg, max_rot_deg], length U[min_trans, max_trans]; walk=True draws
+    it as a walking camera's step instead (yaw +-U[min_rot_deg, max_rot_deg],
+    pitch +-U[0, max_pitch_step_deg], horizontal translation)."""
+
+    def __init__(self, ctx, cam, scenes, n, seed=1, max_trans=0.10, max_rot_deg=15.0, min_trans=0.0,
+                 min_rot_deg=0.0, walk=False, max_pitch_step_deg=2.0):
         self.ctx = ctx
         arr = (Scene * len(scenes))(*scenes)
         h = ctypes.c_void_p()
-        ctx.check(ctx._lib.kifi_pairset_generate(ctx.handle, ctypes.byref(cam), ctypes.cast(arr, ctypes.c_void_p),
-                                                 len(scenes), n, seed, max_trans, max_rot_deg, ctypes.byref(h)),
-                  "kifi_pairset_generate")
+        if min_trans == 0.0 and min_rot_deg == 0.0 and not walk:
+            ctx.check(ctx._lib.kifi_pairset_generate(ctx.handle, ctypes.byref(cam), ctypes.cast(arr, ctypes.c_void_p),
+                                                     len(scenes), n, seed, max_trans, max_rot_deg, ctypes.byref(h)),
+                      "kifi_pairset_generate")
+        else:
+            ms = MotionSteps(min_trans, max_trans, min_rot_deg, max_rot_deg, max_pitch_step_deg, int(bool(walk)))
+            ctx.check(ctx._lib.kifi_pairset_generate_steps(ctx.handle, ctypes.byref(cam),
+                                                           ctypes.cast(arr, ctypes.c_void_p), len(scenes), n, seed,
+                                                           ctypes.byref(ms), ctypes.byref(h)),
+                      "kifi_pairset_generate_steps")
         self._h = h
         self.size = n
         ctx._adopt(self)

@@ -745,6 +745,65 @@ __device__ void horn_increment(const double S[9], const double ms[3], const doub
     D[15] = 1.0;
 }
 
+// Motion prior (kifi_icp_params.motion_prior).  A world-frame increment
+// (w, v) moves the camera centre c by w x c + v to first order; the prior
+// measures the motion since the pass's start as m = M xi with
+// M = [[I, 0], [-[c]x, I]] (c: the start's centre), accumulated over the
+// pass's solves, and each solve minimises |J xi - r|^2 + mu |m_acc + M xi|^2:
+// (A + mu M^T M) xi = b - mu M^T m_acc.
+__device__ void prior_reset(IcpState *st, const double *start)
+{
+    for (int i = 0; i < 6; ++i)
+        st->prior_acc[i] = 0.0;
+    for (int i = 0; i < 3; ++i)
+        st->prior_c[i] = start[12 + i];
+}
+
+__device__ void prior_matrix(const double c[3], double M[36])
+{
+    for (int i = 0; i < 36; ++i)
+        M[i] = 0.0;
+    for (int i = 0; i < 6; ++i)
+        M[6 * i + i] = 1.0;
+    // rows 3..5, columns 0..2: -[c]x
+    M[6 * 3 + 1] = c[2];
+    M[6 * 3 + 2] = -c[1];
+    M[6 * 4 + 0] = -c[2];
+    M[6 * 4 + 2] = c[0];
+    M[6 * 5 + 0] = c[1];
+    M[6 * 5 + 1] = -c[0];
+}
+
+__device__ void prior_apply(const IcpState *st, double mu, double A[36], double b[6])
+{
+    double M[36];
+    prior_matrix(st->prior_c, M);
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 6; ++k)
+            s += M[6 * k + i] * st->prior_acc[k];
+        b[i] -= mu * s;
+        for (int j = 0; j < 6; ++j) {
+            double h = 0.0;
+            for (int k = 0; k < 6; ++k)
+                h += M[6 * k + i] * M[6 * k + j];
+            A[6 * i + j] += mu * h;
+        }
+    }
+}
+
+__device__ void prior_accumulate(IcpState *st, const double xi[6])
+{
+    double M[36];
+    prior_matrix(st->prior_c, M);
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 6; ++j)
+            s += M[6 * i + j] * xi[j];
+        st->prior_acc[i] += s;
+    }
+}
+
 // Reduce the block partials (compensated), solve in FP64, compose the pose.
 // Run by the whole last workgroup of an association kernel; the partials of the
 // other workgroups are read with device-scope loads after the ticket.
@@ -799,9 +858,14 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
             }
         for (int i = 0; i < 6; ++i)
             b[i] = v[21 + i];
+        if (sp.prior > 0.0)
+            prior_apply(st, sp.prior * cnt, A, b);
         ok = cholesky6(A, b, xi);
-        if (ok)
+        if (ok) {
             twist_to_mat(xi, D);
+            if (sp.prior > 0.0)
+                prior_accumulate(st, xi);
+        }
     } else {
         const double ms[3] = { v[0] / cnt, v[1] / cnt, v[2] / cnt }, md[3] = { v[3] / cnt, v[4] / cnt, v[5] / cnt };
         double S[9];
@@ -868,6 +932,7 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     st->rms_1m = 0.0f;
     st->inliers = 0;
     st->ticket = 0;
+    prior_reset(st, g);
 }
 
 
@@ -882,6 +947,7 @@ __device__ __forceinline__ bool fallback_needed(const IcpState *st, float max_rm
 
 __device__ void fallback_setup(IcpState *st)
 {
+    prior_reset(st, st->prev);
     st->fallback = 1;
     st->alt_rms = st->failed ? FLT_MAX : st->rms_1m;
     st->alt_inliers = st->inliers;
@@ -1410,6 +1476,7 @@ static IcpSolveParams solve_params(const kifi_icp_params *ip, int l)
     sp.eps_rot = ip->eps_rot * g;
     sp.eps_trans = ip->eps_trans * g;
     sp.min_inliers = ip->min_inliers;
+    sp.prior = ip->motion_prior > 0.0f ? (double)ip->motion_prior : 0.0;
     return sp;
 }
 

@@ -34,6 +34,7 @@ struct IcpLevel {
 struct IcpSolveParams {
     double eps_rot, eps_trans;
     int    min_inliers;
+    double prior;           // motion prior weight per inlier (0: off)
 };
 
 // Device-resident tracking state of one frame (plus running totals).
@@ -58,6 +59,8 @@ struct IcpState {
     unsigned gen;           // iterations solved by multi-iteration launches (monotone)
     unsigned spin_timeouts; // waits of a multi-iteration launch that gave up (0 unless broken)
     double sums[30];        // reduced sums of the last association (icp_finish layout)
+    double prior_acc[6];    // motion since the pass's start: rotation, camera-centre displacement
+    double prior_c[3];      // camera centre of the pass's start pose
     // running totals over frames
     long long sum_iters;
     long long frames_tracked;

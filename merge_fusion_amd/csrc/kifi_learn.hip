@@ -1475,8 +1475,32 @@ void kifi_pairset_destroy(kifi_pairset *s)
     delete s;
 }
 
+static kifi_status pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, const kifi_scene *scenes, int n_scenes,
+                                    int n, uint64_t seed, const kifi_motion_steps &ms, kifi_pairset **out);
+
 kifi_status kifi_pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, const kifi_scene *scenes, int n_scenes,
                                   int n, uint64_t seed, float max_trans, float max_rot_deg, kifi_pairset **out)
+{
+    kifi_motion_steps ms{};
+    ms.max_trans = max_trans;
+    ms.max_rot_deg = max_rot_deg;
+    return pairset_generate(ctx, cam, scenes, n_scenes, n, seed, ms, out);
+}
+
+kifi_status kifi_pairset_generate_steps(kifi_ctx *ctx, const kifi_intrinsics *cam, const kifi_scene *scenes,
+                                        int n_scenes, int n, uint64_t seed, const kifi_motion_steps *steps,
+                                        kifi_pairset **out)
+{
+    if (!steps || !(steps->min_trans >= 0.0f) || !(steps->max_trans >= steps->min_trans) ||
+        !(steps->min_rot_deg >= 0.0f) || !(steps->max_rot_deg >= steps->min_rot_deg) ||
+        !(steps->max_pitch_step_deg >= 0.0f))
+        return ctx ? set_error(ctx, KIFI_ERR_ARG, "kifi_pairset_generate_steps: invalid step ranges")
+                   : KIFI_ERR_ARG;
+    return pairset_generate(ctx, cam, scenes, n_scenes, n, seed, *steps, out);
+}
+
+static kifi_status pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, const kifi_scene *scenes, int n_scenes,
+                                    int n, uint64_t seed, const kifi_motion_steps &ms, kifi_pairset **out)
 {
     if (!ctx || !cam || !scenes || n_scenes < 1 || n < 1 || !out || cam->width > 4 * kInW ||
         cam->height > 4 * kInH + 3 || cam->width % 4 || cam->height % 4)
@@ -1526,24 +1550,52 @@ kifi_status kifi_pairset_generate(kifi_ctx *ctx, const kifi_intrinsics *cam, con
         mul44(Ty, Tx, T1);
         mul44(T1, Tz, PA);
         PA[12] = px; PA[13] = py; PA[14] = pz;
-        // relative motion: random axis, angle U[0, max_rot]; random direction, length U[0, max_trans]
-        double ax[3], dir[3], na = 0, nd = 0;
-        for (int j = 0; j < 3; ++j) {
-            ax[j] = gauss_host(r);
-            dir[j] = gauss_host(r);
-            na += ax[j] * ax[j];
-            nd += dir[j] * dir[j];
+        if (!ms.walk) {
+            // relative motion: random axis, angle U[min_rot, max_rot]; random
+            // direction, length U[min_trans, max_trans]
+            double ax[3], dir[3], na = 0, nd = 0;
+            for (int j = 0; j < 3; ++j) {
+                ax[j] = gauss_host(r);
+                dir[j] = gauss_host(r);
+                na += ax[j] * ax[j];
+                nd += dir[j] * dir[j];
+            }
+            na = std::sqrt(na);
+            nd = std::sqrt(nd);
+            const double a = (ms.min_rot_deg + uniform_host(r) * (ms.max_rot_deg - ms.min_rot_deg)) * deg;
+            const double l = ms.min_trans + uniform_host(r) * (ms.max_trans - ms.min_trans);
+            for (int j = 0; j < 3; ++j) {
+                xi[j] = ax[j] / na * a;
+                xi[3 + j] = dir[j] / nd * l;
+            }
+            double T[16];
+            twist_exp(xi, T);
+            mul44(PA, T, PB);
+        } else {
+            // a step of a walk: yaw +-U[min_rot, max_rot], pitch +-U[0, max_pitch_step]
+            // (kept in [-20, 0] deg), horizontal translation U[min_trans, max_trans]
+            // in a random direction; the same roll
+            const double ang = (ms.min_rot_deg + uniform_host(r) * (ms.max_rot_deg - ms.min_rot_deg)) * deg *
+                               (uniform_host(r) < 0.5 ? -1.0 : 1.0);
+            double p2 = pitch + (2.0 * uniform_host(r) - 1.0) * ms.max_pitch_step_deg * deg;
+            p2 = std::min(0.0, std::max(-20.0 * deg, p2));
+            const double len = ms.min_trans + uniform_host(r) * (ms.max_trans - ms.min_trans);
+            const double dir = uniform_host(r) * 2.0 * 3.14159265358979323846;
+            const double wy2[6] = { 0, yaw + ang, 0, 0, 0, 0 }, wx2[6] = { p2, 0, 0, 0, 0, 0 };
+            double Ty2[16], Tx2[16], T2[16];
+            twist_exp(wy2, Ty2);
+            twist_exp(wx2, Tx2);
+            mul44(Ty2, Tx2, T2);
+            mul44(T2, Tz, PB);
+            PB[12] = px + len * std::cos(dir);
+            PB[13] = py;
+            PB[14] = pz + len * std::sin(dir);
+            // the twist of PA^-1 PB (eye(B) -> eye(A)) for the truth record
+            double Ai[16], D[16];
+            inv_rigid(PA, Ai);
+            mul44(Ai, PB, D);
+            twist_log(D, xi);
         }
-        na = std::sqrt(na);
-        nd = std::sqrt(nd);
-        const double a = uniform_host(r) * max_rot_deg * deg, l = uniform_host(r) * max_trans;
-        for (int j = 0; j < 3; ++j) {
-            xi[j] = ax[j] / na * a;
-            xi[3 + j] = dir[j] / nd * l;
-        }
-        double T[16];
-        twist_exp(xi, T);
-        mul44(PA, T, PB);
     }
     e = hipMemcpyAsync(d_pose, poses.data(), poses.size() * 8, hipMemcpyHostToDevice, st);
     kifi_status status = e == hipSuccess ? KIFI_OK : hip_error(ctx, e, "kifi_pairset_generate upload");

@@ -122,6 +122,7 @@ kifi_status kifi_icp_params_default(kifi_icp_params *p)
     p->spin_limit = 0;
     for (int l = 0; l < 3; ++l)
         p->first_iters[l] = 0;    // the first pass runs iters
+    p->motion_prior = 0.0f;
     return KIFI_OK;
 }
 

@@ -291,6 +291,9 @@ def _icp_sigs(L):
     L.orc_icp_level_f64.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P, F64P,
                                     i32, d, d, d, F64P]
     L.orc_icp_level_f64.restype = i32
+    L.orc_icp_level_f64_prior.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P,
+                                          F64P, i32, d, d, d, d, F64P, F64P, F64P]
+    L.orc_icp_level_f64_prior.restype = i32
     L.orc_icp_sums_f64.argtypes = [F32P, i32, i32, d, d, d, d, F32P, F32P, ctypes.POINTER(Camera), F64P, F64P, d,
                                    F64P]
     L.orc_icp_sums_f64.restype = None
@@ -338,10 +341,11 @@ def depth_pyramid(depth_mm, levels=3):
 
 
 def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dist_thresh=0.1,
-            eps_rot=2e-4, eps_trans=2e-4, gate_growth=1.0, eps_growth=4.0):
+            eps_rot=2e-4, eps_trans=2e-4, gate_growth=1.0, eps_growth=4.0, motion_prior=0.0):
     """FP64 point-to-plane ICP over the pyramid; returns (pose, total iterations, last rms).
     Level l gates correspondences at dist_thresh x gate_growth^l and converges
-    at eps_rot, eps_trans x eps_growth^l."""
+    at eps_rot, eps_trans x eps_growth^l; motion_prior > 0 adds the prior on the
+    motion from guess_e2w (kifi_icp_params.motion_prior), carried across levels."""
     L = lib()
     if not hasattr(L, "_icp_ready"):
         _icp_sigs(L)
@@ -353,13 +357,17 @@ def icp_f64(depth_mm, cam, pts, nrm, model_w2e, guess_e2w, iters=(10, 5, 4), dis
     P = np.ascontiguousarray(pts, np.float32)
     N = np.ascontiguousarray(nrm, np.float32)
     total, st = 0, np.zeros(2)
+    centre = pose[12:15].copy()
+    acc = np.zeros(6)
     for l in range(len(iters) - 1, -1, -1):
         d = pyr[l]
         s = 1.0 / (1 << l)
-        n = L.orc_icp_level_f64(f32p(d), d.shape[1], d.shape[0], cam.fx * s, cam.fy * s, cam.cx * s, cam.cy * s,
-                                f32p(P), f32p(N), ctypes.byref(c), W2E.ctypes.data_as(F64P),
-                                pose.ctypes.data_as(F64P), iters[l], dist_thresh * gate_growth ** l,
-                                eps_rot * eps_growth ** l, eps_trans * eps_growth ** l, st.ctypes.data_as(F64P))
+        n = L.orc_icp_level_f64_prior(f32p(d), d.shape[1], d.shape[0], cam.fx * s, cam.fy * s, cam.cx * s,
+                                      cam.cy * s, f32p(P), f32p(N), ctypes.byref(c), W2E.ctypes.data_as(F64P),
+                                      pose.ctypes.data_as(F64P), iters[l], dist_thresh * gate_growth ** l,
+                                      eps_rot * eps_growth ** l, eps_trans * eps_growth ** l, float(motion_prior),
+                                      centre.ctypes.data_as(F64P), acc.ctypes.data_as(F64P),
+                                      st.ctypes.data_as(F64P))
         total += n
     return pose, total, st[0]
 

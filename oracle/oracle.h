@@ -164,6 +164,14 @@ int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl
                       const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
                       double pose[16], int max_iters, double dist_thresh, double eps_rot,
                       double eps_trans, double stats[2]);
+/* the same with the motion prior of kifi_icp_params.motion_prior: weight `prior` per
+ * inlier, prior_c the pass's start camera centre, prior_acc (in/out, zero at the
+ * pass's start) the accumulated motion (rotation, camera-centre displacement) */
+int orc_icp_level_f64_prior(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                            const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                            double pose[16], int max_iters, double dist_thresh, double eps_rot,
+                            double eps_trans, double prior, const double prior_c[3], double prior_acc[6],
+                            double stats[2]);
 /* the FP64 normal equations of one association (layout of kifi_icp_normal_equations) */
 void orc_icp_sums_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
                       const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],

@@ -392,12 +392,35 @@ void orc_icp_sums_f32terms(const float *depth, int level, const float *pts, cons
  * with world->eye modelW2E.  pose_io: eye->world guess in, result out.
  * Convergence: |w| < eps_rot and |t| < eps_trans.  Returns iterations run;
  * stats[0] = RMS of the last association, stats[1] = its inlier count. */
-int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
-                      const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
-                      double pose[16], int max_iters, double dist_thresh, double eps_rot,
-                      double eps_trans, double stats[2])
+/* The motion prior of kifi_icp_params.motion_prior, restated: the motion since
+ * the pass's start is m = M xi accumulated over the solves, with
+ * M = [[I, 0], [-[c]x, I]] for the start's camera centre c (a world-frame
+ * increment (w, v) moves c by w x c + v to first order), and each solve is
+ * (A + mu M^T M) xi = b - mu M^T m_acc with mu = weight x inliers: the normal
+ * equations of |J xi - r|^2 + mu |m_acc + M xi|^2.  No reference counterpart. */
+static void prior_matrix(const double c[3], double M[36])
+{
+    memset(M, 0, 36 * sizeof(double));
+    for (int i = 0; i < 6; ++i)
+        M[6 * i + i] = 1.0;
+    /* rows 3..5, columns 0..2: -[c]x, so that row 3 + k is (w x c)_k */
+    M[6 * 3 + 1] = c[2];
+    M[6 * 3 + 2] = -c[1];
+    M[6 * 4 + 0] = -c[2];
+    M[6 * 4 + 2] = c[0];
+    M[6 * 5 + 0] = c[1];
+    M[6 * 5 + 1] = -c[0];
+}
+
+int orc_icp_level_f64_prior(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                            const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                            double pose[16], int max_iters, double dist_thresh, double eps_rot,
+                            double eps_trans, double prior, const double prior_c[3], double prior_acc[6],
+                            double stats[2])
 {
     int it = 0;
+    double M[36];
+    prior_matrix(prior_c, M);
     stats[0] = 0.0;
     stats[1] = 0.0;
     for (; it < max_iters; ++it) {
@@ -417,9 +440,31 @@ int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl
         stats[1] = cnt;
         if (cnt < 12.0)
             break;
+        if (prior > 0.0) {
+            const double mu = prior * cnt;
+            for (int i = 0; i < 6; ++i) {
+                double s = 0.0;
+                for (int k = 0; k < 6; ++k)
+                    s += M[6 * k + i] * prior_acc[k];
+                b[i] -= mu * s;
+                for (int j = 0; j < 6; ++j) {
+                    double h = 0.0;
+                    for (int k = 0; k < 6; ++k)
+                        h += M[6 * k + i] * M[6 * k + j];
+                    A[6 * i + j] += mu * h;
+                }
+            }
+        }
         double xi[6];
         if (cholesky6(A, b, xi))
             break;
+        if (prior > 0.0)
+            for (int i = 0; i < 6; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < 6; ++j)
+                    s += M[6 * i + j] * xi[j];
+                prior_acc[i] += s;
+            }
         double D[16];
         twist_to_mat(xi, D);
         mat_mul_d(D, pose, pose);
@@ -431,4 +476,15 @@ int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl
         }
     }
     return it;
+}
+
+int orc_icp_level_f64(const float *depth, int Wl, int Hl, double fxl, double fyl, double cxl, double cyl,
+                      const float *pts, const float *nrm, const orc_camera *cam, const double W2E[16],
+                      double pose[16], int max_iters, double dist_thresh, double eps_rot,
+                      double eps_trans, double stats[2])
+{
+    const double c[3] = { 0.0, 0.0, 0.0 };
+    double acc[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+    return orc_icp_level_f64_prior(depth, Wl, Hl, fxl, fyl, cxl, cyl, pts, nrm, cam, W2E, pose, max_iters,
+                                   dist_thresh, eps_rot, eps_trans, 0.0, c, acc, stats);
 }

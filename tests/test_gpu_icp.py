@@ -68,6 +68,43 @@ def test_icp_matches_fp64_oracle(ctx, target, eps_growth):
     assert 1 <= st["iterations"] <= 19 and st["inliers"] > 10000 and not st["failed"]
 
 
+@pytest.mark.parametrize("mu", [1e-4, 1e-3])
+def test_icp_motion_prior_matches_fp64_oracle(ctx, mu):
+    """The motion prior (kifi_icp_params.motion_prior) with the fast tracker's
+    schedule 10/6/4 and 10/20/40 cm gates, from a start 1.2 deg / 4 cm off,
+    against the FP64 oracle's prior ICP; 1 mm / 0.1 deg."""
+    sc, gv, ov = _room(ctx, [0, 1, 2])
+    p2 = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 2)
+    w2e = mf.invert_rigid(p2).astype(np.float32)
+    m = mf.Model(ctx, CAM)
+    m.predict(gv, w2e)
+    pts, nrm = m.download()
+    pt = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, 5)
+    d = mf.render(sc, CAM, pt, noise_seed=77)
+    P = np.asarray(pt, np.float64).reshape(4, 4).T
+    ax = np.array([0.2, 1.0, -0.3]) / np.linalg.norm([0.2, 1.0, -0.3])
+    a = np.radians(1.2)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    T = np.eye(4)
+    T[:3, :3] = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+    T[:3, 3] = (0.03, -0.02, 0.015)
+    start = (P @ T).T.reshape(16)
+    prm = mf.IcpParams.default()
+    prm.iters[0], prm.iters[1], prm.iters[2] = 10, 6, 4
+    prm.level_gate_growth = 2.0
+    prm.motion_prior = mu
+    gpose, st = m.icp_align(d, start, prm)
+    opose, oit, _ = orc.icp_f64(d, CAM, pts, nrm, w2e.astype(np.float64), start, iters=(10, 6, 4),
+                                gate_growth=2.0, motion_prior=mu)
+    dt, da = mf.pose_error(gpose, opose)
+    assert dt <= POSE_TOL_M and da <= POSE_TOL_DEG, (dt, da, st, oit)
+    assert not st["failed"] and 1 <= st["iterations"] <= 20
+    # and without the prior the same start ends elsewhere (the prior is applied)
+    prm.motion_prior = 0.0
+    free, _ = m.icp_align(d, start, prm)
+    assert max(mf.pose_error(free, gpose)[0] / POSE_TOL_M, mf.pose_error(free, gpose)[1] / POSE_TOL_DEG) > 0.05
+
+
 def test_icp_horn_mode_matches_reference_align(ctx):
     """KIFI_ICP_HORN = the reference's ICP::Align (4 window-search Horn steps, ICP.cpp:9-265),
     against its binary32 restatement; tolerance 1 mm / 0.1 deg (different reduction order)."""

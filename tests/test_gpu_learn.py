@@ -76,6 +76,32 @@ def test_gradients_match_oracle(ctx):
         assert np.linalg.norm(a - r64) / np.linalg.norm(r64) <= 0.2, i
 
 
+@pytest.mark.parametrize("walk", [False, True])
+def test_pairs_follow_the_motion_ranges(ctx, walk):
+    """kifi_pairset_generate_steps: each pair's true relative motion has a
+    translation of U[2, 10] cm (the twist's translation is the displacement of
+    the camera, [R | v]) and a rotation of U[2, 15] deg; a walking step is a
+    yaw of that size plus a pitch change of at most 2 deg, so its angle lies
+    between 2 deg and about 15.2 deg."""
+    ps = mf.PairSet(ctx, CAM, [mf.make_scene(mf.SCENE_ROOM, 30 + k) for k in range(4)], 128, seed=5,
+                    min_trans=0.02, max_trans=0.10, min_rot_deg=2.0, max_rot_deg=15.0, walk=walk)
+    _, _, truth, valid = ps.download(inputs=False)
+    ps.close()
+    ang = np.degrees(np.linalg.norm(truth[:, :3], axis=1))
+    ln = np.linalg.norm(truth[:, 3:], axis=1)
+    assert valid.mean() >= 0.9
+    assert ln.min() >= 0.02 - 1e-6 and ln.max() <= 0.10 + 1e-6
+    if walk:
+        assert ang.min() >= 2.0 - 1e-3 and ang.max() <= np.hypot(15.0, 2.0) + 1e-3
+    else:
+        assert ang.min() >= 2.0 - 1e-3 and ang.max() <= 15.0 + 1e-3
+    # spread over the ranges
+    assert np.percentile(ln, 10) < 0.035 and np.percentile(ln, 90) > 0.085
+    assert np.percentile(ang, 10) < 4.5 and np.percentile(ang, 90) > 12.5
+    with pytest.raises(mf.KifiError):
+        mf.PairSet(ctx, CAM, [mf.make_scene(mf.SCENE_ROOM, 1)], 4, min_trans=0.2, max_trans=0.1, walk=True)
+
+
 def test_pairs_have_icp_converged_targets(ctx):
     ps = mf.PairSet(ctx, CAM, [mf.make_scene(mf.SCENE_ROOM, 10 + k) for k in range(4)], 96, seed=3)
     x, t, truth, valid = ps.download()
@@ -186,7 +212,8 @@ def _rigid(axis, deg, t):
     return T
 
 
-def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained):
+@pytest.mark.parametrize("cfg", ["first_10_5_2", "prior_10_6_4"])
+def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained, cfg):
     """The learned-init + fallback pipeline (FP16 + u8 storage, fast motion,
     10/20/40 cm gates, first pass 10/5/2 from the prediction, fallback 10/5/4
     from the previous pose at a 1 m-scaled residual above 5 mm) over 32 frames.
@@ -199,8 +226,13 @@ def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained):
     from the previous pose.  (A start only 0.5 m / 30 deg off can converge with
     an acceptable residual, so no fallback fires; from so far off, GPU and FP64
     ICP can settle centimetres apart on the last bits of their sums, which
-    makes it no test of the fallback path.)"""
+    makes it no test of the fallback path.)  prior_10_6_4 is the fast
+    tracker's configuration: 10/6/4 for both passes, the motion prior 1e-4,
+    fallback also when the first pass used all 20 iterations."""
     net, _ = trained
+    prior = cfg == "prior_10_6_4"
+    iters = (10, 6, 4) if prior else (10, 5, 4)
+    mu = 1e-4 if prior else 0.0
     sc = mf.make_scene(mf.SCENE_ROOM, 2)
     poses = [mf.trajectory_pose(mf.TRAJ_FAST, 5, f) for f in range(32)]
     frames = [mf.render_device(ctx, sc, CAM, p, 900 + f) for f, p in enumerate(poses)]
@@ -208,7 +240,10 @@ def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained):
     prm.level_gate_growth = 2.0
     prm.fallback_rms = 0.005
     prm.fallback_iters = 20
-    prm.first_iters[2] = 2
+    prm.iters[0], prm.iters[1], prm.iters[2] = iters
+    prm.motion_prior = mu
+    if not prior:
+        prm.first_iters[2] = 2
     pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02, storage=mf.STORAGE_F16U8, params=prm)
     pipe.set_twistnet(net)
     checked, fallbacks = 0, []
@@ -226,8 +261,9 @@ def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained):
         pose, start = pipe.pose(), np.asarray(st["start_pose"])
         model_w2e = orc.invert_transform(np.asarray(prev, np.float64).astype(np.float32)).astype(np.float64)
         guessed = not np.array_equal(start, np.asarray(prev))
-        first_iters = (10, 5, 2) if guessed else (10, 5, 4)
-        o1, _, _ = orc.icp_f64(d, CAM, pts, nrm, model_w2e, start, iters=first_iters, gate_growth=2.0)
+        first_iters = (10, 5, 2) if guessed and not prior else iters
+        o1, it1, _ = orc.icp_f64(d, CAM, pts, nrm, model_w2e, start, iters=first_iters, gate_growth=2.0,
+                                 motion_prior=mu)
         if not st["fallback"]:
             e = mf.pose_error(pose, o1)
             assert e[0] <= POSE_TOL_M and e[1] <= POSE_TOL_DEG, (f, e)
@@ -235,8 +271,9 @@ def test_learned_pipeline_poses_match_fp64_oracle(ctx, trained):
             fallbacks.append(f)
             # the oracle's first pass breaks the limit too (1 m-scaled residual of its last association)
             s1 = orc.icp_sums_f64(d, CAM, pts, nrm, model_w2e, o1, level=0, dist_thresh=0.1)
-            assert s1[27] < 64 or np.sqrt(s1[29] / s1[27]) > 0.005 * 0.9, (f, s1[27], s1[29])
-            o2, _, _ = orc.icp_f64(d, CAM, pts, nrm, model_w2e, prev, iters=(10, 5, 4), gate_growth=2.0)
+            assert s1[27] < 64 or np.sqrt(s1[29] / s1[27]) > 0.005 * 0.9 or it1 >= 20, (f, s1[27], s1[29], it1)
+            o2, _, _ = orc.icp_f64(d, CAM, pts, nrm, model_w2e, prev, iters=iters, gate_growth=2.0,
+                                   motion_prior=mu)
             e1, e2 = mf.pose_error(pose, o1), mf.pose_error(pose, o2)
             assert min(max(e1[0] / POSE_TOL_M, e1[1] / POSE_TOL_DEG),
                        max(e2[0] / POSE_TOL_M, e2[1] / POSE_TOL_DEG)) <= 1.0, (f, e1, e2)
