@@ -284,6 +284,15 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth_mm, 
  * initial pose (it reads mm); tracking starts from the previous pose. */
 kifi_status kifi_pipeline_integrate_f32(kifi_pipeline *p, const float *depth_m, int depth_on_device,
                                         const float worldToEye[16]);
+/* Tracks the frame as kifi_pipeline_integrate with no pose does (the tracked
+ * pose, kifi_pipeline_pose, the statistics and the network's frame sequence
+ * are this frame's), then fuses it at fuseWorldToEye instead of the tracked
+ * pose; the next frame is tracked from fuseWorldToEye's pose.  The first frame
+ * is fused at fuseWorldToEye untracked.  For trackers evaluated against known
+ * poses (volumes equal whatever the start) and callers with an external pose
+ * source.  No counterpart in the reference. */
+kifi_status kifi_pipeline_integrate_at(kifi_pipeline *p, const uint16_t *depth_mm, int depth_on_device,
+                                       const float fuseWorldToEye[16]);
 /* Initial guess (eye->world) for the next tracked frame instead of the previous
  * pose, e.g. a learned prediction; plain ICP from the previous pose remains the
  * fallback per kifi_icp_params. */

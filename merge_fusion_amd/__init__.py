@@ -36,7 +36,7 @@ HIP_EXPORTS = [
     "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
-    "kifi_pipeline_set_guess", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
+    "kifi_pipeline_set_guess", "kifi_pipeline_integrate_at", "kifi_pipeline_set_pose", "kifi_pipeline_pose", "kifi_pipeline_stats",
     "kifi_pipeline_reset", "kifi_icp_align_device_noalloc",
     "kifi_timing_enable", "kifi_timing_read", "kifi_device_alloc", "kifi_device_free", "kifi_host_alloc", "kifi_host_free", "kifi_memcpy_to_device",
     "kifi_device_copy", "kifi_integrate_totals", "kifi_icp_align_points",
@@ -212,6 +212,7 @@ def load_hip():
             "kifi_render": ([vp, vp, ctypes.POINTER(Intrinsics), f64p, ctypes.c_uint64, ctypes.c_double,
                              ctypes.c_double, vp, i32], i32),
             "kifi_pipeline_set_twistnet": ([vp, vp], i32),
+            "kifi_pipeline_integrate_at": ([vp, vp, i32, f32p], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(lib, name)
@@ -725,6 +726,20 @@ class Pipeline:
         m = None if world_to_eye is None else _f32p(np.ascontiguousarray(world_to_eye, np.float32).reshape(16))
         fn = self.ctx._lib.kifi_pipeline_integrate_f32 if metres else self.ctx._lib.kifi_pipeline_integrate
         self.ctx.check(fn(self._h, ctypes.c_void_p(depth_ptr), 1, m), "kifi_pipeline_integrate")
+
+    def integrate_at(self, depth, fuse_world_to_eye, on_device=False):
+        """kifi_pipeline_integrate_at: track the mm frame (host array, or a device
+        pointer with on_device), fuse it at fuse_world_to_eye."""
+        m = _f32p(np.ascontiguousarray(fuse_world_to_eye, np.float32).reshape(16))
+        if on_device:
+            ptr = ctypes.c_void_p(depth)
+        else:
+            d = np.ascontiguousarray(depth, np.uint16)
+            if d.shape != (self.cam.height, self.cam.width):
+                raise KifiError("frame shape does not match the intrinsics")
+            ptr = d.ctypes.data
+        self.ctx.check(self.ctx._lib.kifi_pipeline_integrate_at(self._h, ptr, int(bool(on_device)), m),
+                       "kifi_pipeline_integrate_at")
 
     def set_twistnet(self, net):
         """Learned initial pose for tracked frames (None detaches)."""

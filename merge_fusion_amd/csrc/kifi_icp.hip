@@ -1164,6 +1164,18 @@ __global__ void k_icp_set_pose_w2e(IcpState *__restrict__ st, Mat4 w2e)
     }
 }
 
+// the previous pose only (the start of the next frame), from a binary32
+// world->eye; the current pose keeps the tracked result
+__global__ void k_icp_set_prev_w2e(IcpState *__restrict__ st, Mat4 w2e)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    Mat4 e = w2e;
+    invert_transform(e.m);
+    for (int i = 0; i < 16; ++i)
+        st->prev[i] = e.m[i];
+}
+
 // world->eye (float) of the tracked pose, as Pipeline.cpp:41-42 derives it
 __global__ void k_icp_w2e(const IcpState *__restrict__ st, Mat4 *__restrict__ w2e)
 {
@@ -1564,6 +1576,14 @@ __global__ void k_store_mat(Mat4 v, Mat4 *__restrict__ out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
         *out = v;
+}
+
+void icp_set_prev(IcpWork *w, const float *w2e, hipStream_t st)
+{
+    Mat4 m;
+    for (int i = 0; i < 16; ++i)
+        m.m[i] = w2e[i];
+    hipLaunchKernelGGL(k_icp_set_prev_w2e, dim3(1), dim3(64), 0, st, w->d_state, m);
 }
 
 void store_w2e(const float *w2e, Mat4 *d_out, hipStream_t st)

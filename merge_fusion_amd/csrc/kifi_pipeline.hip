@@ -11,6 +11,7 @@ namespace kifi {
 kifi_status integrate_dispatch(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 void icp_set_pose(IcpWork *w, const double *e2w, const float *w2e, hipStream_t st);
+void icp_set_prev(IcpWork *w, const float *w2e, hipStream_t st);
 bool twistnet_guess(kifi_twistnet *n, const uint16_t *d_mm, int W, int H, const double *prev_pose_dev,
                     double *guess_dev, hipStream_t st);
 void twistnet_observe(kifi_twistnet *n, const uint16_t *d_mm, int W, int H, hipStream_t st);
@@ -573,7 +574,7 @@ kifi_status kifi_pipeline_set_pose(kifi_pipeline *p, const double e2w[16])
 // One frame of the loop; depth is uint16 mm or binary32 metres (a metre frame
 // does not feed the learned initial pose, which reads mm).
 static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool metres, int on_device,
-                                      const float w2e[16])
+                                      const float w2e[16], const float *fuse_w2e = nullptr)
 {
     if (!p || !depth)
         return KIFI_ERR_ARG;
@@ -642,10 +643,19 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
     // Pipeline.cpp:41-46: worldToEye = invert(eyeToWorld), integrate; the
     // model for the next frame (Pipeline.cpp:48-52, seen from this pose) is
     // splatted inside the integration's voxel pass, then resolved
-    if (w2e)
+    if (w2e) {
         store_w2e(w2e, p->d_w2e, st);         // the given worldToEye as is (Pipeline.cpp:35)
-    else if (!tracked)
+    } else if (fuse_w2e) {
+        // fused at the caller's pose, which the next frame is tracked from; the
+        // tracked pose stays this frame's result
+        store_w2e(fuse_w2e, p->d_w2e, st);
+        if (tracked)
+            icp_set_prev(&p->work, fuse_w2e, st);
+        else
+            icp_set_pose(&p->work, nullptr, fuse_w2e, st);
+    } else if (!tracked) {
         icp_w2e(&p->work, p->d_w2e, st);
+    }
     int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
     model_begin(p->model, p->vol, nullptr, p->d_w2e);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
@@ -669,6 +679,14 @@ kifi_status kifi_pipeline_integrate(kifi_pipeline *p, const uint16_t *depth, int
 kifi_status kifi_pipeline_integrate_f32(kifi_pipeline *p, const float *depth_m, int on_device, const float w2e[16])
 {
     return pipeline_integrate(p, depth_m, true, on_device, w2e);
+}
+
+kifi_status kifi_pipeline_integrate_at(kifi_pipeline *p, const uint16_t *depth, int on_device,
+                                       const float fuse_w2e[16])
+{
+    if (!fuse_w2e)
+        return KIFI_ERR_ARG;
+    return pipeline_integrate(p, depth, false, on_device, nullptr, fuse_w2e);
 }
 
 kifi_status kifi_pipeline_pose(kifi_pipeline *p, double e2w[16])

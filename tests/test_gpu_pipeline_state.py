@@ -3,6 +3,7 @@
 - kifi_pipeline_reset: a pipeline reset after a trajectory runs the next one
   exactly as a new pipeline does (the bench's step is a whole trajectory from
   an empty volume);
+- kifi_pipeline_integrate_at: tracked as usual, fused at the caller's pose;
 - the learned initial pose predicts the motion between consecutive mm frames:
   a frame tracked from the caller's guess (kifi_pipeline_set_guess) still
   becomes the network's previous frame, and a metre frame starts a new
@@ -105,6 +106,43 @@ def test_metre_frame_starts_a_new_network_sequence(ctx):
     start = pipe.stats()["start_pose"]
     pipe.close()
     assert np.array_equal(np.asarray(start), np.asarray(prev))
+
+
+def test_integrate_at_tracks_then_fuses_at_the_given_pose(ctx):
+    """kifi_pipeline_integrate_at: the volume is the one integrated at the given
+    poses (bit for bit), each frame's tracked pose is the one plain
+    kifi_pipeline_integrate computes from the same state, and the next frame
+    starts from the given pose."""
+    frames, poses = _frames(6, seed=5)
+    w2e = [mf.invert_rigid(p).astype(np.float32) for p in poses]
+    pipe = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+    tracked = []
+    for f, d in enumerate(frames):
+        pipe.integrate_at(d, w2e[f])
+        tracked.append(pipe.pose())
+        if f > 0:
+            start = np.asarray(pipe.stats()["start_pose"])
+            fused_prev = np.linalg.inv(w2e[f - 1].astype(np.float64).reshape(4, 4).T)
+            assert np.allclose(start.reshape(4, 4).T, fused_prev, rtol=0, atol=1e-6), f
+    k, v = pipe.volume.download()
+    pipe.close()
+    vol = mf.Volume(ctx, 1024, 4.096, 0.02)
+    for d, w in zip(frames, w2e):
+        vol.integrate(d, CAM, w)
+    k2, v2 = vol.download()
+    vol.close()
+    assert np.array_equal(k, k2) and np.array_equal(np.asarray(v).view(np.uint32), np.asarray(v2).view(np.uint32))
+    # frame 4 tracked by the plain call from the same state
+    ref = mf.Pipeline(ctx, CAM, 1024, 4.096, 0.02)
+    for f in range(4):
+        ref.integrate_at(frames[f], w2e[f])
+    ref.integrate(frames[4])
+    assert np.array_equal(ref.pose(), tracked[4])
+    ref.close()
+    e = mf.pose_error(tracked[4], poses[4])
+    assert e[0] < 0.01 and e[1] < 0.5, e
+    assert not np.allclose(tracked[4], np.linalg.inv(w2e[4].astype(np.float64).reshape(4, 4).T).T.reshape(16),
+                           rtol=0, atol=1e-9)
 
 
 @pytest.fixture()
