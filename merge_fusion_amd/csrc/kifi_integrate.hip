@@ -196,20 +196,25 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
 }
 
 // ================================================================ radix sort
-// LSD radix sort with 8-bit digits (the reference's util::radix_sort,
-// algorithm.h:63-91, is the same digit order and stability contract).
-// Per pass: upsweep (tile digit counts, wave-aggregated LDS histogram),
-// scan (one workgroup per digit), downsweep (stable wave-match ranking,
-// LDS staging, coalesced scatter).
+// LSD radix sort with DB-bit digits (the reference's util::radix_sort,
+// algorithm.h:63-91, sorts 8-bit digits with the same stability contract; any
+// digit width gives the same order).  Only the bits a key can use are sorted:
+// 2 kb + log2(res) (54 for 64-bit keys at res 4096: six 9-bit passes instead
+// of eight 8-bit ones).  Per pass: upsweep (tile digit counts, wave-aggregated
+// LDS histogram), scan (one workgroup per digit), downsweep (stable wave-match
+// ranking, LDS staging, coalesced scatter).
 
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ keys,
                                                          const uint32_t *__restrict__ d_n, int shift,
                                                          uint32_t *__restrict__ counts, int Tb)
 {
-    __shared__ uint32_t hist[256];
+    constexpr int ND = 1 << DB;
+    constexpr uint32_t MASK = ND - 1;
+    __shared__ uint32_t hist[ND];
     const uint32_t n = *d_n;
-    hist[threadIdx.x] = 0;
+    for (int q = threadIdx.x; q < ND; q += kThreads)
+        hist[q] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kRsTile;
     if (base < n) {
@@ -221,19 +226,18 @@ __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ k
         for (int j = 0; j < kRsItems; ++j) {
             const size_t i = base + (size_t)j * kThreads + threadIdx.x;
             const bool ok = i < n;
-            const uint32_t d = ok ? (uint32_t)(kk[j] >> shift) & 0xffu : 0u;
+            const uint32_t d = ok ? (uint32_t)(kk[j] >> shift) & MASK : 0u;
             const uint64_t act = __ballot(ok);
-            const uint64_t m = match_digit(d, 8, act);
+            const uint64_t m = match_digit(d, DB, act);
             if (ok && (m & lanemask_lt()) == 0)
                 atomicAdd(&hist[d], popc64(m));
         }
     }
     __syncthreads();
-    counts[(size_t)threadIdx.x * Tb + blockIdx.x] = hist[threadIdx.x];
+    for (int q = threadIdx.x; q < ND; q += kThreads)
+        counts[(size_t)q * Tb + blockIdx.x] = hist[q];
 }
 
-// One workgroup per digit: exclusive scan of that digit's tile counts, offset
-// by the number of keys with smaller digits.
 // Per digit (one workgroup each): exclusive scan of the digit's tile counts,
 // in place, and the digit's total (no atomics; the downsweep scans the totals).
 __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ counts, int Tb,
@@ -266,16 +270,20 @@ __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ cou
         totals[d] = carry;
 }
 
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__ in,
                                                            K *__restrict__ out,
                                                            const uint32_t *__restrict__ d_n, int shift,
                                                            const uint32_t *__restrict__ offsets, int Tb,
                                                            const uint32_t *__restrict__ totals)
 {
-    __shared__ uint32_t wcnt[kThreads / kWave][256];
-    __shared__ uint32_t dstart[256];
-    __shared__ uint32_t gbase[256];
+    constexpr int ND = 1 << DB;
+    constexpr uint32_t MASK = ND - 1;
+    constexpr int DPT = ND / kThreads;     // digits per thread in the digit scans
+    static_assert(ND % kThreads == 0, "digit count a multiple of the workgroup");
+    __shared__ uint32_t wcnt[kThreads / kWave][ND];
+    __shared__ uint32_t dstart[ND];
+    __shared__ uint32_t gbase[ND];
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     __shared__ K stage[kRsTile];
     const uint32_t n = *d_n;
@@ -283,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     if (tbase >= n)
         return;
     const int w = threadIdx.x / kWave, lane = __lane_id();
-    for (int q = lane; q < 256; q += kWave)
+    for (int q = lane; q < ND; q += kWave)
         wcnt[w][q] = 0;
     // Each wave ranks its own contiguous 64 x kRsItems-key sub-tile, item by item, so the
     // per-wave counters need no barrier (a wave's LDS operations are in order).
@@ -298,9 +306,9 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
         const size_t i = wbase + (size_t)j * kWave + lane;
         const bool ok = i < n;
         const K k = keys[j];
-        const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+        const uint32_t d = (uint32_t)(k >> shift) & MASK;
         const uint64_t act = __ballot(ok);
-        const uint64_t m = match_digit(d, 8, act);
+        const uint64_t m = match_digit(d, DB, act);
         const uint32_t before = popc64(m & lanemask_lt());
         uint32_t old = 0;
         if (ok)
@@ -311,27 +319,42 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     }
     __syncthreads();
     {
-        const int d = threadIdx.x;
-        uint32_t c[kThreads / kWave], acc = 0;
+        // thread t owns digits t*DPT .. t*DPT+DPT-1: per digit the waves'
+        // exclusive offsets, the tile's digit starts and the global bases
+        const int d0 = threadIdx.x * DPT;
+        uint32_t cnt[DPT], tsum = 0, gsum = 0, gtot[DPT];
 #pragma unroll
-        for (int q = 0; q < kThreads / kWave; ++q) {
-            c[q] = wcnt[q][d];
-            wcnt[q][d] = acc;
-            acc += c[q];
+        for (int e = 0; e < DPT; ++e) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int q = 0; q < kThreads / kWave; ++q) {
+                const uint32_t c = wcnt[q][d0 + e];
+                wcnt[q][d0 + e] = acc;
+                acc += c;
+            }
+            cnt[e] = acc;
+            tsum += acc;
+            gtot[e] = totals[d0 + e];
+            gsum += gtot[e];
         }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<kThreads>(acc, tmp, tot);
-        dstart[d] = ex;
-        // digit base: exclusive scan of the 256 digit totals
-        const uint32_t dbase = block_exclusive_scan<kThreads>(totals[d], tmp, tot);
-        gbase[d] = dbase + offsets[(size_t)d * Tb + blockIdx.x] - ex;
+        uint32_t ex = block_exclusive_scan<kThreads>(tsum, tmp, tot);
+        uint32_t dbase = block_exclusive_scan<kThreads>(gsum, tmp, tot);
+#pragma unroll
+        for (int e = 0; e < DPT; ++e) {
+            const int d = d0 + e;
+            dstart[d] = ex;
+            gbase[d] = dbase + offsets[(size_t)d * Tb + blockIdx.x] - ex;
+            ex += cnt[e];
+            dbase += gtot[e];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
         const size_t i = wbase + (size_t)j * kWave + lane;
         if (i < n) {
-            const uint32_t d = (uint32_t)(keys[j] >> shift) & 0xffu;
+            const uint32_t d = (uint32_t)(keys[j] >> shift) & MASK;
             stage[dstart[d] + wcnt[w][d] + rank[j]] = keys[j];
         }
     }
@@ -339,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     const uint32_t tn = (uint32_t)min((size_t)kRsTile, (size_t)n - tbase);
     for (uint32_t i = threadIdx.x; i < tn; i += kThreads) {
         const K k = stage[i];
-        const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+        const uint32_t d = (uint32_t)(k >> shift) & MASK;
         out[gbase[d] + i] = k;
     }
 }
@@ -988,8 +1011,8 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed);
 // scratch layout for one integration (all offsets 256-byte aligned)
 struct IntegrateScratch {
     void     *candA, *candB;
-    uint32_t *rs_counts;     // [256][Tb]
-    uint32_t *rs_totals;     // [passes][256]
+    uint32_t *rs_counts;     // [digits][Tb]
+    uint32_t *rs_totals;     // [passes][digits]
     uint32_t *uq_tiles;      // [Tb]
     float    *depth_m;       // [P] a uint16 frame in metres
     size_t    bytes;
@@ -997,7 +1020,7 @@ struct IntegrateScratch {
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int passes)
+static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int passes, int digits)
 {
     IntegrateScratch s{};
     const size_t ncand = 8 * P;
@@ -1005,8 +1028,8 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     size_t off = 0;
     s.candA = base + off; off += align256(ncand * kbytes);
     s.candB = base + off; off += align256(ncand * kbytes);
-    s.rs_counts = (uint32_t *)(base + off); off += align256(256 * Tb * 4);
-    s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * 256 * 4);
+    s.rs_counts = (uint32_t *)(base + off); off += align256((size_t)digits * Tb * 4);
+    s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * digits * 4);
     s.uq_tiles = (uint32_t *)(base + off); off += align256((Tb + 1) * 4);
     s.depth_m = (float *)(base + off); off += align256(P * 4);
     s.bytes = off;
@@ -1020,7 +1043,13 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     kifi_ctx *ctx = vol->ctx;
     const int W = cam->width, H = cam->height;
     const size_t P = (size_t)W * H;
-    const int passes = KeyTraits<K>::sort_bits / 8;
+    // the bits a key can use: z << 2 kb | y << kb | x with every axis below res
+    constexpr int DB = KeyTraits<K>::digit_bits;
+    constexpr int ND = 1 << DB;
+    int res_bits = 0;
+    while ((1 << res_bits) < vol->desc.res)
+        ++res_bits;
+    const int passes = (2 * KeyTraits<K>::kb + res_bits + DB - 1) / DB;
     const size_t Tb = (8 * P + kRsTile - 1) / kRsTile;
 
     // capacity: the volume can grow by at most 8P voxels per frame
@@ -1043,13 +1072,13 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     const size_t Tm = (vol->size_bound + 8 * P + kMgTile - 1) / kMgTile + 1;
 
     // scratch: integration buffers + merge partitions/dups
-    const IntegrateScratch probe = layout_scratch(nullptr, P, sizeof(K), passes);
+    const IntegrateScratch probe = layout_scratch(nullptr, P, sizeof(K), passes, ND);
     const size_t merge_bytes = align256((Tm + 2) * 8) + align256((Tm + 2) * 4);
     kifi_status st = ensure_scratch(ctx, probe.bytes + merge_bytes);
     if (st != KIFI_OK)
         return st;
     char *base = (char *)ctx->scratch;
-    IntegrateScratch s = layout_scratch(base, P, sizeof(K), passes);
+    IntegrateScratch s = layout_scratch(base, P, sizeof(K), passes, ND);
     uint64_t *part_a = (uint64_t *)(base + s.bytes);
     uint32_t *tile_dups = (uint32_t *)(base + s.bytes + align256((Tm + 2) * 8));
 
@@ -1060,7 +1089,7 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     eye_to_clip_rh(cam->width, cam->height, cam->fx, cam->fy, cam->cx, cam->cy, cam->znear, cam->zfar, clip.m);
     for (int i = 0; i < 16; ++i)
         wval.m[i] = w2e ? w2e[i] : 0.0f;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * 256, clip, wval,
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * ND, clip, wval,
                        w2e_dev, vol->d_mats, vol->consts);
 
     KeygenParams kp;
@@ -1082,19 +1111,19 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
         hipLaunchKernelGGL((k_keygen<K, uint16_t>), kg_grid, dim3(kThreads), 0, stream,
                            (const uint16_t *)d_depth.p, kp, (K *)s.candA, cnt, s.depth_m);
 
-    // radix sort of the candidates (even pass count: result back in candA)
+    // radix sort of the candidates (the sorted keys end in src)
     K *src = (K *)s.candA, *dst = (K *)s.candB;
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 8 * pass;
-        uint32_t *tot = s.rs_totals + pass * 256;
-        hipLaunchKernelGGL(k_rs_upsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
+        const int shift = DB * pass;
+        uint32_t *tot = s.rs_totals + pass * ND;
+        hipLaunchKernelGGL((k_rs_upsweep<K, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
                            &cnt->n_cand, shift, s.rs_counts, (int)Tb);
-        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(kThreads), 0, stream, s.rs_counts, (int)Tb, tot);
-        hipLaunchKernelGGL(k_rs_downsweep<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, dst,
+        hipLaunchKernelGGL(k_rs_scan, dim3(ND), dim3(kThreads), 0, stream, s.rs_counts, (int)Tb, tot);
+        hipLaunchKernelGGL((k_rs_downsweep<K, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, dst,
                            &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
         K *t = src; src = dst; dst = t;
     }
-    // unique (minus the maximum) -> E keys in candB
+    // unique (minus the maximum) -> E keys in dst
     hipLaunchKernelGGL(k_unique_count<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
                        &cnt->n_cand, s.uq_tiles);
     hipLaunchKernelGGL(k_unique_scan, dim3(1), dim3(1024), 0, stream, s.uq_tiles, (int)Tb, cnt);
