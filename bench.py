@@ -50,6 +50,11 @@ TRAJ_FRAMES, MESH_EVERY = 200, 50
 LEVEL_PIXELS = (P, P // 4, P // 16)   # ICP pyramid 512x424, 256x212, 128x106
 
 
+def log(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +73,10 @@ def parse():
     ap.add_argument("--precision-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain copy kernel of this size 20 times (HBM counter calibration)")
+    ap.add_argument("--pmc-window", metavar="COUNTER_CSV",
+                    help="no GPU work: from a rocprofv3 --pmc counter collection of a run with the same --steps and "
+                         "--warmup, each counter's mean over the roofline kernel's headline-pass timed launches, "
+                         "and over the copy kernel's launches of a --calibrate-copy-mib run")
     ap.add_argument("--trace-window", metavar="KERNEL_TRACE_CSV",
                     help="no GPU work: from a rocprofv3 kernel trace of a run with the same --steps and --warmup, "
                          "the roofline kernel's launches of that run's roofline pass, timed window only")
@@ -363,7 +372,11 @@ def learned_section(mf, ctx, cam, args):
     spe = args.train_pairs // 256
     losses = []
     t0 = time.perf_counter()
+    log("fast trajectory: %d training pairs generated in %.1f s; training %d epochs" % (
+        args.train_pairs, t_gen, args.train_epochs))
     for ep in range(args.train_epochs):
+        if ep % 8 == 0:
+            log("  epoch %d" % ep)
         lr = 1e-3 * 0.5 * (1.0 + np.cos(np.pi * ep / args.train_epochs))
         losses.append(net.train(pairs, spe, 256, lr, seed=100 + ep))
     ctx.synchronize()
@@ -601,26 +614,77 @@ ROOF_KERNEL = "k_merge_update<unsigned int, kifi::VoxF32, true>"
 
 
 def trace_window(args):
-    """The headline's roofline-kernel launches in a rocprofv3 kernel trace of a
-    bench run with the same --steps and --warmup.  Every frame of the headline
-    passes launches the roofline kernel's tracked binary32 instance once, and
-    nothing before them does: pass 1 (no events) is launches [0, n), pass 2
-    (the roofline pass) is [n, 2n) with n = 200 (warmup + steps), and its timed
-    window is the last 200 x steps of those.  Prints their count and mean
-    duration, the figure the bench's avg_launch_ms measures with HIP events."""
+    """The headline's launches in a rocprofv3 kernel trace of a bench run with
+    the same --steps and --warmup.  Every frame of the headline passes launches
+    the roofline kernel's tracked binary32 instance once, and nothing before
+    them does: pass 1 (no events) is launches [0, n), pass 2 (the roofline
+    pass) is [n, 2n) with n = 200 (warmup + steps), and their timed windows are
+    the last 200 x steps of each.  Prints the roofline kernel's mean duration
+    over pass 2's window (the figure the bench's avg_launch_ms measures with
+    HIP events) and, for pass 1's window (the headline value's own frames),
+    every kernel's calls and mean duration per frame, the kernel time per frame
+    and the span per frame (from the end of the roofline launch before the
+    window to the end of the window's last one)."""
     rows = []
     with open(args.trace_window, newline="") as fh:
         for r in csv.DictReader(fh):
-            if r["Kernel_Name"].startswith("void kifi::" + ROOF_KERNEL) or r["Kernel_Name"].startswith(ROOF_KERNEL):
-                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
     rows.sort()
+    roof = [(s_, e) for s_, e, k in rows if k.startswith("void kifi::" + ROOF_KERNEL) or k.startswith(ROOF_KERNEL)]
     n = TRAJ_FRAMES * (args.warmup + args.steps)
-    win = rows[n + TRAJ_FRAMES * args.warmup: 2 * n]
-    dur = np.array([e - s for s, e in win], np.float64) * 1e-6
-    print(json.dumps({"kernel": ROOF_KERNEL, "launches_in_trace": len(rows), "window_launches": len(win),
-                      "window": "[%d, %d)" % (n + TRAJ_FRAMES * args.warmup, 2 * n),
-                      "mean_ms": round(float(dur.mean()), 5) if len(dur) else None,
-                      "median_ms": round(float(np.median(dur)), 5) if len(dur) else None}))
+    lo = TRAJ_FRAMES * args.warmup
+    win = roof[n + lo: 2 * n]
+    dur = np.array([e - s_ for s_, e in win], np.float64) * 1e-6
+    out = {"kernel": ROOF_KERNEL, "launches_in_trace": len(roof), "window_launches": len(win),
+           "window": "[%d, %d)" % (n + lo, 2 * n),
+           "mean_ms": round(float(dur.mean()), 5) if len(dur) else None,
+           "median_ms": round(float(np.median(dur)), 5) if len(dur) else None}
+    if len(roof) >= n and lo > 0:
+        t0, t1 = roof[lo - 1][1], roof[n - 1][1]
+        frames = n - lo
+        per, busy, nk = {}, 0, 0
+        for s_, e, k in rows:
+            if t0 < e <= t1:
+                name = k.split("(")[0].replace("void ", "")
+                c = per.setdefault(name, [0, 0])
+                c[0] += 1
+                c[1] += e - s_
+                busy += e - s_
+                nk += 1
+        out["headline_pass_window"] = {
+            "frames": frames, "span_us_per_frame": round((t1 - t0) * 1e-3 / frames, 2),
+            "kernel_us_per_frame": round(busy * 1e-3 / frames, 2),
+            "launches_per_frame": round(nk / frames, 2),
+            "kernels": {k: {"calls_per_frame": round(c / frames, 3), "avg_us": round(t * 1e-3 / c, 2),
+                            "us_per_frame": round(t * 1e-3 / frames, 2)}
+                        for k, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1])}}
+    print(json.dumps(out))
+
+
+def pmc_window(args):
+    """Counter means of a rocprofv3 --pmc collection: the roofline kernel's
+    launches of the headline pass's timed trajectories (as trace_window picks
+    them: pass 1 is the first TRAJ_FRAMES x (warmup + steps) launches), and the
+    device copy kernel's launches (a --calibrate-copy-mib run)."""
+    vals = {}
+    with open(args.pmc_window, newline="") as fh:
+        for r in csv.DictReader(fh):
+            name = r["Kernel_Name"]
+            kind = "roof" if ROOF_KERNEL in name else "copy" if "k_copy16" in name else None
+            if kind is None:
+                continue
+            key = (kind, r["Counter_Name"])
+            vals.setdefault(key, {})
+            d = int(r["Dispatch_Id"])
+            vals[key][d] = vals[key].get(d, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (kind, counter), per in sorted(vals.items()):
+        v = np.array([per[d] for d in sorted(per)], np.float64)
+        if kind == "roof":
+            lo, hi = TRAJ_FRAMES * args.warmup, TRAJ_FRAMES * (args.warmup + args.steps)
+            v = v[lo:hi]
+        out["%s_%s" % (kind, counter)] = {"launches": int(len(v)), "mean": float(v.mean()) if len(v) else None}
+    print(json.dumps(out))
 
 
 def calibrate_copy(mf, mib):
@@ -869,6 +933,9 @@ def main():
     if args.trace_window:
         trace_window(args)
         return
+    if args.pmc_window:
+        pmc_window(args)
+        return
     import merge_fusion_amd as mf
     if args.calibrate_copy_mib:
         calibrate_copy(mf, args.calibrate_copy_mib)
@@ -878,12 +945,18 @@ def main():
 
     ctx = mf.Context(0)
     dframes = ctx.device_upload(frames)
+    log("headline: %d + %d trajectories of %d frames, 5 passes" % (args.warmup, args.steps, TRAJ_FRAMES))
     head = headline(mf, ctx, cam, args, frames, dframes, poses)
+    log("headline done")
     copy_gbs = copy_rate_gbs(mf, ctx)
     c1 = config1_section(mf, ctx, cam)
+    log("config 1 done")
     learned = None if args.no_learned else learned_section(mf, ctx, cam, args)
+    log("fast trajectory done")
     precision = None if args.no_precision else precision_section(mf, ctx, cam, args)
+    log("2 mm variant done")
     cpu = None if args.no_cpu else cpu_leg(mf, ctx, cam, args, frames, poses, head, c1, learned, precision)
+    log("CPU leg done")
     for k in [k for k in (learned or {}) if k.startswith("_")]:
         del learned[k]
     c1.pop("_result")
