@@ -658,6 +658,14 @@ def trace_window(args):
             "kernels": {k: {"calls_per_frame": round(c / frames, 3), "avg_us": round(t * 1e-3 / c, 2),
                             "us_per_frame": round(t * 1e-3 / frames, 2)}
                         for k, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1])}}
+    # the 2 mm variant's timed launches: the last --precision-steps of each 64-bit-key instance
+    tails = {}
+    for s_, e, k in rows:
+        if "k_merge_update<unsigned long" in k:
+            tails.setdefault(k.split("(")[0].replace("void ", ""), []).append(e - s_)
+    if tails:
+        out["precision_timed_merge_update_us"] = {
+            k: round(float(np.mean(v[-args.precision_steps:])) * 1e-3, 2) for k, v in sorted(tails.items())}
     print(json.dumps(out))
 
 
