@@ -796,6 +796,56 @@ __device__ __forceinline__ void splat_point(K key, uint64_t o, const SplatParams
         }
 }
 
+// A tile's splat goes through an LDS table first: pixel -> minimum 64-bit key,
+// kZSlots slots, open addressing with at most kZProbe probes; a pixel that
+// finds no slot goes straight to the global z-buffer.  The tile then issues one
+// global atomic per occupied slot.  The candidates of one tile (consecutive
+// keys: neighbouring voxel rows) hit the same pixels often, and the global
+// 64-bit atomics are what the splat costs.  The minimum is the same.
+constexpr int kZSlots = 512;
+constexpr int kZProbe = 8;
+constexpr uint32_t kZEmpty = 0xffffffffu;
+
+__device__ __forceinline__ void zslot_min(uint32_t *tkey, unsigned long long *tval, unsigned long long *zbuf,
+                                          uint32_t pix, unsigned long long zk)
+{
+    uint32_t s = (pix * 2654435761u) >> (32 - 9);      // kZSlots = 2^9
+    for (int probe = 0; probe < kZProbe; ++probe) {
+        const uint32_t prev = atomicCAS(&tkey[s], kZEmpty, pix);
+        if (prev == kZEmpty || prev == pix) {
+            atomicMin(&tval[s], zk);
+            return;
+        }
+        s = (s + 1) & (kZSlots - 1);
+    }
+    atomicMin(&zbuf[pix], zk);
+}
+
+template <typename K>
+__device__ __forceinline__ void splat_point_lds(K key, uint64_t o, const SplatParams &sp, const Mat4 &w2e,
+                                                uint32_t *tkey, unsigned long long *tval)
+{
+    float z, fxe, fye;
+    if (!splat_visible<K>(key, sp, w2e, z, fxe, fye))
+        return;
+    const float u = fxe / z + sp.cx;
+    const float v = sp.cy - fye / z;
+    if (!(u > -8.0f && u < (float)sp.W + 8.0f && v > -8.0f && v < (float)sp.H + 8.0f))
+        return;
+    const float kf = ceilf(sp.vl * sp.fx / z);
+    const int k = kf < 1.0f ? 1 : (kf > 3.0f ? 3 : (int)kf);
+    const float half = 0.5f * (float)(k - 1);
+    const int u0 = (int)floorf(u - half), v0 = (int)floorf(v - half);
+    const unsigned long long zk = (unsigned long long)orderable_z(z) << 32 | (uint32_t)o;
+    for (int dv = 0; dv < k; ++dv)
+        for (int du = 0; du < k; ++du) {
+            const int pu = u0 + du, pv = v0 + dv;
+            if (pu < 0 || pu >= sp.W || pv < 0 || pv >= sp.H)
+                continue;
+            zslot_min(tkey, tval, sp.zbuf, (uint32_t)(pv * sp.W + pu), zk);
+        }
+}
+
 template <typename K, typename V>
 __device__ __forceinline__ void splat_voxel(K key, const V &val, uint64_t o, const SplatParams &sp, const Mat4 &w2e)
 {
@@ -821,6 +871,8 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     constexpr bool kF16 = sizeof(V) == 4;
     __shared__ float recip[kF16 ? 256 : 1];   // binary16 storage: R[w] = 1/(w + 1)
+    __shared__ uint32_t ztk[SPLAT ? kZSlots : 1];              // the splat's pixel table
+    __shared__ unsigned long long ztv[SPLAT ? kZSlots : 1];
     const uint64_t T = cnt->n_tiles;
     if (blockIdx.x >= T || cnt->overflow)
         return;
@@ -990,11 +1042,23 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
         for (int q = 0; q < PER; ++q)
             if (cand >> q & 1u)
                 h[pos++] = ent[q];
-        __syncthreads();
-        const Mat4 w2e = *sp.w2e;
-        for (uint32_t t = threadIdx.x; t < ncand; t += kThreads) {
-            const uint32_t e = h[t];
-            splat_point<K>(sA[e & 0xffffu], obase + (e >> 16), sp, w2e);
+        if (ncand) {   // block-uniform
+            for (int t = threadIdx.x; t < kZSlots; t += kThreads) {
+                ztk[t] = kZEmpty;
+                ztv[t] = ~0ull;
+            }
+            __syncthreads();   // the list in h and the cleared table
+            const Mat4 w2e = *sp.w2e;
+            for (uint32_t t = threadIdx.x; t < ncand; t += kThreads) {
+                const uint32_t e = h[t];
+                splat_point_lds<K>(sA[e & 0xffffu], obase + (e >> 16), sp, w2e, ztk, ztv);
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < kZSlots; t += kThreads) {
+                const uint32_t pix = ztk[t];
+                if (pix != kZEmpty)
+                    atomicMin(&sp.zbuf[pix], ztv[t]);
+            }
         }
     }
     // A, the frame's updated voxels: one atomic per tile
