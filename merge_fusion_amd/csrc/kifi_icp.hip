@@ -382,7 +382,7 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 template <int MODE, bool COMP, int NV, bool GEN = false>
 __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, CSum *partials, IcpState *st, int level,
                                             const IcpSolveParams &sp, const ModelPose *mpose, unsigned gen = 0,
-                                            int nblocks = 0)
+                                            int nblocks = 0, unsigned *gen_ctr = nullptr)
 {
     __shared__ bool last;
     const unsigned nb = GEN ? (unsigned)nblocks : gridDim.x;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void icp_publish(CSum (&acc)[NV], CsumLds<NV> &lds, C
         st->ticket = 0;
         if (GEN) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&st->gen, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen_ctr ? gen_ctr : &st->gen, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -932,6 +932,8 @@ __global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict_
     st->rms_1m = 0.0f;
     st->inliers = 0;
     st->ticket = 0;
+    for (int i = 0; i < 6; ++i)
+        st->tail_gen[i] = 0;   // each level tail of the frame counts its hand-offs from 0
     prior_reset(st, g);
 }
 
@@ -1040,13 +1042,14 @@ struct IcpFallback {
 // visible before it moved gen.  An agent-scope acquire here would invalidate the
 // L2 of the XCD once per wave, evicting the model maps that the other
 // workgroups are gathering from.
-__device__ bool icp_wait_gen(IcpState *st, unsigned target, unsigned spin_limit)
+__device__ bool icp_wait_gen(IcpState *st, unsigned target, unsigned spin_limit, unsigned *gen_ctr = nullptr)
 {
     __shared__ int ok;
     if (threadIdx.x == 0) {
         int good = 1;
         unsigned n = 0;
-        while ((int)(__hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+        while ((int)(__hip_atomic_load(gen_ctr ? gen_ctr : &st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                     target) < 0) {
             if (++n > spin_limit) {
                 good = 0;
                 __hip_atomic_fetch_add(&st->spin_timeouts, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1115,6 +1118,58 @@ __global__ __launch_bounds__(kIT) void k_icp_fallback_pass(IcpFallback a, ModelP
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the last solve's state
         fallback_pick(st, a.max_rms);
+    }
+}
+
+// ------------------------------------------------ the rest of a level in one launch
+// A pass launches the first iterations of a level one by one and the rest of
+// the level as one launch: a level usually converges within its first
+// iterations, and each launch that only finds the level converged costs a
+// dispatch.  The launch has the level's own workgroup count (every workgroup
+// takes part, so the sums equal those of launched iterations bit for bit) and
+// hands iterations over as the fallback pass does: the solving workgroup of
+// icp_publish releases the state and moves st->gen, the others poll it
+// (bounded by spin_limit) and read the pose and the level flag with coherent
+// loads.  Each tail launch of a frame has a counter of its own
+// (st->tail_gen[pass * 3 + level]), which k_icp_begin clears: it starts at 0,
+// so no workgroup reads a counter that another may already have moved, and
+// nothing needs resetting on the way out.
+template <bool COMP>
+__global__ __launch_bounds__(kIT) void k_icp_level_tail(IcpLevel lv, ModelParams mp,
+                                                        const ModelPose *__restrict__ mpose,
+                                                        const float4 *__restrict__ mpts,
+                                                        const float4 *__restrict__ mnrm, IcpState *st, float gate,
+                                                        CSum *__restrict__ partials, int level, IcpSolveParams sp,
+                                                        int n_iters, unsigned spin_limit, int slot)
+{
+    constexpr int MODE = KIFI_ICP_POINT_TO_PLANE;
+    constexpr int NV = IcpNV<MODE>::nv;
+    __shared__ CsumLds<NV> lds;
+    __shared__ float s_pose[16];
+    __shared__ int s_done;
+    unsigned gen = 0;
+    for (int it = 0; it < n_iters; ++it) {
+        if (threadIdx.x < 16)
+            s_pose[threadIdx.x] = __hip_atomic_load(&st->pose_f.m[threadIdx.x], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        else if (threadIdx.x == 16)
+            s_done = __hip_atomic_load(&st->level_done[level], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const bool done = s_done != 0;
+        Mat4 pose;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            pose.m[i] = s_pose[i];
+        __syncthreads();     // read by every wave before the next iteration rewrites them
+        if (done)
+            break;
+        ++gen;
+        CSum acc[NV];
+        icp_accumulate<MODE, COMP>(lv, mp, mpose, mpts, mnrm, pose, gate, (int)gridDim.x, acc);
+        icp_publish<MODE, COMP, NV, true>(acc, lds, partials, st, level, sp, mpose, gen, (int)gridDim.x,
+                                          &st->tail_gen[slot]);
+        if (!icp_wait_gen(st, gen, spin_limit, &st->tail_gen[slot]))
+            break;
     }
 }
 
@@ -1394,7 +1449,8 @@ void icp_free(IcpWork *w)
     w->d_state = nullptr;
 }
 
-static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels);
+static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels,
+                     int pass);
 static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip,
                                int levels);
 
@@ -1430,11 +1486,11 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrins
     for (int l = 0; l < 3; ++l)
         if (with_fallback && ip->first_iters[l] > 0)
             first.iters[l] = ip->first_iters[l];
-    icp_pass(m, w, cam, &first, levels);
+    icp_pass(m, w, cam, &first, levels, 0);
     if (with_fallback && !icp_fallback_multi(m, w, cam, ip, levels)) {
         hipLaunchKernelGGL(k_icp_fallback_check, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms,
                            ip->fallback_iters);
-        icp_pass(m, w, cam, ip, levels);
+        icp_pass(m, w, cam, ip, levels, 1);
         hipLaunchKernelGGL(k_icp_fallback_pick, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms);
     }
     hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state, w2e_out);
@@ -1499,16 +1555,38 @@ static int icp_blocks(const kifi_ctx *ctx, const IcpLevel &lv)
     return std::min((lv.W * lv.H + kIT - 1) / kIT, std::max(1, ctx->num_cus));
 }
 
-static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels)
+// Iterations of a level launched one by one before the rest of the level runs
+// as one launch (k_icp_level_tail), per level 512x424, 256x212, 128x106: most
+// frames converge within them (1.5, 1.3 and 2.5 solves on average on the room
+// trajectory).
+constexpr int kLaunchedIters[3] = { 2, 2, 3 };
+
+static bool icp_tail_ok(kifi_ctx *ctx)
+{
+    if (ctx->tail_ok < 0) {
+        int a = 0, b = 0;
+        const hipError_t e0 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_icp_level_tail<true>, kIT, 0);
+        const hipError_t e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_icp_level_tail<false>, kIT, 0);
+        (void)hipGetLastError();
+        ctx->tail_ok = e0 == hipSuccess && e1 == hipSuccess && a >= 1 && b >= 1 ? 1 : 0;
+    }
+    return ctx->tail_ok != 0;
+}
+
+static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip, int levels,
+                     int pass)
 {
     hipStream_t st = m->ctx->stream;
+    const bool tails = ip->mode != KIFI_ICP_HORN && icp_tail_ok(m->ctx);
+    const unsigned spin_limit = ip->spin_limit > 0 ? (unsigned)ip->spin_limit : 1u << 18;
     // Horn mode is the reference's ICP::Align: full resolution only
     for (int l = ip->mode == KIFI_ICP_HORN ? 0 : levels - 1; l >= 0; --l) {
         const IcpSolveParams sp = solve_params(ip, l);
         float gate;
         const IcpLevel lv = icp_level(w, cam, ip, l, &gate);
         const int nb = icp_blocks(m->ctx, lv);
-        for (int it = 0; it < ip->iters[l]; ++it) {
+        const int launched = tails ? std::min(ip->iters[l], kLaunchedIters[l]) : ip->iters[l];
+        for (int it = 0; it < launched; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
                 const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, std::max(1, m->ctx->num_cus));
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
@@ -1523,6 +1601,11 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
                                        lv, m->mp, m->d_pose, m->d_pts, m->d_nrm, w->d_state, gate,
                                        (CSum *)w->partials, l, sp);
             }
+        }
+        if (launched < ip->iters[l]) {
+            auto k = ip->naive_sums ? k_icp_level_tail<false> : k_icp_level_tail<true>;
+            hipLaunchKernelGGL(k, dim3(nb), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts, m->d_nrm, w->d_state,
+                               gate, (CSum *)w->partials, l, sp, ip->iters[l] - launched, spin_limit, pass * 3 + l);
         }
     }
 }

@@ -58,6 +58,7 @@ struct IcpState {
     unsigned ticket;        // workgroups done in the current association launch
     unsigned gen;           // iterations solved by multi-iteration launches (monotone)
     unsigned spin_timeouts; // waits of a multi-iteration launch that gave up (0 unless broken)
+    unsigned tail_gen[6];   // hand-offs of each level-tail launch of the frame (pass x 3 + level)
     double sums[30];        // reduced sums of the last association (icp_finish layout)
     double prior_acc[6];    // motion since the pass's start: rotation, camera-centre displacement
     double prior_c[3];      // camera centre of the pass's start pose

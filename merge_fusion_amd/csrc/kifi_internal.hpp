@@ -221,6 +221,7 @@ struct kifi_ctx {
     int          stage_slot = 0;
     int          num_cus = 0;
     int          multi_ok = -1;    // multi-iteration ICP launch fits one workgroup per CU (-1: not yet asked)
+    int          tail_ok = -1;     // the same for the level-tail launch
     // stage timers (HIP event pairs recorded on `stream` while enabled)
     unsigned     timing_mask = 0;   // bit s: time stage s
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timers[KIFI_STAGE_COUNT];
