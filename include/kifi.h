@@ -98,7 +98,11 @@ kifi_status kifi_volume_load(kifi_ctx *ctx, const char *path, kifi_volume **out)
  * so the caller may refill or free it at once (the reference's Integrate takes
  * the frame by const reference and returns when done with it).  The copy of a
  * frame waits for the kernels of the frame two before it (the same staging
- * buffer), so the host runs at most two frames ahead of the device. */
+ * buffer), so the host runs at most two frames ahead of the device.  Device
+ * frames: a call waits for the integration four before it on the same volume
+ * to finish, so the host runs at most four frames ahead; the volume's
+ * capacity keeps room for that many frames of growth (8 voxels per pixel
+ * each), so integration does not stall the stream to check it. */
 kifi_status kifi_integrate(kifi_volume *vol, const uint16_t *depth_mm, const kifi_intrinsics *cam,
                            const float worldToEye[16]);
 kifi_status kifi_integrate_device(kifi_volume *vol, const uint16_t *depth_mm_dev,

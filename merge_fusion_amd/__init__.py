@@ -10,6 +10,7 @@ worldToEye), Mesher.mesh(volume[, with indices]).
 There is no CPU fallback: if the HIP library is not built or no GPU is
 present, every call raises.
 """
+import atexit
 import ctypes
 import os
 import weakref
@@ -287,6 +288,21 @@ def invert_rigid(m):
 
 
 # ---------------------------------------------------------------- HIP objects
+# Contexts still open when the interpreter exits are closed by an exit hook, in
+# order (children, host buffers, then the context), before module teardown
+# starts dropping objects in an arbitrary order.
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     """kifi_ctx: one HIP stream on one device."""
 
@@ -300,6 +316,10 @@ class Context:
         self._lib = lib
         # objects created on this context; closed before the context is destroyed
         self._children = weakref.WeakSet()
+        # page-locked host buffers by address, freed when their arrays are gone
+        # or when the context closes
+        self._host = {}
+        _live_contexts.add(self)
 
     def _adopt(self, obj):
         self._children.add(obj)
@@ -325,6 +345,9 @@ class Context:
                     obj.close()
                 except Exception:
                     pass
+            for buf in list(self._host.values()):
+                buf.free()
+            self._host.clear()
             self._lib.kifi_ctx_destroy(self._h)
             self._h = None
 
@@ -348,18 +371,24 @@ class Context:
         return ms.value, n.value
 
     def host_empty(self, shape, dtype):
-        """A numpy array in page-locked host memory (kifi_host_alloc), freed with the
-        context or when the array and its views are gone."""
+        """A numpy array in page-locked host memory (kifi_host_alloc), freed when the
+        array and its views are gone, or when the context closes (the array must
+        not be used after that)."""
         dt = np.dtype(dtype)
         nbytes = int(np.prod(shape)) * dt.itemsize
         p = ctypes.c_void_p()
         self.check(self._lib.kifi_host_alloc(self._h, nbytes, ctypes.byref(p)), "kifi_host_alloc")
-        buf = HostBuffer(self, p, nbytes)
-        self._adopt(buf)
+        self._host[p.value] = HostBuffer(self, p, nbytes)
         raw = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
         arr = np.frombuffer(raw, dtype=dt, count=int(np.prod(shape))).reshape(shape)
-        buf.keep = arr
+        # every view of arr keeps raw alive: free the buffer once the last one goes
+        weakref.finalize(raw, self._release_host, p.value)
         return arr
+
+    def _release_host(self, addr):
+        buf = self._host.pop(addr, None)
+        if buf is not None:
+            buf.free()
 
     def device_upload(self, array):
         """Copy a numpy array into a new device buffer; returns DeviceBuffer."""

@@ -90,8 +90,37 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed)
     }
     vol->cur = 0;
     vol->cap = cap;
-    vol->size_bound = n;
+    ring_settle(vol, n);
     return KIFI_OK;
+}
+
+void ring_refresh(kifi_volume *v)
+{
+    for (;;) {
+        int s = -1;
+        for (int i = 0; i < kifi_volume::kSizeRing; ++i)
+            if (v->ring_frame[i] && (s < 0 || v->ring_frame[i] < v->ring_frame[s]))
+                s = i;
+        if (s < 0 || hipEventQuery(v->ring_ev[s]) != hipSuccess)
+            return;
+        const FrameCounters &c = v->h_ring[s];
+        if (c.overflow) {
+            v->ring_overflow = true;     // the volume kept its size; known_size stays a bound
+        } else if (v->ring_frame[s] > v->known_frame) {
+            v->known_size = (size_t)c.v_new;
+            v->known_frame = v->ring_frame[s];
+        }
+        v->ring_frame[s] = 0;
+    }
+}
+
+void ring_settle(kifi_volume *v, size_t n)
+{
+    ring_refresh(v);                      // keeps the overflow flags of the frames just finished
+    for (int i = 0; i < kifi_volume::kSizeRing; ++i)
+        v->ring_frame[i] = 0;
+    v->known_size = n;
+    v->known_frame = v->frames_enq;
 }
 
 // Segmented scans and reduction with the workgroup primitives every kernel
@@ -406,20 +435,22 @@ kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *d, kifi_vo
     v->consts.vlInv = 1.0f / v->consts.vl;
     v->consts.minOverNegVl = v->consts.vlInv * -minx;
     v->consts.trunc = d->trunc;
+    const uint64_t zero = 0;
     hipError_t e = hipMalloc(&v->d_counters, sizeof(FrameCounters));
-    if (e == hipSuccess) e = hipHostMalloc(&v->h_counters, sizeof(FrameCounters),
-                                                 hipHostMallocMapped | hipHostMallocCoherent);
-    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&v->h_counters_dev, v->h_counters, 0);
+    if (e == hipSuccess) e = hipHostMalloc(&v->h_ring, kifi_volume::kSizeRing * sizeof(FrameCounters),
+                                           hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&v->h_ring_dev, v->h_ring, 0);
     if (e == hipSuccess) e = hipMalloc(&v->d_size, 8);
     if (e == hipSuccess) e = hipMalloc(&v->d_mats, sizeof(FrameMats));
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&v->size_event, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMemcpy(v->d_size, &v->size_bound, 8, hipMemcpyHostToDevice);
+    for (int i = 0; i < kifi_volume::kSizeRing && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&v->ring_ev[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemcpy(v->d_size, &zero, 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(v->d_counters, 0, sizeof(FrameCounters));
     if (e != hipSuccess) {
         kifi_volume_destroy(v);
         return hip_error(ctx, e, "kifi_volume_create");
     }
-    std::memset(v->h_counters, 0, sizeof(FrameCounters));
+    std::memset(v->h_ring, 0, kifi_volume::kSizeRing * sizeof(FrameCounters));
     kifi_status st = grow_volume(v, (size_t)1 << 20);
     if (st != KIFI_OK) {
         kifi_volume_destroy(v);
@@ -440,10 +471,11 @@ void kifi_volume_destroy(kifi_volume *v)
         if (v->vals[b]) (void)hipFree(v->vals[b]);
     }
     if (v->d_counters) (void)hipFree(v->d_counters);
-    if (v->h_counters) (void)hipHostFree(v->h_counters);
+    if (v->h_ring) (void)hipHostFree(v->h_ring);
     if (v->d_size) (void)hipFree(v->d_size);
     if (v->d_mats) (void)hipFree(v->d_mats);
-    if (v->size_event) (void)hipEventDestroy(v->size_event);
+    for (hipEvent_t ev : v->ring_ev)
+        if (ev) (void)hipEventDestroy(ev);
     delete v;
 }
 
@@ -451,13 +483,13 @@ static kifi_status sync_size(kifi_volume *v, uint64_t *n)
 {
     kifi_ctx *ctx = v->ctx;
     KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (v->size_event_pending) {
-        v->size_event_pending = false;
-        if (v->h_counters->overflow)
-            return set_error(ctx, KIFI_ERR_STATE, "device-side capacity overflow in the last integration");
-    }
+    ring_refresh(v);
+    const bool overflow = v->ring_overflow;
+    v->ring_overflow = false;
     KIFI_HIP(ctx, hipMemcpy(n, v->d_size, 8, hipMemcpyDeviceToHost));
-    v->size_bound = *n;
+    ring_settle(v, (size_t)*n);
+    if (overflow)
+        return set_error(ctx, KIFI_ERR_STATE, "device-side capacity overflow in an integration since the last check");
     return KIFI_OK;
 }
 
@@ -469,8 +501,7 @@ kifi_status kifi_volume_clear(kifi_volume *v)
     KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t zero = 0;
     KIFI_HIP(ctx, hipMemcpy(v->d_size, &zero, 8, hipMemcpyHostToDevice));
-    v->size_bound = 0;
-    v->size_event_pending = false;
+    ring_settle(v, 0);
     return KIFI_OK;
 }
 
@@ -521,8 +552,7 @@ kifi_status kifi_volume_upload(kifi_volume *v, const void *keys, const void *val
     }
     uint64_t s = n;
     KIFI_HIP(ctx, hipMemcpy(v->d_size, &s, 8, hipMemcpyHostToDevice));
-    v->size_bound = n;
-    v->size_event_pending = false;
+    ring_settle(v, n);
     return KIFI_OK;
 }
 

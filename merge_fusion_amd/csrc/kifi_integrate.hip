@@ -1116,24 +1116,31 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     const int passes = (2 * KeyTraits<K>::kb + res_bits + DB - 1) / DB;
     const size_t Tb = (8 * P + kRsTile - 1) / kRsTile;
 
-    // capacity: the volume can grow by at most 8P voxels per frame
-    if (vol->size_event_pending && hipEventQuery(vol->size_event) == hipSuccess) {
-        vol->size_bound = vol->h_counters->overflow ? vol->size_bound : (size_t)vol->h_counters->v_new;
-        vol->size_event_pending = false;
-    }
-    if (vol->size_bound + 8 * P > vol->cap) {
+    // Capacity: the volume can grow by at most 8P voxels per frame.  This
+    // frame's ring slot is free once the frame that used it kSizeRing frames
+    // ago has finished (the wait leaves the frames after it queued).
+    constexpr int kRing = kifi_volume::kSizeRing;
+    const int slot = (int)(vol->frames_enq % kRing);
+    if (vol->ring_frame[slot])
+        KIFI_HIP(ctx, hipEventSynchronize(vol->ring_ev[slot]));
+    ring_refresh(vol);
+    size_t bound = vol->known_size + 8 * P * (size_t)(vol->frames_enq - vol->known_frame);   // V0 <= bound
+    if (bound + 8 * P > vol->cap) {
         KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
         uint64_t exact = 0;
         KIFI_HIP(ctx, hipMemcpy(&exact, vol->d_size, 8, hipMemcpyDeviceToHost));
-        vol->size_bound = exact;
-        vol->size_event_pending = false;
-        if (exact + 8 * P > vol->cap) {
-            kifi_status st = grow_volume(vol, exact + 8 * P);
+        ring_settle(vol, exact);
+        bound = exact;
+        if (exact + 8 * P * kRing > vol->cap) {
+            // room for a full ring of frames in flight, so that the bound does
+            // not force another synchronisation before the ring throttles
+            // (the capacity grows by factors of 1.5, so such syncs stay rare)
+            kifi_status st = grow_volume(vol, exact + 8 * P * (kRing + 1));
             if (st != KIFI_OK)
                 return st;
         }
     }
-    const size_t Tm = (vol->size_bound + 8 * P + kMgTile - 1) / kMgTile + 1;
+    const size_t Tm = (bound + 8 * P + kMgTile - 1) / kMgTile + 1;
 
     // scratch: integration buffers + merge partitions/dups
     const IntegrateScratch probe = layout_scratch(nullptr, P, sizeof(K), passes, ND);
@@ -1204,7 +1211,7 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                        (const K *)vol->keys[cur], Ekeys, part_a, cnt, tile_dups);
     hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
-                       (uint64_t)vol->cap, vol->h_counters_dev);
+                       (uint64_t)vol->cap, vol->h_ring_dev + slot);
     UpdateParams up;
     up.mats = vol->d_mats;
     up.vl = vol->consts.vl;
@@ -1229,11 +1236,10 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
 
-    // k_merge_scan wrote the counters into host-mapped memory; the event marks
-    // when they may be read to refresh the host size bound
-    KIFI_HIP(ctx, hipEventRecord(vol->size_event, stream));
-    vol->size_event_pending = true;
-    vol->size_bound += 8 * P;
+    // k_merge_scan wrote the counters into the frame's host-mapped ring slot;
+    // the event marks when they may be read to refresh the host size bound
+    KIFI_HIP(ctx, hipEventRecord(vol->ring_ev[slot], stream));
+    vol->ring_frame[slot] = ++vol->frames_enq;
     return KIFI_OK;
 }
 

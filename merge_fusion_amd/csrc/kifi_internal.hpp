@@ -249,14 +249,25 @@ struct kifi_volume {
     void            *vals[2] = { nullptr, nullptr };
     int              cur = 0;         // which buffer pair holds the volume
     size_t           cap = 0;         // capacity of each buffer (voxels)
-    size_t           size_bound = 0;  // host upper bound of the device-side size
     kifi::FrameCounters *d_counters = nullptr;   // device
-    kifi::FrameCounters *h_counters = nullptr;   // pinned, device-mapped host mirror
-    kifi::FrameCounters *h_counters_dev = nullptr;   // its device address
     uint64_t        *d_size = nullptr;           // device-resident volume size
     kifi::FrameMats *d_mats = nullptr;           // matrices of the last integration
-    hipEvent_t       size_event = nullptr;       // marks the last async size read-back
-    bool             size_event_pending = false;
+    // Size read-back without stalling the stream: each integration's last
+    // bookkeeping kernel copies the frame's counters into ring slot
+    // (frame % kSizeRing) of pinned, device-mapped host memory, and an event
+    // marks when the slot may be read.  The host bounds the device-side size by
+    // the newest size it has read plus 8P per frame enqueued after that one.  An
+    // integration reusing a slot first waits for the frame that last used it,
+    // so at most kSizeRing frames are in flight.
+    static constexpr int kSizeRing = 4;
+    kifi::FrameCounters *h_ring = nullptr;       // kSizeRing slots
+    kifi::FrameCounters *h_ring_dev = nullptr;   // their device address
+    hipEvent_t       ring_ev[kSizeRing] = {};
+    uint64_t         ring_frame[kSizeRing] = {}; // frame number written into the slot; 0: nothing pending
+    uint64_t         frames_enq = 0;             // integrations enqueued (frames numbered from 1)
+    uint64_t         known_frame = 0;            // the newest frame whose resulting size the host knows
+    size_t           known_size = 0;             // the volume size after known_frame
+    bool             ring_overflow = false;      // a completed frame reported a capacity overflow
 };
 
 namespace kifi {
@@ -271,6 +282,11 @@ kifi_status hip_error(kifi_ctx *ctx, hipError_t e, const char *what);
         if (e_ != hipSuccess)                                        \
             return ::kifi::hip_error((ctx), e_, #call);              \
     } while (0)
+
+// Takes the counters of every completed frame in the size ring, oldest first.
+void ring_refresh(kifi_volume *v);
+// After a stream synchronisation: every enqueued frame is done and the size is n.
+void ring_settle(kifi_volume *v, size_t n);
 
 // scratch allocators (grow; contents not preserved across a growth)
 kifi_status ensure_scratch(kifi_ctx *ctx, size_t bytes);
