@@ -228,9 +228,9 @@ __global__ void k_model_begin(Mat4 w2e_val, const Mat4 *__restrict__ w2e_dev, Ma
 }
 
 // ------------------------------------------------------------ depth pyramid
-// 2x2 mean of valid depths in the order (0,0),(1,0),(0,1),(1,1)
-// Levels 1 and 2 in one launch: the thread of a level-2 pixel computes its four
-// level-1 pixels (each exactly as k_depth_down does) and then itself from them.
+// A pixel of pyramid level l+1: the mean of the valid (> 0) depths of its 2x2
+// pixels at level l, summed in the order (0,0),(1,0),(0,1),(1,1); 0 if none is.
+// k_icp_begin builds the levels.
 __device__ __forceinline__ float mean_valid4(float a, float b, float c, float d)
 {
     float s = 0.0f, n = 0.0f;
@@ -244,48 +244,6 @@ __device__ __forceinline__ float mean_valid4(float a, float b, float c, float d)
     return n > 0.0f ? s / n : 0.0f;
 }
 
-__global__ void k_depth_down2(const float *__restrict__ l0, int W, int H, float *__restrict__ l1,
-                              float *__restrict__ l2)
-{
-    const int W1 = W / 2, W2 = W / 4, H2 = H / 4;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= W2 * H2)
-        return;
-    const int y = q / W2, x = q - y * W2;
-    float c1[4];
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int x1 = 2 * x + i, y1 = 2 * y + j;
-            const float *r0 = l0 + (size_t)(2 * y1) * W + 2 * x1, *r1 = r0 + W;
-            const float v = mean_valid4(r0[0], r0[1], r1[0], r1[1]);
-            l1[(size_t)y1 * W1 + x1] = v;
-            c1[2 * j + i] = v;
-        }
-    l2[q] = mean_valid4(c1[0], c1[1], c1[2], c1[3]);
-}
-
-__global__ void k_depth_down(const float *__restrict__ src, int W, int H, float *__restrict__ dst)
-{
-    const int w = W / 2, h = H / 2;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= w * h)
-        return;
-    const int y = q / w, x = q - y * w;
-    float s = 0.0f, c = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const float d = src[(size_t)(2 * y + j) * W + 2 * x + i];
-            if (d > 0.0f) {
-                s += d;
-                c += 1.0f;
-            }
-        }
-    dst[q] = c > 0.0f ? s / c : 0.0f;
-}
 
 // ------------------------------------------------------------ compensated sums
 struct CSum {
@@ -905,14 +863,48 @@ __device__ void icp_finish(const CSum *partials, int nblocks, IcpState *st, int 
 }
 
 // state control kernels
-// Frame start: the state from the guess (one thread) and the level-0 depth map
-// in metres (DepthStream.cpp:89-90), in one launch.
+// Level-1 pixel (x, y) of the pyramid straight from the raw frame: mean_valid4
+// of its 2x2 level-0 depths in metres (the values k_icp_begin stores at level 0).
 template <typename D>
-__global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val,
-                            const D *__restrict__ depth, int n, float *__restrict__ lv0)
+__device__ __forceinline__ float pyramid_l1(const D *depth, int W, int x, int y)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    const size_t r0 = (size_t)(2 * y) * W + 2 * x, r1 = r0 + W;
+    return mean_valid4(depth_m(depth, r0), depth_m(depth, r0 + 1), depth_m(depth, r1), depth_m(depth, r1 + 1));
+}
+
+// Frame start, in one launch: the state from the guess (one thread), the
+// level-0 depth map in metres (DepthStream.cpp:89-90) and, with LV levels, the
+// coarser ones (each pixel the mean_valid4 of its 2x2 pixels one level
+// finer).  Levels 1 and 2 are computed from the raw
+// frame, not from stored level-0 values, so nothing in the launch waits on
+// another thread's store: threads [0, n1) write level 1, threads [n1, n1 + n2)
+// level 2 (each recomputing its four level-1 values).  The grid has at least
+// W*H >= n1 + n2 threads.
+template <typename D, int LV>
+__global__ void k_icp_begin(IcpState *__restrict__ st, const double *__restrict__ guess_dev, Pose64 guess_val,
+                            const D *__restrict__ depth, int W, int H, float *__restrict__ lv0,
+                            float *__restrict__ lv1, float *__restrict__ lv2)
+{
+    const int n = W * H;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = gt; i < n; i += gridDim.x * blockDim.x)
         lv0[i] = depth_m(depth, i);
+    if (LV >= 2) {
+        const int W1 = W / 2, n1 = W1 * (H / 2);
+        if (gt < n1) {
+            const int y = gt / W1, x = gt - y * W1;
+            lv1[gt] = pyramid_l1(depth, W, x, y);
+        }
+        if (LV == 3) {
+            const int W2 = W / 4, q = gt - n1;
+            if (q >= 0 && q < W2 * (H / 4)) {
+                const int y = q / W2, x = q - y * W2;
+                lv2[q] = mean_valid4(pyramid_l1(depth, W, 2 * x, 2 * y), pyramid_l1(depth, W, 2 * x + 1, 2 * y),
+                                     pyramid_l1(depth, W, 2 * x, 2 * y + 1),
+                                     pyramid_l1(depth, W, 2 * x + 1, 2 * y + 1));
+            }
+        }
+    }
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     const double *g = guess_dev ? guess_dev : guess_val.m;
@@ -1263,6 +1255,13 @@ static void launch_model(kifi_model *m, kifi_volume *vol, hipStream_t st, bool s
 }
 
 // Model pose of this prediction and a cleared z-buffer.
+void model_prepare(kifi_model *m, kifi_volume *vol)
+{
+    m->mp.vl = vol->consts.vl;
+    m->mp.c0 = vol->consts.vlHalfPlusMin;
+    m->mp.dmax = vol->consts.vl;                 // near-surface band: |D| < one voxel
+}
+
 void model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev)
 {
     hipStream_t st = m->ctx->stream;
@@ -1272,9 +1271,7 @@ void model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const M
     Mat4 clip;
     eye_to_clip_rh(m->cam.width, m->cam.height, m->cam.fx, m->cam.fy, m->cam.cx, m->cam.cy, m->cam.znear,
                    m->cam.zfar, clip.m);
-    m->mp.vl = vol->consts.vl;
-    m->mp.c0 = vol->consts.vlHalfPlusMin;
-    m->mp.dmax = vol->consts.vl;                 // near-surface band: |D| < one voxel
+    model_prepare(m, vol);
     const int P = m->mp.W * m->mp.H;
     hipLaunchKernelGGL(k_model_begin, dim3((P + 255) / 256), dim3(256), 0, st, wv, w2e_dev, clip, m->d_pose,
                        m->d_zbuf, P);
@@ -1296,6 +1293,7 @@ SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev)
     sp.c0 = m->mp.c0;
     sp.dmax = m->mp.dmax;
     sp.znear = m->mp.znear;
+    sp.pose_out = nullptr;
     return sp;
 }
 
@@ -1454,6 +1452,22 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
 static bool icp_fallback_multi(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, const kifi_icp_params *ip,
                                int levels);
 
+template <typename D>
+static void launch_icp_begin(int levels, int P, hipStream_t st, IcpWork *w, const double *guess_dev, Pose64 gv,
+                             const D *depth, int W, int H)
+{
+    const dim3 grid((P + 255) / 256), block(256);
+    if (levels == 3)
+        hipLaunchKernelGGL((k_icp_begin<D, 3>), grid, block, 0, st, w->d_state, guess_dev, gv, depth, W, H, w->lv[0],
+                           w->lv[1], w->lv[2]);
+    else if (levels == 2)
+        hipLaunchKernelGGL((k_icp_begin<D, 2>), grid, block, 0, st, w->d_state, guess_dev, gv, depth, W, H, w->lv[0],
+                           w->lv[1], w->lv[2]);
+    else
+        hipLaunchKernelGGL((k_icp_begin<D, 1>), grid, block, 0, st, w->d_state, guess_dev, gv, depth, W, H, w->lv[0],
+                           w->lv[1], w->lv[2]);
+}
+
 // Runs the whole pyramid schedule asynchronously on the context's stream.
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip, Mat4 *w2e_out)
@@ -1466,19 +1480,11 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrins
     for (int i = 0; i < 16; ++i)
         gv.m[i] = guess_host ? guess_host[i] : 0.0;
     const int t_icp = timer_begin(ctx, KIFI_STAGE_ICP);
-    if (depth.metres)
-        hipLaunchKernelGGL(k_icp_begin<float>, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv,
-                           (const float *)depth.p, P, w->lv[0]);
-    else
-        hipLaunchKernelGGL(k_icp_begin<uint16_t>, dim3((P + 255) / 256), dim3(256), 0, st, w->d_state, guess_dev, gv,
-                           (const uint16_t *)depth.p, P, w->lv[0]);
     const int levels = ip->levels < 1 ? 1 : (ip->levels > 3 ? 3 : ip->levels);
-    if (levels == 3)
-        hipLaunchKernelGGL(k_depth_down2, dim3((W / 4 * (H / 4) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
-                           w->lv[1], w->lv[2]);
-    else if (levels == 2)
-        hipLaunchKernelGGL(k_depth_down, dim3((W / 2 * (H / 2) + 255) / 256), dim3(256), 0, st, w->lv[0], W, H,
-                           w->lv[1]);
+    if (depth.metres)
+        launch_icp_begin(levels, P, st, w, guess_dev, gv, (const float *)depth.p, W, H);
+    else
+        launch_icp_begin(levels, P, st, w, guess_dev, gv, (const uint16_t *)depth.p, W, H);
     // the first pass starts from the predicted pose when a fallback is configured,
     // and may stop its levels earlier than the fallback from the previous pose
     const bool with_fallback = ip->fallback_rms > 0.0f || ip->fallback_iters > 0;

@@ -10,13 +10,6 @@ struct Pose64 {
     double m[16];   // eye->world, column storage
 };
 
-// world->eye of the model camera and its centre in world coordinates
-struct ModelPose {
-    Mat4  w2e;
-    Mat4  proj;     // EyeToClipRH() * w2e (the reference's association projection)
-    float cam[3];
-};
-
 struct ModelParams {
     int   W, H;
     float fx, fy, cx, cy;   // model camera intrinsics (full resolution)
@@ -95,6 +88,9 @@ kifi_status model_alloc(kifi_model *m);
 void        model_free(kifi_model *m);
 kifi_status model_predict(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
 void        model_begin(kifi_model *m, kifi_volume *vol, const float *w2e_host, const Mat4 *w2e_dev);
+// The host side of model_begin only: an integration given model_splat_params
+// with pose_out set then writes the pose and clears the z-buffer itself.
+void        model_prepare(kifi_model *m, kifi_volume *vol);
 SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev);
 void        model_end(kifi_model *m, kifi_volume *vol, bool splat);
 kifi_status model_set_points(kifi_model *m, const float *xyz, const float *w2e, const kifi_intrinsics *proj_cam);

@@ -49,8 +49,11 @@ __device__ void clip_pre(const Mat4 &w2c, const VolumeConsts &vc, ClipPre &p)
 
 // Frame start: per-frame counters and radix totals cleared (a fill kernel, never
 // a memset node), and the frame's matrices from worldToEye (Integrator.cpp:41-42).
+// With model_out, also the model pose of the prediction this frame splats
+// (k_model_begin's, from the same matrices).
 __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, Mat4 clip, Mat4 w2e_val,
-                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc)
+                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc,
+                              ModelPose *__restrict__ model_out)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
         totals[i] = 0;
@@ -65,6 +68,15 @@ __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, 
     invert_transform(f.e2w.m);
     clip_pre(f.w2c, vc, f.pre);
     *out = f;
+    if (model_out) {
+        ModelPose mp;
+        mp.w2e = f.w2e;
+        mp.proj = f.w2c;
+        mp.cam[0] = f.e2w.m[12];
+        mp.cam[1] = f.e2w.m[13];
+        mp.cam[2] = f.e2w.m[14];
+        *model_out = mp;
+    }
 }
 
 // ==================================================================== keygen
@@ -73,6 +85,8 @@ struct KeygenParams {
     float flInvX, flInvY, ppX, ppY, maxIndex;
     VolumeConsts vc;
     const FrameMats *mats;   // device: e2w of this frame
+    unsigned long long *zbuf;   // when set: the model z-buffer of nz pixels, cleared to ~0 (a fill, never a memset)
+    int   nz;
 };
 
 // Integrator.cpp:41-42: worldToClip = EyeToClipRH() * worldToEye and
@@ -123,6 +137,9 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     __shared__ uint32_t base_s;
     const int P = p.W * p.H;
+    if (p.zbuf)
+        for (int i = blockIdx.x * kThreads + threadIdx.x; i < p.nz; i += gridDim.x * kThreads)
+            p.zbuf[i] = ~0ull;
     K keys[kKgItems];
     uint32_t flags = 0, c = 0;
     // every depth load in flight before the first use (a load under a branch
@@ -1160,8 +1177,9 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     eye_to_clip_rh(cam->width, cam->height, cam->fx, cam->fy, cam->cx, cam->cy, cam->znear, cam->zfar, clip.m);
     for (int i = 0; i < 16; ++i)
         wval.m[i] = w2e ? w2e[i] : 0.0f;
+    const bool model_start = splat && splat->pose_out;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * ND, clip, wval,
-                       w2e_dev, vol->d_mats, vol->consts);
+                       w2e_dev, vol->d_mats, vol->consts, model_start ? splat->pose_out : nullptr);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;
@@ -1172,6 +1190,8 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     kp.maxIndex = (float)(vol->desc.res - 1);
     kp.vc = vol->consts;
     kp.mats = vol->d_mats;
+    kp.zbuf = model_start ? splat->zbuf : nullptr;
+    kp.nz = model_start ? splat->W * splat->H : 0;
     const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);
     const dim3 kg_grid((unsigned)((P + kThreads * kKgItems - 1) / (kThreads * kKgItems)));
     const float *frame_m = d_depth.metres ? (const float *)d_depth.p : s.depth_m;

@@ -173,12 +173,23 @@ struct DepthIn {
 __host__ __device__ __forceinline__ float depth_m(const uint16_t *d, size_t i) { return (float)d[i] * 0.001f; }
 __host__ __device__ __forceinline__ float depth_m(const float *d, size_t i) { return d[i]; }
 
+// world->eye of the model camera and its centre in world coordinates
+struct ModelPose {
+    Mat4  w2e;
+    Mat4  proj;     // EyeToClipRH() * w2e (the reference's association projection)
+    float cam[3];
+};
+
 // Optional model splat folded into the merge + update voxel pass.
 struct SplatParams {
     const Mat4 *w2e;                 // device: world->eye of this frame
     unsigned long long *zbuf;        // W*H, cleared to ~0
     int   W, H;
     float fx, fy, cx, cy, vl, c0, dmax, znear;
+    // When set, the integration also starts the prediction: k_frame_begin
+    // writes the model pose of this frame (the camera is the integration's)
+    // and k_keygen clears zbuf.  Null when the caller has done both.
+    ModelPose *pose_out;
 };
 
 // Reset list for per-frame device counters (zeroed by a fill kernel, never by a

@@ -656,14 +656,15 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
     } else if (!tracked) {
         icp_w2e(&p->work, p->d_w2e, st);
     }
-    int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
-    model_begin(p->model, p->vol, nullptr, p->d_w2e);
-    timer_end(ctx, KIFI_STAGE_MODEL, t_m);
-    const SplatParams sp = model_splat_params(p->model, p->d_w2e);
+    // the model pose and the z-buffer clear ride on the integration's first
+    // kernels (its camera is the model's)
+    model_prepare(p->model, p->vol);
+    SplatParams sp = model_splat_params(p->model, p->d_w2e);
+    sp.pose_out = p->model->d_pose;
     s = integrate_dispatch(p->vol, din, &p->cam, nullptr, p->d_w2e, &sp);
     if (s != KIFI_OK)
         return s;
-    t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
+    const int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
     model_end(p->model, p->vol, false);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
     KIFI_HIP(ctx, hipGetLastError());
