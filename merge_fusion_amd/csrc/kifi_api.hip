@@ -67,15 +67,24 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed)
     uint64_t n = 0;
     KIFI_HIP(ctx, hipMemcpy(&n, vol->d_size, 8, hipMemcpyDeviceToHost));
     void *nk[2] = { nullptr, nullptr }, *nv[2] = { nullptr, nullptr };
+    void *ns1[2] = { nullptr, nullptr }, *ns2[2] = { nullptr, nullptr };
+    // key samples: every 16th and every 4096th key; the key buffers carry 32
+    // keys of padding (the integration reads aligned windows of up to 32 keys
+    // that start below the size)
+    const size_t s1b = ((cap + 15) / 16 + 1) * vol->key_bytes, s2b = ((cap + 4095) / 4096 + 1) * vol->key_bytes;
     for (int b = 0; b < 2; ++b) {
-        hipError_t e1 = hipMalloc(&nk[b], cap * vol->key_bytes);
-        hipError_t e2 = e1 == hipSuccess ? hipMalloc(&nv[b], cap * vol->val_bytes) : e1;
-        if (e2 != hipSuccess) {
+        hipError_t e = hipMalloc(&nk[b], (cap + 32) * vol->key_bytes);
+        if (e == hipSuccess) e = hipMalloc(&nv[b], cap * vol->val_bytes);
+        if (e == hipSuccess) e = hipMalloc(&ns1[b], s1b);
+        if (e == hipSuccess) e = hipMalloc(&ns2[b], s2b);
+        if (e != hipSuccess) {
             for (int q = 0; q <= b; ++q) {
                 if (nk[q]) (void)hipFree(nk[q]);
                 if (nv[q]) (void)hipFree(nv[q]);
+                if (ns1[q]) (void)hipFree(ns1[q]);
+                if (ns2[q]) (void)hipFree(ns2[q]);
             }
-            return hip_error(ctx, e2, "hipMalloc(volume)");
+            return hip_error(ctx, e, "hipMalloc(volume)");
         }
     }
     if (n) {
@@ -85,10 +94,15 @@ kifi_status grow_volume(kifi_volume *vol, size_t needed)
     for (int b = 0; b < 2; ++b) {
         if (vol->keys[b]) KIFI_HIP(ctx, hipFree(vol->keys[b]));
         if (vol->vals[b]) KIFI_HIP(ctx, hipFree(vol->vals[b]));
+        if (vol->samp1[b]) KIFI_HIP(ctx, hipFree(vol->samp1[b]));
+        if (vol->samp2[b]) KIFI_HIP(ctx, hipFree(vol->samp2[b]));
         vol->keys[b] = nk[b];
         vol->vals[b] = nv[b];
+        vol->samp1[b] = ns1[b];
+        vol->samp2[b] = ns2[b];
     }
     vol->cur = 0;
+    vol->samples_valid = false;
     vol->cap = cap;
     ring_settle(vol, n);
     return KIFI_OK;
@@ -469,6 +483,8 @@ void kifi_volume_destroy(kifi_volume *v)
     for (int b = 0; b < 2; ++b) {
         if (v->keys[b]) (void)hipFree(v->keys[b]);
         if (v->vals[b]) (void)hipFree(v->vals[b]);
+        if (v->samp1[b]) (void)hipFree(v->samp1[b]);
+        if (v->samp2[b]) (void)hipFree(v->samp2[b]);
     }
     if (v->d_counters) (void)hipFree(v->d_counters);
     if (v->h_ring) (void)hipHostFree(v->h_ring);
@@ -552,6 +568,7 @@ kifi_status kifi_volume_upload(kifi_volume *v, const void *keys, const void *val
     }
     uint64_t s = n;
     KIFI_HIP(ctx, hipMemcpy(v->d_size, &s, 8, hipMemcpyHostToDevice));
+    v->samples_valid = false;
     ring_settle(v, n);
     return KIFI_OK;
 }

@@ -1,6 +1,8 @@
 // kifi_integrate.hip -- per-frame fusion on the GPU:
-//   keygen (+2x2x2 neighbourhood) -> LSD radix sort -> unique (drop max)
-//   -> merge-path union with the volume fused with the weighted TSDF update.
+//   keygen (+2x2x2 neighbourhood) -> LSD radix sort -> locate (each unique
+//   candidate's rank in the volume; drop the maximum and the keys the volume
+//   holds) -> compact -> partition into merged tiles -> union with the volume
+//   fused with the weighted TSDF update.
 //
 // Reference path: Integrator::Integrate (KinectFission/kifi/Integrator.cpp:26-71).
 // The reference sorts the K pixel keys, uniques them and dilates them with
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
     // The 2x2x2 block of a key is its x+0 half (key + {0, dy, dz, dz+dy}) and its
     // x+1 half (the same + dx).  A half already in the left lane's block (the
     // previous pixel of the row, same key or one x step away) is not written:
-    // the sort and unique that follow see the same key set (every dropped key
+    // the sort and locate that follow see the same key set (every dropped key
     // is written by some lane further left), from fewer candidates.
     constexpr K dx = 1, dy = K(1) << KeyTraits<K>::kb, dz = K(1) << (2 * KeyTraits<K>::kb);
     const int lane = __lane_id();
@@ -384,226 +386,520 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     }
 }
 
-// ============================================================ unique (E keys)
-// Heads of runs of equal keys; the count of the last unique key is dropped
-// (the reference's x pass never emits the maximum, Integrator.cpp:231-245).
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_unique_count(const K *__restrict__ keys,
-                                                           const uint32_t *__restrict__ d_n,
-                                                           uint32_t *__restrict__ tile_counts)
+// ================================================= locate / compact / partition
+// The sorted candidates B (with repeats) against the volume A, three launches:
+//   k_locate:    per unique candidate b, r = #A <= b (util::set_union puts A
+//                first on ties, algorithm.h:138-198).  b is dropped when A
+//                holds it already or when it is the largest candidate (the
+//                reference's x pass never emits it, Integrator.cpp:231-245).
+//   k_compact:   the kept keys (the frame's new voxels) in key order, with
+//                their merged positions: the j-th kept key lands at r + j.
+//   k_partition: merged tiles of kMgTile outputs; tile t starts at old voxel
+//                kMgTile t - m and new voxel m, m = #(merged positions <
+//                kMgTile t).
+// Every search runs on a sorted array W with two sampled levels beside it,
+// S1[i] = W[i << SH1] and S2[i] = W[i << kS2Shift]: a search over S2 (a few KB,
+// cache resident) and two short windows, instead of a binary search of the
+// whole array whose every step waits for HBM.  The volume's keys keep every
+// 16th (kVolS1) and every 4096th key; the new voxels' merged positions every
+// 64th (kPosS1) and every 4096th.
+
+constexpr int kVolS1 = 4;
+constexpr int kKeyPad = 32;   // keys of padding past a volume key buffer's capacity
+constexpr int kPosS1 = 6;
+constexpr int kS2Shift = 12;
+
+template <int SH1>
+__host__ __device__ constexpr uint64_t s1_count(uint64_t n) { return (n + (1u << SH1) - 1) >> SH1; }
+__host__ __device__ constexpr uint64_t s2_count(uint64_t n) { return (n + 4095) >> kS2Shift; }
+
+// r[e] = #W[0, n) <= x[e] for n >= 1; NE independent searches interleaved
+template <typename K, int NE, int SH1>
+__device__ __forceinline__ void rank_le(const K *__restrict__ W, const K *__restrict__ S1, const K *__restrict__ S2,
+                                        uint64_t n, const K (&x)[NE], uint64_t (&r)[NE])
 {
-    __shared__ uint32_t tmp[kThreads / kWave + 1];
-    const uint32_t n = *d_n;
-    const size_t base = (size_t)blockIdx.x * kRsTile;
-    uint32_t c = 0;
-    if (base < n) {
-        const size_t mine = base + (size_t)threadIdx.x * kRsItems, last = (size_t)n - 1;
-        // all loads in flight together: clamped, not conditional
-        K prev = keys[mine > 0 ? min(mine - 1, last) : 0];
-        K kk[kRsItems];
+    constexpr uint64_t G1 = 1u << (kS2Shift - SH1), G0 = 1u << SH1;   // S1 per S2 entry, W per S1 entry
+    const uint64_t n2 = s2_count(n), n1 = s1_count<SH1>(n);
+    uint64_t top = 1;
+    while (top * 2 <= n2)
+        top *= 2;
+    uint64_t c[NE];
 #pragma unroll
-        for (int q = 0; q < kRsItems; ++q)
-            kk[q] = keys[min(mine + q, last)];
+    for (int e = 0; e < NE; ++e)
+        c[e] = 0;
+    // every probe is a load (clamped into the array), the step is a select:
+    // the NE searches keep their loads in flight together
+    for (uint64_t s = top; s; s >>= 1)
 #pragma unroll
-        for (int q = 0; q < kRsItems; ++q) {
-            const size_t i = mine + q;
-            c += i < n && (i == 0 || kk[q] != prev);
-            prev = kk[q];
+        for (int e = 0; e < NE; ++e) {
+            const K w = S2[min(c[e] + s, n2) - 1];
+            if (c[e] + s <= n2 && w <= x[e])
+                c[e] += s;
         }
+    // S1[G1 (c-1)] = S2[c-1] <= x < S2[c] = S1[G1 c]: the rest of that window
+    uint64_t w0[NE];
+    uint32_t len[NE], k[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const uint64_t cc = c[e] ? c[e] : 1;
+        w0[e] = G1 * (cc - 1) + 1;
+        len[e] = (uint32_t)(min(G1 * cc, n1) - w0[e]);
+        k[e] = 0;
     }
-    c = block_reduce_sum<kThreads>(c, tmp);
-    if (threadIdx.x == 0)
-        tile_counts[blockIdx.x] = c;
+#pragma unroll
+    for (uint32_t s = (uint32_t)G1 / 2; s; s >>= 1)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const K w = S1[w0[e] + min(k[e] + s, len[e]) - 1];   // S1[w0 - 1] when the window is empty
+            if (k[e] + s <= len[e] && w <= x[e])
+                k[e] += s;
+        }
+    // W[G0 (c1-1)] = S1[c1-1] <= x: the rest of that window of W
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const uint64_t c1 = w0[e] + k[e];
+        w0[e] = G0 * (c1 - 1) + 1;
+        len[e] = (uint32_t)(min(G0 * c1, n) - w0[e]);
+        k[e] = 0;
+    }
+#pragma unroll
+    for (uint32_t s = (uint32_t)G0 / 2; s; s >>= 1)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const K w = W[w0[e] + min(k[e] + s, len[e]) - 1];
+            if (k[e] + s <= len[e] && w <= x[e])
+                k[e] += s;
+        }
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+        r[e] = c[e] ? w0[e] + k[e] : 0;
 }
 
-__global__ __launch_bounds__(1024) void k_unique_scan(uint32_t *__restrict__ tile_counts, int Tb,
-                                                      FrameCounters *__restrict__ cnt)
-{
-    __shared__ uint32_t total;
-    block_scan_array<1024>(tile_counts, (size_t)Tb, &total);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        cnt->n_unique = total;
-        cnt->n_new = total ? total - 1 : 0;
-    }
-}
-
+// lo + #W[lo, hi) <= x, found by the 64 lanes of one wave together (x is
+// wave-uniform): each round probes 64 evenly spaced entries at once and keeps
+// the gap between the last probe <= x and the first one above it.
 template <typename K>
-__global__ __launch_bounds__(kThreads) void k_unique_scatter(const K *__restrict__ keys,
-                                                             const uint32_t *__restrict__ d_n,
-                                                             const uint32_t *__restrict__ tile_off,
-                                                             const FrameCounters *__restrict__ cnt,
-                                                             K *__restrict__ out)
+__device__ __forceinline__ uint64_t wave_count_le(const K *__restrict__ W, uint64_t lo, uint64_t hi, K x)
 {
+    const uint64_t lane = (uint64_t)__lane_id();
+    while (hi > lo) {
+        const uint64_t step = (hi - lo + 63) >> 6;
+        const uint64_t p = lo + (lane + 1) * step - 1;
+        const bool le = p < hi && W[p] <= x;
+        const uint64_t c = (uint64_t)popc64(__ballot(le));
+        if (step == 1)
+            return lo + c;
+        const uint64_t nlo = lo + c * step;
+        hi = min(hi, lo + (c + 1) * step - 1);
+        lo = nlo;
+    }
+    return lo;
+}
+
+// #W[0, n) <= x for n >= 1 through the two sampled levels, one wave
+template <typename K, int SH1>
+__device__ __forceinline__ uint64_t wave_rank_le(const K *__restrict__ W, const K *__restrict__ S1,
+                                                 const K *__restrict__ S2, uint64_t n, K x)
+{
+    constexpr uint64_t G1 = 1u << (kS2Shift - SH1), G0 = 1u << SH1;
+    const uint64_t c2 = wave_count_le(S2, 0, s2_count(n), x);
+    if (c2 == 0)
+        return 0;
+    const uint64_t c1 = wave_count_le(S1, G1 * (c2 - 1) + 1, min(G1 * c2, s1_count<SH1>(n)), x);
+    return wave_count_le(W, G0 * (c1 - 1) + 1, min(G0 * c1, n), x);
+}
+
+// dst[0, len) = src[0, len) for len <= kThreads * R through LDS: every load in
+// flight at once (one HBM round trip; a load-store loop would wait for each)
+template <typename K, int R>
+__device__ __forceinline__ void lds_fill(K *dst, const K *__restrict__ src, uint32_t len)
+{
+    if (len == 0)
+        return;
+    K t[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+        if ((uint32_t)q * kThreads < len)   // workgroup-uniform: only the rounds len needs
+            t[q] = src[min(q * kThreads + threadIdx.x, len - 1)];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+        if (q * kThreads + threadIdx.x < len)
+            dst[q * kThreads + threadIdx.x] = t[q];
+}
+
+// c[e] = #win[0, len) <= x[e] for every e in `mask` (win sorted in LDS;
+// power-of-two steps from top, the largest power of two <= len); the NE
+// searches interleave
+template <typename K, int NE>
+__device__ __forceinline__ void lds_count_le(const K *win, uint32_t len, uint32_t top, const K (&x)[NE],
+                                             uint32_t mask, uint32_t (&c)[NE])
+{
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+        c[e] = 0;
+    if (len == 0)
+        return;
+    // every probe is a read (clamped into the window) and the step a select:
+    // the NE reads of a step are in flight together, where a read under a
+    // branch would be waited for on its own
+    for (uint32_t st = top; st; st >>= 1) {
+        K w[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            w[e] = win[min(c[e] + st, len) - 1];
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            c[e] += ((mask >> e & 1u) && c[e] + st <= len && w[e] <= x[e]) ? st : 0u;
+    }
+}
+
+// (number of keys <= x in the window W[w0, w0 + 16 NW), whether x is one of
+// them); keys at nA or beyond do not count.  The window is read as aligned
+// 16-byte vectors: w0 is a multiple of 16 keys and the key buffers hold
+// kKeyPad keys of padding past their capacity.
+template <typename K, int NW, int NE>
+__device__ __forceinline__ void window_count(const K *__restrict__ A, uint64_t w0, uint64_t nA, const K (&x)[NE],
+                                             uint32_t mask, uint32_t (&cnt)[NE], uint32_t &eqm)
+{
+    constexpr int NV = NW * 16 * (int)sizeof(K) / 16;
+    uint4 v[NV];
+    const uint4 *vp = (const uint4 *)(A + w0);
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        v[q] = vp[q];
+    K wk[NW * 16];
+    __builtin_memcpy(wk, v, sizeof(v));
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        uint32_t c = 0;
+        bool eq = false;
+#pragma unroll
+        for (int q = 0; q < NW * 16; ++q) {
+            const bool in = w0 + q < nA;
+            c += in && wk[q] <= x[e];
+            eq |= in && wk[q] == x[e];
+        }
+        cnt[e] = c;
+        if ((mask >> e & 1u) && eq)
+            eqm |= 1u << e;
+    }
+}
+
+// Chunks of kRsTile sorted candidates; candidate e * kThreads + t of a chunk
+// belongs to thread t (so at every step of the lookups below the 64 lanes of a
+// wave hold neighbouring candidates, whose LDS probes fall on neighbouring
+// words).  Two waves bound the volume ranks of the chunk's first and last
+// candidate through the sampled levels (c1 = #S1 <= x); every rank lies in
+// A[16 (c1_first - 1), 16 c1_last).  When that stretch fits the LDS window it
+// is loaded there in one round and each candidate is looked up in LDS.
+// Otherwise the stretch's every-16th keys go through LDS (in pages) and each
+// candidate reads the 16-key volume window its rank lies in.  The kept keys
+// and their ranks are compacted, in key order, within the chunk's own region
+// of Bc / Rc.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_locate(const K *__restrict__ B, const uint32_t *__restrict__ d_n,
+                                                     const K *__restrict__ A, const K *__restrict__ AS1,
+                                                     const K *__restrict__ AS2, const uint64_t *__restrict__ d_size,
+                                                     FrameCounters *__restrict__ cnt, K *__restrict__ Bc,
+                                                     uint64_t *__restrict__ Rc, uint32_t *__restrict__ chunk_kept,
+                                                     uint32_t *__restrict__ chunk_heads)
+{
+    constexpr uint32_t WIN = 32768 / sizeof(K);
+    constexpr int RF = WIN / kThreads;
+    constexpr int NWV = kThreads / kWave;
+    constexpr uint64_t G1 = 1u << (kS2Shift - kVolS1), G0 = 1u << kVolS1;
+    __shared__ K win[WIN];
+    __shared__ uint64_t sh_c1[2];
+    __shared__ uint32_t sh_off[kRsItems * NWV + 1];
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const uint32_t n = *d_n;
+    const uint64_t nA = *d_size;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt->v_old = nA;
     const size_t base = (size_t)blockIdx.x * kRsTile;
     if (base >= n)
         return;
-    const uint32_t E = cnt->n_new;
-    const size_t mine = base + (size_t)threadIdx.x * kRsItems;
-    K loc[kRsItems];
-    uint32_t flags = 0, c = 0;
-    K prev = mine > 0 && mine - 1 < n ? keys[mine - 1] : K(0);
+    const size_t last = (size_t)n - 1;
+    const size_t tlast = min(base + kRsTile, (size_t)n) - 1;
+    const int w = threadIdx.x / kWave, lane = __lane_id();
+    // every load in flight together (clamped, not conditional); the two bound
+    // searches start at once, their first loads behind these
+    const K bmax = B[last];
+    K b[kRsItems], pv[kRsItems];
 #pragma unroll
-    for (int q = 0; q < kRsItems; ++q) {
-        const size_t i = mine + q;
-        K k = i < n ? keys[i] : K(0);
-        loc[q] = k;
-        const bool h = i < n && (i == 0 || k != prev);
-        flags |= (uint32_t)h << q;
-        c += h;
-        prev = k;
+    for (int e = 0; e < kRsItems; ++e) {
+        const size_t i = base + (size_t)e * kThreads + threadIdx.x;
+        b[e] = B[min(i, last)];
+        pv[e] = B[i > 0 ? min(i - 1, last) : 0];
     }
-    uint32_t tot;
-    uint32_t pos = tile_off[blockIdx.x] + block_exclusive_scan<kThreads>(c, tmp, tot);
+    if (nA > 0 && w < 2) {
+        const K x = B[w == 0 ? base : tlast];
+        const uint64_t c2 = wave_count_le(AS2, 0, s2_count(nA), x);
+        const uint64_t c1 = c2 ? wave_count_le(AS1, G1 * (c2 - 1) + 1, min(G1 * c2, s1_count<kVolS1>(nA)), x) : 0;
+        if (lane == 0)
+            sh_c1[w] = c1;
+    }
+    uint32_t look = 0, heads = 0;   // look bit e: a unique key other than the maximum
 #pragma unroll
-    for (int q = 0; q < kRsItems; ++q) {
-        if (flags >> q & 1u) {
-            if (pos < E)
-                out[pos] = loc[q];
-            ++pos;
+    for (int e = 0; e < kRsItems; ++e) {
+        const size_t i = base + (size_t)e * kThreads + threadIdx.x;
+        const bool head = i < n && (i == 0 || b[e] != pv[e]);
+        heads += head;
+        look |= (uint32_t)(head && b[e] != bmax) << e;
+    }
+    uint64_t r[kRsItems];
+#pragma unroll
+    for (int e = 0; e < kRsItems; ++e)
+        r[e] = 0;
+    uint32_t dupm = 0;   // candidates found in the volume (A wins)
+    if (nA > 0) {   // block-uniform
+        __syncthreads();
+        const uint64_t c1f = sh_c1[0], c1l = sh_c1[1];
+        const uint64_t ws = c1f ? G0 * (c1f - 1) : 0, we = min(G0 * c1l, nA);
+        const uint64_t len = we > ws ? we - ws : 0;   // A[ws] <= every candidate (c1f > 0), A[we] above them
+        if (len <= WIN) {   // block-uniform
+            lds_fill<K, RF>(win, A + ws, (uint32_t)len);
+            __syncthreads();
+            uint32_t top = 1;
+            while (2 * top <= (uint32_t)len)
+                top *= 2;
+            uint32_t c[kRsItems];
+            lds_count_le<K, kRsItems>(win, (uint32_t)len, top, b, look, c);
+            K at[kRsItems];   // the window key just below each rank (win[0] stands in when there is none)
+#pragma unroll
+            for (int e = 0; e < kRsItems; ++e)
+                at[e] = win[c[e] ? c[e] - 1 : 0];
+#pragma unroll
+            for (int e = 0; e < kRsItems; ++e) {
+                r[e] = ws + c[e];
+                if ((look >> e & 1u) && c[e] && at[e] == b[e])
+                    dupm |= 1u << e;
+            }
+        } else {
+            // S1[i_lo, c1l) through LDS, in pages of WIN: S1[i_lo] <= every
+            // candidate (c1f > 0) and S1[c1l] above them; c1 = #S1 <= b sums the
+            // pages' counts
+            const uint64_t i_lo = c1f ? c1f - 1 : 0, ns = c1l - i_lo;
+            if (ns <= 8 * (uint64_t)WIN) {   // block-uniform
+                uint32_t c[kRsItems];
+#pragma unroll
+                for (int e = 0; e < kRsItems; ++e)
+                    c[e] = 0;
+#pragma unroll 1
+                for (uint64_t p0 = 0; p0 < ns; p0 += WIN) {
+                    const uint32_t pl = (uint32_t)min(ns - p0, (uint64_t)WIN);
+                    if (p0)
+                        __syncthreads();   // the previous page's lookups are done
+                    // two rounds of half a window: a whole-window round here
+                    // would hold the kernel at half the occupancy
+                    for (uint32_t k0 = 0; k0 < pl; k0 += RF / 2 * kThreads)
+                        lds_fill<K, RF / 2>(win + k0, AS1 + i_lo + p0 + k0, min(pl - k0, (uint32_t)(RF / 2 * kThreads)));
+                    __syncthreads();
+                    uint32_t top = 1;
+                    while (2 * top <= pl)
+                        top *= 2;
+                    uint32_t cp[kRsItems];
+                    lds_count_le<K, kRsItems>(win, pl, top, b, look, cp);
+#pragma unroll
+                    for (int e = 0; e < kRsItems; ++e)
+                        c[e] += cp[e];
+                }
+                // c1 = #S1 <= b: the rank lies in A[16 (c1-1), 16 c1] (0 below A[0]);
+                // one 16-key window per candidate, GE candidates' reads at once
+                constexpr int GE = sizeof(K) == 4 ? 2 : 1;
+#pragma unroll
+                for (int e0 = 0; e0 < kRsItems; e0 += GE) {
+                    uint64_t w0[GE];
+                    uint4 v[GE][sizeof(K)];
+#pragma unroll
+                    for (int g = 0; g < GE; ++g) {
+                        const uint64_t c1 = i_lo + c[e0 + g];
+                        w0[g] = (look >> (e0 + g) & 1u) && c1 ? G0 * (c1 - 1) : 0;
+                        const uint4 *vp = (const uint4 *)(A + w0[g]);
+#pragma unroll
+                        for (int q = 0; q < (int)sizeof(K); ++q)
+                            v[g][q] = vp[q];
+                    }
+#pragma unroll
+                    for (int g = 0; g < GE; ++g) {
+                        K wk[16];
+                        __builtin_memcpy(wk, v[g], sizeof(wk));
+                        uint32_t cn = 0;
+                        bool eq = false;
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) {
+                            const bool in = w0[g] + q < nA;
+                            cn += in && wk[q] <= b[e0 + g];
+                            eq |= in && wk[q] == b[e0 + g];
+                        }
+                        r[e0 + g] = w0[g] + cn;
+                        if ((look >> (e0 + g) & 1u) && eq)
+                            dupm |= 1u << (e0 + g);
+                    }
+                    asm volatile("" ::: "memory");   // one group's windows in registers at a time
+                }
+            } else {
+                rank_le<K, kRsItems, kVolS1>(A, AS1, AS2, nA, b, r);
+                K at[kRsItems];
+#pragma unroll
+                for (int e = 0; e < kRsItems; ++e)
+                    at[e] = A[r[e] ? r[e] - 1 : 0];
+#pragma unroll
+                for (int e = 0; e < kRsItems; ++e)
+                    if (r[e] && at[e] == b[e])
+                        dupm |= 1u << e;
+            }
+        }
+    }
+    // compaction in candidate order e * kThreads + t: per (e, wave) counts by
+    // ballot, their exclusive scan, then each kept candidate's offset
+    const uint32_t keep = look & ~dupm;
+    uint64_t bal[kRsItems];
+#pragma unroll
+    for (int e = 0; e < kRsItems; ++e) {
+        bal[e] = __ballot(keep >> e & 1u);
+        if (lane == 0)
+            sh_off[e * NWV + w] = (uint32_t)popc64(bal[e]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int q = 0; q < kRsItems * NWV; ++q) {
+            const uint32_t t = sh_off[q];
+            sh_off[q] = acc;
+            acc += t;
+        }
+        sh_off[kRsItems * NWV] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kRsItems; ++e)
+        if (keep >> e & 1u) {
+            const uint32_t off = sh_off[e * NWV + w] + (uint32_t)popc64(bal[e] & lanemask_lt());
+            Bc[base + off] = b[e];
+            Rc[base + off] = r[e];
+        }
+    const uint32_t hsum = block_reduce_sum<kThreads>(heads, tmp);
+    if (threadIdx.x == 0) {
+        chunk_kept[blockIdx.x] = sh_off[kRsItems * NWV];
+        chunk_heads[blockIdx.x] = hsum;
+    }
+}
+
+// The kept keys of every chunk, in key order, with their merged positions (the
+// j-th kept key lands at its rank + j) and the positions' two sampled levels.
+// The last chunk (chunk 0 when there are no candidates) records the counts.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_compact(const K *__restrict__ Bc, const uint64_t *__restrict__ Rc,
+                                                      const uint32_t *__restrict__ d_n,
+                                                      const uint32_t *__restrict__ chunk_kept,
+                                                      const uint32_t *__restrict__ chunk_heads,
+                                                      FrameCounters *__restrict__ cnt, K *__restrict__ Bn,
+                                                      uint64_t *__restrict__ on, uint64_t *__restrict__ onS1,
+                                                      uint64_t *__restrict__ onS2)
+{
+    __shared__ uint32_t tmp[kThreads / kWave + 1];
+    const uint32_t n = *d_n;
+    const uint32_t nchunks = (n + kRsTile - 1) / kRsTile;
+    const uint32_t c = blockIdx.x;
+    const bool lastc = c + 1 == (nchunks ? nchunks : 1u);
+    const uint32_t kc = c < nchunks ? chunk_kept[c] : 0u;
+    if (kc == 0 && !lastc)
+        return;
+    const size_t base = (size_t)c * kRsTile;
+    // the chunk's first kept entries are loaded before the prefix is known
+    K kb = K(0);
+    uint64_t kr = 0;
+    if (threadIdx.x < kc) {
+        kb = Bc[base + threadIdx.x];
+        kr = Rc[base + threadIdx.x];
+    }
+    uint32_t s = 0;
+    for (uint32_t q = threadIdx.x; q < c; q += kThreads)
+        s += chunk_kept[q];
+    const uint32_t pre = block_reduce_sum<kThreads>(s, tmp);
+    for (uint32_t i = threadIdx.x; i < kc; i += kThreads) {
+        if (i >= kThreads) {
+            kb = Bc[base + i];
+            kr = Rc[base + i];
+        }
+        const uint32_t j = pre + i;
+        const uint64_t o = kr + j;
+        Bn[j] = kb;
+        on[j] = o;
+        if ((j & 63u) == 0)
+            onS1[j >> kPosS1] = o;
+        if ((j & 4095u) == 0)
+            onS2[j >> kS2Shift] = o;
+    }
+    if (lastc) {
+        uint32_t hs = 0;
+        for (uint32_t q = threadIdx.x; q < nchunks; q += kThreads)
+            hs += chunk_heads[q];
+        hs = block_reduce_sum<kThreads>(hs, tmp);
+        if (threadIdx.x == 0) {
+            cnt->n_kept = pre + kc;
+            cnt->n_unique = hs;
+            cnt->n_new = hs ? hs - 1 : 0;
         }
     }
 }
 
-// ======================================================== merge + update
-// util::set_union with values (algorithm.h:138-198) as flat_map::insert uses it
-// (flat_map.h:164-197): old volume A first on ties, new keys B get Voxel().
-// Merge path over output diagonals of kMgTile; B elements equal to an A element
-// (dups) are dropped, so each tile's output base is its diagonal minus the
-// dups of all earlier tiles.
-
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_merge_partition(const K *__restrict__ A,
-                                                              const uint64_t *__restrict__ d_nA,
-                                                              const K *__restrict__ B,
-                                                              FrameCounters *__restrict__ cnt,
-                                                              uint64_t *__restrict__ part_a, int Tbound)
+// One thread per tile boundary t in [0, Tbound]: m = #(merged positions of new
+// voxels < min(kMgTile t, V1)).  Thread 0 also closes the frame's counts: the
+// new size, the capacity checks, the running totals and the host ring slot.
+__global__ __launch_bounds__(kThreads) void k_partition(const uint64_t *__restrict__ on,
+                                                        const uint64_t *__restrict__ onS1,
+                                                        const uint64_t *__restrict__ onS2,
+                                                        FrameCounters *__restrict__ cnt,
+                                                        uint64_t *__restrict__ part_a,
+                                                        uint64_t *__restrict__ part_b, int Tbound,
+                                                        uint64_t *__restrict__ d_size, uint64_t cap,
+                                                        FrameCounters *__restrict__ host_cnt)
 {
-    const uint64_t nA = *d_nA, nB = cnt->n_new, tot = nA + nB;
-    const uint64_t T = (tot + kMgTile - 1) / kMgTile;
+    const uint64_t v0 = cnt->v_old, nk = cnt->n_kept, v1 = v0 + nk;
+    const uint64_t T = (v1 + kMgTile - 1) / kMgTile;
     const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (t == 0) {
-        cnt->v_old = nA;
-        cnt->n_tiles = (uint32_t)T;
-        if (T > (uint64_t)Tbound)
-            cnt->overflow = 1;
-    }
-    if (t > T || t > (uint64_t)Tbound)
-        return;
-    const uint64_t diag = min(t * kMgTile, tot);
-    uint64_t lo = diag > nB ? diag - nB : 0, hi = min(diag, nA);
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (A[mid] <= B[diag - 1 - mid])
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    part_a[t] = lo;
-}
-
-// Shared between the dup count and the merge: B[b0+j] is a dup when it equals
-// an A element of the same tile or the A element just before the tile.
-template <typename K>
-struct MergeTile {
-    uint64_t a0, a1, b0, b1;
-    uint32_t na, nb;
-};
-
-template <typename K>
-__device__ __forceinline__ MergeTile<K> merge_tile(const uint64_t *part_a, uint64_t t, uint64_t tot)
-{
-    MergeTile<K> mt;
-    const uint64_t d0 = t * kMgTile, d1 = min(d0 + kMgTile, tot);
-    mt.a0 = part_a[t];
-    mt.a1 = part_a[t + 1];
-    mt.b0 = d0 - mt.a0;
-    mt.b1 = d1 - mt.a1;
-    mt.na = (uint32_t)(mt.a1 - mt.a0);
-    mt.nb = (uint32_t)(mt.b1 - mt.b0);
-    return mt;
-}
-
-// Per tile: the new keys equal to an old key of the tile or the one just
-// before it (ties put A first; keys are unique).  The tile's old keys are
-// loaded into LDS with every load issued up front, then each new key (spread
-// over all lanes) is looked up there.
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_merge_dups(const K *__restrict__ A,
-                                                         const K *__restrict__ B,
-                                                         const uint64_t *__restrict__ part_a,
-                                                         const FrameCounters *__restrict__ cnt,
-                                                         uint32_t *__restrict__ tile_dups)
-{
-    __shared__ K sA[kMgTile];
-    __shared__ uint32_t tmp[kThreads / kWave + 1];
-    const uint64_t T = cnt->n_tiles;
-    if (blockIdx.x >= T || cnt->overflow)
-        return;
-    const uint64_t tot = cnt->v_old + cnt->n_new;
-    const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
-    if (mt.nb == 0) {   // no new keys in the tile (most tiles of a grown volume): nothing to look up
-        if (threadIdx.x == 0)
-            tile_dups[blockIdx.x] = 0;
-        return;
-    }
-    constexpr int PER = kMgTile / kThreads;
-    K ak[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na)
-            ak[q] = A[mt.a0 + i];
-    }
-    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na)
-            sA[i] = ak[q];
-    }
-    __syncthreads();
-    uint32_t c = 0;
-    for (uint32_t j = threadIdx.x; j < mt.nb; j += kThreads) {
-        const K b = B[mt.b0 + j];
-        const uint32_t lb = lds_lower_bound(sA, mt.na, b);
-        c += (lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b);
-    }
-    c = block_reduce_sum<kThreads>(c, tmp);
-    if (threadIdx.x == 0)
-        tile_dups[blockIdx.x] = c;
-}
-
-__global__ __launch_bounds__(1024) void k_merge_scan(uint32_t *__restrict__ tile_dups,
-                                                     FrameCounters *__restrict__ cnt,
-                                                     uint64_t *__restrict__ d_size, uint64_t cap,
-                                                     FrameCounters *__restrict__ host_cnt)
-{
-    __shared__ uint64_t total;
-    const uint32_t T = cnt->n_tiles;
-    if (cnt->overflow) {
-        if (threadIdx.x == 0)
-            *host_cnt = *cnt;                // host-mapped: read after the frame's event
-        return;
-    }
-    block_scan_array<1024>(tile_dups, (size_t)T, &total);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint64_t v1 = cnt->v_old + cnt->n_new - total;
         cnt->v_new = v1;
-        if (v1 > cap) {
+        cnt->n_tiles = (uint32_t)min(T, (uint64_t)0xffffffffu);
+        if (v1 > cap || T > (uint64_t)Tbound) {
             cnt->overflow = 1;
         } else {
             *d_size = v1;
             cnt->tot_frames += 1;
-            cnt->tot_v_old += cnt->v_old;
+            cnt->tot_v_old += v0;
             cnt->tot_new += cnt->n_new;
             cnt->tot_v_new += v1;
             cnt->tot_keys += cnt->n_keys;
         }
-        *host_cnt = *cnt;
+        *host_cnt = *cnt;   // host-mapped: read after the frame's event
+    }
+    if (t > T || t > (uint64_t)Tbound)
+        return;
+    const uint64_t q = min(t * kMgTile, v1);
+    uint64_t m[1] = { 0 };
+    if (nk > 0 && q > 0) {
+        const uint64_t x[1] = { q - 1 };
+        rank_le<uint64_t, 1, kPosS1>(on, onS1, onS2, nk, x, m);
+    }
+    part_b[t] = m[0];
+    part_a[t] = q - m[0];
+}
+
+// The two sampled levels of the volume's keys, rebuilt after the host wrote
+// the volume (upload, load, growth); an integration keeps them current.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_build_samples(const K *__restrict__ A,
+                                                            const uint64_t *__restrict__ d_size,
+                                                            K *__restrict__ S1, K *__restrict__ S2)
+{
+    const uint64_t n = *d_size, n1 = s1_count<kVolS1>(n);
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n1; i += (uint64_t)gridDim.x * kThreads) {
+        const K k = A[i << kVolS1];
+        S1[i] = k;
+        if ((i & ((1u << (kS2Shift - kVolS1)) - 1)) == 0)
+            S2[i >> (kS2Shift - kVolS1)] = k;
     }
 }
 
@@ -878,13 +1174,13 @@ constexpr int kHistWords = kMgTile + kMgTile / 32;
 
 template <typename K, typename V, bool SPLAT>
 __global__ __launch_bounds__(kThreads) void k_merge_update(
-    const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B,
-    const uint64_t *__restrict__ part_a, const uint32_t *__restrict__ tile_base_dups,
-    const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, UpdateParams up,
-    SplatParams sp, FrameCounters *__restrict__ upd_cnt)
+    const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B, const uint64_t *__restrict__ on,
+    const uint64_t *__restrict__ part_a, const uint64_t *__restrict__ part_b,
+    const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, K *__restrict__ OS1,
+    K *__restrict__ OS2, UpdateParams up, SplatParams sp, FrameCounters *__restrict__ upd_cnt)
 {
     __shared__ K sA[kMgTile];
-    __shared__ uint32_t h[kHistWords];   // per A position: kept new keys sorting just below it
+    __shared__ uint32_t h[kHistWords];   // per A position: new keys sorting just below it
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     constexpr bool kF16 = sizeof(V) == 4;
     __shared__ float recip[kF16 ? 256 : 1];   // binary16 storage: R[w] = 1/(w + 1)
@@ -900,85 +1196,88 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     // so the two are not live together
     const Mat4 w2c = up.mats->w2c;
     const ClipPre pre = up.mats->pre;
-    const uint64_t tot = cnt->v_old + cnt->n_new;
     const uint64_t upd_end = cnt->v_new / 4 * 4;                   // Integrator.cpp:269
-    const MergeTile<K> mt = merge_tile<K>(part_a, blockIdx.x, tot);
-    const uint64_t obase = (uint64_t)blockIdx.x * kMgTile - tile_base_dups[blockIdx.x];
+    // the tile: merged outputs [obase, obase + kMgTile), old voxels [a0, a1),
+    // new voxels [b0, b1) (none of them in the volume, each with its merged
+    // position)
+    const uint64_t a0 = part_a[blockIdx.x], a1 = part_a[blockIdx.x + 1];
+    const uint64_t b0 = part_b[blockIdx.x], b1 = part_b[blockIdx.x + 1];
+    const uint64_t obase = (uint64_t)blockIdx.x * kMgTile;
+    // ranges that do not add up to the tile (never, unless the partition is
+    // broken) are not touched: no access outside the buffers
+    if (a1 < a0 || b1 < b0 || a1 > cnt->v_old || b1 > cnt->n_kept ||
+        (a1 - a0) + (b1 - b0) != min((uint64_t)kMgTile, cnt->v_new - obase))
+        return;
+    const uint32_t na = (uint32_t)(a1 - a0), nb = (uint32_t)(b1 - b0);
     // outputs by 32-bit offset from the tile's base (offsets < 2^13); updates
     // stop at offset ulim
     K *const Ot = O + obase;
     V *const Ovt = Ov + obase;
     const uint32_t ulim = upd_end > obase ? (uint32_t)min(upd_end - obase, (uint64_t)0xffffffffu) : 0u;
-    // Every global load of the tile is issued up front: old keys, new keys, then
-    // old values, which stay in registers until the old-voxel pass.  A lane has
-    // up to 3 x PER loads in flight, where a loop of load-use iterations would
-    // hold one.  Element i of either range belongs to lane i % kThreads, so the
-    // loads coalesce and a tile's few new keys spread over many lanes.
+    // Every global load of the tile is issued up front: old keys, new keys and
+    // their positions, then old values, which stay in registers until the
+    // old-voxel pass.  Element i of either range belongs to lane i % kThreads, so
+    // the loads coalesce and a tile's few new keys spread over many lanes.
     constexpr int PER = kMgTile / kThreads;
     K ak[PER], bk[PER];
+    uint32_t bp[PER];   // a new key's offset in the tile
     V av[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na)
-            ak[q] = A[mt.a0 + i];
+        if (i < na)
+            ak[q] = A[a0 + i];
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.nb)
-            bk[q] = B[mt.b0 + i];
+        if (i < nb) {
+            bk[q] = B[b0 + i];
+            bp[q] = (uint32_t)(on[b0 + i] - obase);
+        }
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na)
-            av[q] = Av[mt.a0 + i];
+        if (i < na)
+            av[q] = Av[a0 + i];
     }
-    const K before = mt.a0 > 0 ? A[mt.a0 - 1] : K(0);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na)
+        if (i < na)
             sA[i] = ak[q];
     }
     __syncthreads();
-    // New keys, kThreads per round: a key already in the volume is dropped (A wins
-    // ties; the A element just before the tile can match the first new key).  A
-    // kept key goes to obase + (kept keys before it) + (A keys below it).
-    uint32_t lbq[PER], keepm = 0, nbk = 0, nupd = 0;   // nupd: voxels this lane updated (A)
+    // New voxels: Voxel() default, then the update.  New key j of the tile at
+    // offset p has p - j old keys of the tile before it.
+    uint32_t lbq[PER], nupd = 0;   // nupd: voxels this lane updated (A)
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        lbq[q] = 0;
-        if ((uint32_t)q * kThreads < mt.nb) {                     // uniform over the block
-            const uint32_t j = q * kThreads + threadIdx.x;
-            bool keep = false;
-            if (j < mt.nb) {
-                const K b = bk[q];
-                const uint32_t lb = lds_lower_bound(sA, mt.na, b);
-                keep = !((lb < mt.na && sA[lb] == b) || (j == 0 && mt.a0 > 0 && before == b));
-                lbq[q] = lb;
+        const uint32_t j = q * kThreads + threadIdx.x;
+        lbq[q] = kMgTile;   // counts for no old voxel
+        // (the offset check holds for a correct partition; it keeps a broken
+        // one inside the tile)
+        if (j < nb && bp[q] < na + nb && bp[q] - j <= na) {
+            const uint32_t ol = bp[q];
+            lbq[q] = ol - j;
+            V val{};
+            float s;
+            if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, pre, s)) {
+                voxel_apply(val, s, recip);
+                ++nupd;
             }
-            uint32_t kept;
-            const uint32_t pos = nbk + block_exclusive_scan<kThreads>((uint32_t)keep, tmp, kept);
-            nbk += kept;
-            keepm |= (uint32_t)keep << q;
-            if (keep) {
-                // new voxel: Voxel() default, then update
-                const uint32_t ol = pos + lbq[q];
-                V val{};
-                float s;
-                if (ol < ulim && voxel_sample<K>(bk[q], up, w2c, pre, s)) {
-                    voxel_apply(val, s, recip);
-                    ++nupd;
-                }
-                Ot[ol] = bk[q];
-                Ovt[ol] = val;
-                if (SPLAT)
-                    splat_voxel<K, V>(bk[q], val, obase + ol, sp, *sp.w2e);
-            }
+            Ot[ol] = bk[q];
+            Ovt[ol] = val;
+            if ((ol & ((1u << kVolS1) - 1)) == 0)
+                OS1[(obase + ol) >> kVolS1] = bk[q];
+            if (((obase + ol) & 4095u) == 0)
+                OS2[(obase + ol) >> kS2Shift] = bk[q];
+            if (SPLAT)
+                splat_voxel<K, V>(bk[q], val, obase + ol, sp, *sp.w2e);
         }
     }
+    const uint32_t nbk = nb;
     // shift of A[i] = kept new keys below it = kept keys with lb <= i: an LDS
     // histogram of the lbs and an inclusive scan over the tile's A positions.
     // nbk is uniform; a tile that inserts nothing copies its old voxels in place.
@@ -988,7 +1287,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            if ((keepm >> q & 1u) && lbq[q] < mt.na)
+            if (lbq[q] < na)
                 atomicAdd(&h[hpad(lbq[q])], 1u);
         __syncthreads();
         uint32_t loc[PER], sum = 0;
@@ -1014,7 +1313,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
         pix[q] = -1;
-        if (i < mt.na && i + (nbk ? h[hpad(i)] : 0u) < ulim)
+        if (i < na && i + (nbk ? h[hpad(i)] : 0u) < ulim)
             pix[q] = voxel_pixel<K>(sA[i], up, w2c, pre);
     }
     // unconditional loads (pixel 0 stands in for "none"): a load under a branch
@@ -1027,7 +1326,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = q * kThreads + threadIdx.x;
-        if (i < mt.na) {
+        if (i < na) {
             const K k = sA[i];
             const uint32_t ol = i + (nbk ? h[hpad(i)] : 0u);
             V val = av[q];
@@ -1038,6 +1337,10 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             }
             Ot[ol] = k;
             Ovt[ol] = val;
+            if ((ol & ((1u << kVolS1) - 1)) == 0)
+                OS1[(obase + ol) >> kVolS1] = k;
+            if (((obase + ol) & 4095u) == 0)
+                OS2[(obase + ol) >> kS2Shift] = k;
             if (SPLAT)
                 cand |= (uint32_t)splat_candidate(val, sp) << q;
         }
@@ -1094,7 +1397,10 @@ struct IntegrateScratch {
     void     *candA, *candB;
     uint32_t *rs_counts;     // [digits][Tb]
     uint32_t *rs_totals;     // [passes][digits]
-    uint32_t *uq_tiles;      // [Tb]
+    uint32_t *chunk_kept;    // [Tb]
+    uint32_t *chunk_heads;   // [Tb]
+    uint64_t *rank;          // [8P] the kept candidates' volume ranks, per chunk
+    uint64_t *on, *onS1, *onS2;   // merged positions of the new voxels and their sampled levels
     float    *depth_m;       // [P] a uint16 frame in metres
     size_t    bytes;
 };
@@ -1111,7 +1417,12 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     s.candB = base + off; off += align256(ncand * kbytes);
     s.rs_counts = (uint32_t *)(base + off); off += align256((size_t)digits * Tb * 4);
     s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * digits * 4);
-    s.uq_tiles = (uint32_t *)(base + off); off += align256((Tb + 1) * 4);
+    s.chunk_kept = (uint32_t *)(base + off); off += align256(Tb * 4);
+    s.chunk_heads = (uint32_t *)(base + off); off += align256(Tb * 4);
+    s.rank = (uint64_t *)(base + off); off += align256(ncand * 8);
+    s.on = (uint64_t *)(base + off); off += align256(ncand * 8);
+    s.onS1 = (uint64_t *)(base + off); off += align256((s1_count<kPosS1>(ncand) + 1) * 8);
+    s.onS2 = (uint64_t *)(base + off); off += align256((s2_count(ncand) + 1) * 8);
     s.depth_m = (float *)(base + off); off += align256(P * 4);
     s.bytes = off;
     return s;
@@ -1159,16 +1470,16 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     }
     const size_t Tm = (bound + 8 * P + kMgTile - 1) / kMgTile + 1;
 
-    // scratch: integration buffers + merge partitions/dups
+    // scratch: integration buffers + merge partitions
     const IntegrateScratch probe = layout_scratch(nullptr, P, sizeof(K), passes, ND);
-    const size_t merge_bytes = align256((Tm + 2) * 8) + align256((Tm + 2) * 4);
+    const size_t merge_bytes = 2 * align256((Tm + 2) * 8);
     kifi_status st = ensure_scratch(ctx, probe.bytes + merge_bytes);
     if (st != KIFI_OK)
         return st;
     char *base = (char *)ctx->scratch;
     IntegrateScratch s = layout_scratch(base, P, sizeof(K), passes, ND);
     uint64_t *part_a = (uint64_t *)(base + s.bytes);
-    uint32_t *tile_dups = (uint32_t *)(base + s.bytes + align256((Tm + 2) * 8));
+    uint64_t *part_b = (uint64_t *)(base + s.bytes + align256((Tm + 2) * 8));
 
     FrameCounters *cnt = vol->d_counters;
     hipStream_t stream = ctx->stream;
@@ -1214,24 +1525,28 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
                            &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
         K *t = src; src = dst; dst = t;
     }
-    // unique (minus the maximum) -> E keys in dst
-    hipLaunchKernelGGL(k_unique_count<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
-                       &cnt->n_cand, s.uq_tiles);
-    hipLaunchKernelGGL(k_unique_scan, dim3(1), dim3(1024), 0, stream, s.uq_tiles, (int)Tb, cnt);
-    hipLaunchKernelGGL(k_unique_scatter<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
-                       &cnt->n_cand, s.uq_tiles, cnt, dst);
-    timer_end(ctx, KIFI_STAGE_SORT, t_sort);
-    const K *Ekeys = dst;
 
-    // merge + update into the other buffer pair
+    // new voxels -> dst, with their merged positions; tile partitions
     const int cur = vol->cur, nxt = cur ^ 1;
-    hipLaunchKernelGGL(k_merge_partition<K>, dim3((unsigned)((Tm + 1 + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, stream, (const K *)vol->keys[cur], vol->d_size, Ekeys, cnt,
-                       part_a, (int)Tm);
-    hipLaunchKernelGGL(k_merge_dups<K>, dim3((unsigned)Tm), dim3(kThreads), 0, stream,
-                       (const K *)vol->keys[cur], Ekeys, part_a, cnt, tile_dups);
-    hipLaunchKernelGGL(k_merge_scan, dim3(1), dim3(1024), 0, stream, tile_dups, cnt, vol->d_size,
-                       (uint64_t)vol->cap, vol->h_ring_dev + slot);
+    if (!vol->samples_valid) {
+        const size_t n1 = s1_count<kVolS1>(bound);
+        const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((n1 + kThreads - 1) / kThreads, 4096));
+        hipLaunchKernelGGL(k_build_samples<K>, dim3(g), dim3(kThreads), 0, stream, (const K *)vol->keys[cur],
+                           vol->d_size, (K *)vol->samp1[cur], (K *)vol->samp2[cur]);
+        vol->samples_valid = true;
+    }
+    // the kept keys per chunk go to dst (free after the sort), then compacted
+    // into src (the sorted candidates are not needed after k_locate)
+    hipLaunchKernelGGL(k_locate<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, &cnt->n_cand,
+                       (const K *)vol->keys[cur], (const K *)vol->samp1[cur], (const K *)vol->samp2[cur],
+                       vol->d_size, cnt, dst, s.rank, s.chunk_kept, s.chunk_heads);
+    hipLaunchKernelGGL(k_compact<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, dst, s.rank, &cnt->n_cand,
+                       s.chunk_kept, s.chunk_heads, cnt, src, s.on, s.onS1, s.onS2);
+    hipLaunchKernelGGL(k_partition, dim3((unsigned)((Tm + 1 + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream,
+                       s.on, s.onS1, s.onS2, cnt, part_a, part_b, (int)Tm, vol->d_size, (uint64_t)vol->cap,
+                       vol->h_ring_dev + slot);
+    timer_end(ctx, KIFI_STAGE_SORT, t_sort);
+    const K *Bnew = src;
     UpdateParams up;
     up.mats = vol->d_mats;
     up.vl = vol->consts.vl;
@@ -1246,17 +1561,19 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     const int t_mu = timer_begin(ctx, KIFI_STAGE_MERGE_UPDATE);
     if (splat)
         hipLaunchKernelGGL((k_merge_update<K, V, true>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
-                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
-                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, *splat, cnt);
+                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Bnew, s.on, part_a, part_b,
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], (K *)vol->samp1[nxt],
+                           (K *)vol->samp2[nxt], up, *splat, cnt);
     else
         hipLaunchKernelGGL((k_merge_update<K, V, false>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
-                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Ekeys, part_a, tile_dups,
-                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], up, SplatParams{}, cnt);
+                           (const K *)vol->keys[cur], (const V *)vol->vals[cur], Bnew, s.on, part_a, part_b,
+                           cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], (K *)vol->samp1[nxt],
+                           (K *)vol->samp2[nxt], up, SplatParams{}, cnt);
     timer_end(ctx, KIFI_STAGE_MERGE_UPDATE, t_mu);
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
 
-    // k_merge_scan wrote the counters into the frame's host-mapped ring slot;
+    // k_partition wrote the counters into the frame's host-mapped ring slot;
     // the event marks when they may be read to refresh the host size bound
     KIFI_HIP(ctx, hipEventRecord(vol->ring_ev[slot], stream));
     vol->ring_frame[slot] = ++vol->frames_enq;

@@ -204,7 +204,7 @@ struct FrameCounters {
     uint32_t n_tiles;     // merge tiles this frame
     uint32_t overflow;    // set when a device-side capacity check fails
     uint32_t n_upd;       // A: voxels whose TSDF the update changed
-    uint32_t pad_;
+    uint32_t n_kept;      // new voxels: V1 - V0
     // running totals over all integrations (never reset by a frame)
     uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys, tot_upd;
 };
@@ -258,6 +258,11 @@ struct kifi_volume {
     size_t           key_bytes = 4, val_bytes = 8;
     void            *keys[2] = { nullptr, nullptr };
     void            *vals[2] = { nullptr, nullptr };
+    // every 64th and every 4096th key of each key buffer (the integration's
+    // searches); valid for the current buffer unless the host wrote the volume
+    void            *samp1[2] = { nullptr, nullptr };
+    void            *samp2[2] = { nullptr, nullptr };
+    bool             samples_valid = false;
     int              cur = 0;         // which buffer pair holds the volume
     size_t           cap = 0;         // capacity of each buffer (voxels)
     kifi::FrameCounters *d_counters = nullptr;   // device
