@@ -279,6 +279,7 @@ __device__ __forceinline__ void cadd(CSum &x, float s2, float c2)
 // chain 6 x 2 x NV dependent permutes per wave (about 9 us for NV = 29), and a
 // single thread folding all slices about 5 us.  Result in lds.result.
 constexpr int kSlices = 8;
+static_assert(kIT / kSlices % 4 == 0, "a slice's rows split over four chains");
 template <int NV>
 struct CsumLds {
     CSum rows[kIT * NV];
@@ -296,20 +297,39 @@ __device__ __forceinline__ void block_csum(const CSum (&v)[NV], CsumLds<NV> &lds
         lds.rows[threadIdx.x * NV + j] = v[j];
     __syncthreads();
     if (threadIdx.x < kSlices * NV) {
+        // a slice's rows in kChains interleaved chains (row r to chain r % kChains):
+        // kChains LDS reads in flight per step instead of one dependent chain
+        constexpr int kChains = 4;
         const int j = threadIdx.x % NV, sl = threadIdx.x / NV;
-        CSum a = lds.rows[(sl * kRows) * NV + j];
-        for (int r = 1; r < kRows; ++r) {
-            const CSum b = lds.rows[(sl * kRows + r) * NV + j];
-            cadd<COMP>(a, b.s, b.c);
+        CSum a[kChains];
+#pragma unroll
+        for (int c = 0; c < kChains; ++c)
+            a[c] = lds.rows[(sl * kRows + c) * NV + j];
+#pragma unroll
+        for (int r = kChains; r < kRows; r += kChains) {
+            CSum b[kChains];
+#pragma unroll
+            for (int c = 0; c < kChains; ++c)
+                b[c] = lds.rows[(sl * kRows + r + c) * NV + j];
+#pragma unroll
+            for (int c = 0; c < kChains; ++c)
+                cadd<COMP>(a[c], b[c].s, b[c].c);
         }
-        lds.slices[sl * NV + j] = a;
+        cadd<COMP>(a[0], a[1].s, a[1].c);
+        cadd<COMP>(a[2], a[3].s, a[3].c);
+        cadd<COMP>(a[0], a[2].s, a[2].c);
+        lds.slices[sl * NV + j] = a[0];
     }
     __syncthreads();
     if (threadIdx.x < NV) {
-        CSum a = lds.slices[threadIdx.x];
+        CSum sv[kSlices];
+#pragma unroll
+        for (int sl = 0; sl < kSlices; ++sl)
+            sv[sl] = lds.slices[sl * NV + threadIdx.x];
+        CSum a = sv[0];
 #pragma unroll
         for (int sl = 1; sl < kSlices; ++sl)
-            cadd<COMP>(a, lds.slices[sl * NV + threadIdx.x].s, lds.slices[sl * NV + threadIdx.x].c);
+            cadd<COMP>(a, sv[sl].s, sv[sl].c);
         lds.result[threadIdx.x] = a;
     }
     __syncthreads();

@@ -231,16 +231,18 @@ __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ k
     constexpr int ND = 1 << DB;
     constexpr uint32_t MASK = ND - 1;
     __shared__ uint32_t hist[ND];
+    // the tile's keys are read before the count arrives (the candidate buffers
+    // hold whole tiles), so the two reads overlap
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    K kk[kRsItems];
+#pragma unroll
+    for (int j = 0; j < kRsItems; ++j)
+        kk[j] = keys[base + (size_t)j * kThreads + threadIdx.x];
     const uint32_t n = *d_n;
     for (int q = threadIdx.x; q < ND; q += kThreads)
         hist[q] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kRsTile;
     if (base < n) {
-        K kk[kRsItems];    // all loads in flight together: clamped, not conditional
-#pragma unroll
-        for (int j = 0; j < kRsItems; ++j)
-            kk[j] = keys[min(base + (size_t)j * kThreads + threadIdx.x, (size_t)n - 1)];
 #pragma unroll
         for (int j = 0; j < kRsItems; ++j) {
             const size_t i = base + (size_t)j * kThreads + threadIdx.x;
@@ -305,21 +307,30 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     __shared__ uint32_t gbase[ND];
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     __shared__ K stage[kRsTile];
-    const uint32_t n = *d_n;
     const size_t tbase = (size_t)blockIdx.x * kRsTile;
-    if (tbase >= n)
-        return;
     const int w = threadIdx.x / kWave, lane = __lane_id();
-    for (int q = lane; q < ND; q += kWave)
-        wcnt[w][q] = 0;
     // Each wave ranks its own contiguous 64 x kRsItems-key sub-tile, item by item, so the
-    // per-wave counters need no barrier (a wave's LDS operations are in order).
+    // per-wave counters need no barrier (a wave's LDS operations are in order).  The keys
+    // are read before the count arrives (the candidate buffers hold whole tiles).
     K keys[kRsItems];
     uint32_t rank[kRsItems];
     const size_t wbase = tbase + (size_t)w * (kWave * kRsItems);
 #pragma unroll
-    for (int j = 0; j < kRsItems; ++j)   // all loads in flight together: clamped, not conditional
-        keys[j] = in[min(wbase + (size_t)j * kWave + lane, (size_t)n - 1)];
+    for (int j = 0; j < kRsItems; ++j)
+        keys[j] = in[wbase + (size_t)j * kWave + lane];
+    // this thread's digits' totals and tile offsets, read with the keys
+    const int d0 = threadIdx.x * DPT;
+    uint32_t gtot[DPT], goff[DPT];
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) {
+        gtot[e] = totals[d0 + e];
+        goff[e] = offsets[(size_t)(d0 + e) * Tb + blockIdx.x];
+    }
+    const uint32_t n = *d_n;
+    if (tbase >= n)
+        return;
+    for (int q = lane; q < ND; q += kWave)
+        wcnt[w][q] = 0;
 #pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
         const size_t i = wbase + (size_t)j * kWave + lane;
@@ -340,8 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     {
         // thread t owns digits t*DPT .. t*DPT+DPT-1: per digit the waves'
         // exclusive offsets, the tile's digit starts and the global bases
-        const int d0 = threadIdx.x * DPT;
-        uint32_t cnt[DPT], tsum = 0, gsum = 0, gtot[DPT];
+        uint32_t cnt[DPT], tsum = 0, gsum = 0;
 #pragma unroll
         for (int e = 0; e < DPT; ++e) {
             uint32_t acc = 0;
@@ -353,7 +363,6 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
             }
             cnt[e] = acc;
             tsum += acc;
-            gtot[e] = totals[d0 + e];
             gsum += gtot[e];
         }
         uint32_t tot;
@@ -363,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
         for (int e = 0; e < DPT; ++e) {
             const int d = d0 + e;
             dstart[d] = ex;
-            gbase[d] = dbase + offsets[(size_t)d * Tb + blockIdx.x] - ex;
+            gbase[d] = dbase + goff[e] - ex;
             ex += cnt[e];
             dbase += gtot[e];
         }
@@ -1413,8 +1422,9 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     const size_t ncand = 8 * P;
     const size_t Tb = (ncand + kRsTile - 1) / kRsTile;
     size_t off = 0;
-    s.candA = base + off; off += align256(ncand * kbytes);
-    s.candB = base + off; off += align256(ncand * kbytes);
+    // whole tiles: the sort reads a tile's keys before it knows the count
+    s.candA = base + off; off += align256(Tb * kRsTile * kbytes);
+    s.candB = base + off; off += align256(Tb * kRsTile * kbytes);
     s.rs_counts = (uint32_t *)(base + off); off += align256((size_t)digits * Tb * 4);
     s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * digits * 4);
     s.chunk_kept = (uint32_t *)(base + off); off += align256(Tb * 4);
