@@ -1191,18 +1191,7 @@ __global__ void k_icp_frame_end(IcpState *__restrict__ st, Mat4 *__restrict__ w2
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
-    for (int i = 0; i < 16; ++i)
-        st->prev[i] = st->pose[i];
-    st->sum_iters += st->total_iters;
-    st->frames_tracked += 1;
-    st->fallbacks += st->fallback;
-    for (int l = 0; l < 3; ++l)
-        st->sum_level_iters[l] += st->iters[l];
-    if (w2e) {
-        Mat4 m = st->pose_f;
-        invert_transform(m.m);
-        *w2e = m;
-    }
+    icp_frame_end_dev(st, w2e);
 }
 
 // Sets the pose without tracking (first frame or a given pose).
@@ -1314,6 +1303,7 @@ SplatParams model_splat_params(kifi_model *m, const Mat4 *w2e_dev)
     sp.dmax = m->mp.dmax;
     sp.znear = m->mp.znear;
     sp.pose_out = nullptr;
+    sp.icp_end = nullptr;
     return sp;
 }
 
@@ -1489,8 +1479,14 @@ static void launch_icp_begin(int levels, int P, hipStream_t st, IcpWork *w, cons
 }
 
 // Runs the whole pyramid schedule asynchronously on the context's stream.
+void icp_frame_end(IcpWork *w, Mat4 *w2e_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state, w2e_out);
+}
+
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
-                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip, Mat4 *w2e_out)
+                    const double *guess_host, const double *guess_dev, const kifi_icp_params *ip, Mat4 *w2e_out,
+                    bool defer_end)
 {
     kifi_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -1519,7 +1515,8 @@ kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrins
         icp_pass(m, w, cam, ip, levels, 1);
         hipLaunchKernelGGL(k_icp_fallback_pick, dim3(1), dim3(64), 0, st, w->d_state, ip->fallback_rms);
     }
-    hipLaunchKernelGGL(k_icp_frame_end, dim3(1), dim3(64), 0, st, w->d_state, w2e_out);
+    if (!defer_end)
+        icp_frame_end(w, w2e_out, st);
     timer_end(ctx, KIFI_STAGE_ICP, t_icp);
     KIFI_HIP(ctx, hipGetLastError());
     return KIFI_OK;

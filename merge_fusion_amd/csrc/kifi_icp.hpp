@@ -62,6 +62,24 @@ struct IcpState {
     long long sum_level_iters[3];   // solves per pyramid level
 };
 
+// End of a tracked frame: the pose becomes the previous pose, the running
+// totals advance, and (w2e set) the frame's worldToEye is written.  One thread.
+__device__ __forceinline__ void icp_frame_end_dev(IcpState *st, Mat4 *w2e)
+{
+    for (int i = 0; i < 16; ++i)
+        st->prev[i] = st->pose[i];
+    st->sum_iters += st->total_iters;
+    st->frames_tracked += 1;
+    st->fallbacks += st->fallback;
+    for (int l = 0; l < 3; ++l)
+        st->sum_level_iters[l] += st->iters[l];
+    if (w2e) {
+        Mat4 m = st->pose_f;
+        invert_transform(m.m);
+        *w2e = m;
+    }
+}
+
 struct IcpWork {
     void    *buf = nullptr;
     float   *lv[3] = { nullptr, nullptr, nullptr };
@@ -99,9 +117,12 @@ kifi_status icp_alloc(IcpWork *w, kifi_ctx *ctx, int W, int H);
 size_t      icp_work_bytes(int W, int H);                        // work buffer bytes (256-aligned parts)
 void        icp_work_carve(IcpWork *w, void *buf, int W, int H); // the levels and partials in buf
 void        icp_free(IcpWork *w);
+// defer_end: the frame's end (icp_frame_end_dev) is left to the caller's next
+// kernel (the integration's k_frame_begin), one launch fewer per tracked frame.
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
-                    Mat4 *w2e_out = nullptr);
+                    Mat4 *w2e_out = nullptr, bool defer_end = false);
+void        icp_frame_end(IcpWork *w, Mat4 *w2e_out, hipStream_t st);
 inline kifi_status icp_run(kifi_model *m, IcpWork *w, const uint16_t *d_depth_mm, const kifi_intrinsics *cam,
                            const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
                            Mat4 *w2e_out = nullptr)

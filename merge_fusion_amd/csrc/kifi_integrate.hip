@@ -12,6 +12,7 @@
 // sort of the 8K candidates replaces sort + unique + three unions.
 #include <cfloat>
 
+#include "kifi_icp.hpp"
 #include "kifi_internal.hpp"
 #include "kifi_prims.hpp"
 
@@ -52,15 +53,18 @@ __device__ void clip_pre(const Mat4 &w2c, const VolumeConsts &vc, ClipPre &p)
 // Frame start: per-frame counters and radix totals cleared (a fill kernel, never
 // a memset node), and the frame's matrices from worldToEye (Integrator.cpp:41-42).
 // With model_out, also the model pose of the prediction this frame splats
-// (k_model_begin's, from the same matrices).
+// (k_model_begin's, from the same matrices).  With icp_end, first the end of
+// the frame's tracking, which writes the worldToEye read here.
 __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, Mat4 clip, Mat4 w2e_val,
-                              const Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc,
-                              ModelPose *__restrict__ model_out)
+                              Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc,
+                              ModelPose *__restrict__ model_out, IcpState *__restrict__ icp_end)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
         totals[i] = 0;
     if (blockIdx.x != 0 || threadIdx.x != 0)
         return;
+    if (icp_end)
+        icp_frame_end_dev(icp_end, w2e_dev);
     c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
     c->v_new = 0; c->n_tiles = 0; c->overflow = 0; c->n_upd = 0;
     FrameMats f;
@@ -1500,7 +1504,8 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
         wval.m[i] = w2e ? w2e[i] : 0.0f;
     const bool model_start = splat && splat->pose_out;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * ND, clip, wval,
-                       w2e_dev, vol->d_mats, vol->consts, model_start ? splat->pose_out : nullptr);
+                       const_cast<Mat4 *>(w2e_dev), vol->d_mats, vol->consts,
+                       model_start ? splat->pose_out : nullptr, splat ? splat->icp_end : nullptr);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;

@@ -181,6 +181,8 @@ struct ModelPose {
 };
 
 // Optional model splat folded into the merge + update voxel pass.
+struct IcpState;   // the tracker's device state (kifi_icp.hpp)
+
 struct SplatParams {
     const Mat4 *w2e;                 // device: world->eye of this frame
     unsigned long long *zbuf;        // W*H, cleared to ~0
@@ -190,6 +192,9 @@ struct SplatParams {
     // writes the model pose of this frame (the camera is the integration's)
     // and k_keygen clears zbuf.  Null when the caller has done both.
     ModelPose *pose_out;
+    // When set, k_frame_begin first ends the frame's tracking (the tracker's
+    // state, its worldToEye into w2e) before it reads w2e.
+    IcpState *icp_end;
 };
 
 // Reset list for per-frame device counters (zeroed by a fill kernel, never by a

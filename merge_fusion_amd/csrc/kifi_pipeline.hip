@@ -607,6 +607,7 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
         twistnet_reset_frames(p->net);
     kifi_status s = KIFI_OK;
     bool tracked = false;     // icp_run wrote this frame's worldToEye
+    bool defer_end = false;   // ... or k_frame_begin writes it, ending the tracking
     if (w2e) {
         // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
@@ -631,12 +632,18 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
         // previous frame: the next prediction is then the motion from this one
         if (p->have_guess && net)
             twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
+        // a frame fused at its tracked pose ends its tracking in the
+        // integration's first kernel (k_frame_begin)
+        defer_end = !fuse_w2e;
         s = icp_run(p->model, &p->work, din, &p->cam, p->have_guess ? p->guess : nullptr,
-                    p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e);
+                    p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e, defer_end);
         tracked = true;
         p->have_guess = false;
-        if (s != KIFI_OK)
+        if (s != KIFI_OK) {
+            if (defer_end)
+                icp_frame_end(&p->work, p->d_w2e, st);
             return s;
+        }
     } else if (net) {
         twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
     }
@@ -661,9 +668,13 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
     model_prepare(p->model, p->vol);
     SplatParams sp = model_splat_params(p->model, p->d_w2e);
     sp.pose_out = p->model->d_pose;
+    sp.icp_end = defer_end ? p->work.d_state : nullptr;
     s = integrate_dispatch(p->vol, din, &p->cam, nullptr, p->d_w2e, &sp);
-    if (s != KIFI_OK)
+    if (s != KIFI_OK) {
+        if (defer_end)   // the integration stopped before its first kernel: end the tracking here
+            icp_frame_end(&p->work, p->d_w2e, st);
         return s;
+    }
     const int t_m = timer_begin(ctx, KIFI_STAGE_MODEL);
     model_end(p->model, p->vol, false);
     timer_end(ctx, KIFI_STAGE_MODEL, t_m);
