@@ -399,27 +399,23 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     }
 }
 
-// ================================================= locate / compact / partition
-// The sorted candidates B (with repeats) against the volume A, three launches:
-//   k_locate:    per unique candidate b, r = #A <= b (util::set_union puts A
-//                first on ties, algorithm.h:138-198).  b is dropped when A
-//                holds it already or when it is the largest candidate (the
-//                reference's x pass never emits it, Integrator.cpp:231-245).
-//   k_compact:   the kept keys (the frame's new voxels) in key order, with
-//                their merged positions: the j-th kept key lands at r + j.
-//   k_partition: merged tiles of kMgTile outputs; tile t starts at old voxel
-//                kMgTile t - m and new voxel m, m = #(merged positions <
-//                kMgTile t).
+// ======================================================== locate / compact
+// The sorted candidates B (with repeats) against the volume A, two launches:
+//   k_locate:  per unique candidate b, r = #A <= b (util::set_union puts A
+//              first on ties, algorithm.h:138-198).  b is dropped when A holds
+//              it already or when it is the largest candidate (the reference's
+//              x pass never emits it, Integrator.cpp:231-245).
+//   k_compact: the kept keys (the frame's new voxels) in key order with their
+//              merged positions (the j-th kept key lands at r + j), and the
+//              partition of the merged volume into tiles of kMgTile outputs.
 // Every search runs on a sorted array W with two sampled levels beside it,
 // S1[i] = W[i << SH1] and S2[i] = W[i << kS2Shift]: a search over S2 (a few KB,
 // cache resident) and two short windows, instead of a binary search of the
 // whole array whose every step waits for HBM.  The volume's keys keep every
-// 16th (kVolS1) and every 4096th key; the new voxels' merged positions every
-// 64th (kPosS1) and every 4096th.
+// 16th (kVolS1) and every 4096th key.
 
 constexpr int kVolS1 = 4;
 constexpr int kKeyPad = 32;   // keys of padding past a volume key buffer's capacity
-constexpr int kPosS1 = 6;
 constexpr int kS2Shift = 12;
 
 template <int SH1>
@@ -800,80 +796,135 @@ __global__ __launch_bounds__(kThreads) void k_locate(const K *__restrict__ B, co
 }
 
 // The kept keys of every chunk, in key order, with their merged positions (the
-// j-th kept key lands at its rank + j) and the positions' two sampled levels.
-// The last chunk (chunk 0 when there are no candidates) records the counts.
+// j-th kept key lands at its rank + j), and the merge's tile partition: tile t
+// starts at old voxel kMgTile t - m and new voxel m, m = #(positions <
+// kMgTile t).  New voxel j owns the tile boundaries in (position of j - 1,
+// position of j] and writes m = j there; the last non-empty chunk (chunk 0
+// when nothing is kept) writes the boundaries after the last new voxel and
+// closes the frame's counts: the new size, the capacity checks, the running
+// totals and the host ring slot.  A gap of more than kGapDirect boundaries is
+// written by the whole workgroup.
+constexpr int kGapDirect = 4;
+constexpr int kGapList = 256;
+
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_compact(const K *__restrict__ Bc, const uint64_t *__restrict__ Rc,
                                                       const uint32_t *__restrict__ d_n,
                                                       const uint32_t *__restrict__ chunk_kept,
                                                       const uint32_t *__restrict__ chunk_heads,
                                                       FrameCounters *__restrict__ cnt, K *__restrict__ Bn,
-                                                      uint64_t *__restrict__ on, uint64_t *__restrict__ onS1,
-                                                      uint64_t *__restrict__ onS2)
+                                                      uint64_t *__restrict__ on, uint64_t *__restrict__ part_a,
+                                                      uint64_t *__restrict__ part_b, int Tbound,
+                                                      uint64_t *__restrict__ d_size, uint64_t cap,
+                                                      FrameCounters *__restrict__ host_cnt)
 {
     __shared__ uint32_t tmp[kThreads / kWave + 1];
+    __shared__ uint64_t gap_lo[kGapList], gap_hi[kGapList];
+    __shared__ uint32_t gap_j[kGapList];
+    __shared__ uint32_t n_gap;
     const uint32_t n = *d_n;
     const uint32_t nchunks = (n + kRsTile - 1) / kRsTile;
     const uint32_t c = blockIdx.x;
-    const bool lastc = c + 1 == (nchunks ? nchunks : 1u);
     const uint32_t kc = c < nchunks ? chunk_kept[c] : 0u;
-    if (kc == 0 && !lastc)
+    if (kc == 0 && c != 0)
         return;
     const size_t base = (size_t)c * kRsTile;
-    // the chunk's first kept entries are loaded before the prefix is known
+    // the chunk's first kept entries are read before the prefix is known
     K kb = K(0);
-    uint64_t kr = 0;
+    uint64_t kr = 0, kp = 0;
     if (threadIdx.x < kc) {
         kb = Bc[base + threadIdx.x];
         kr = Rc[base + threadIdx.x];
+        if (threadIdx.x > 0)
+            kp = Rc[base + threadIdx.x - 1];
     }
-    uint32_t s = 0;
-    for (uint32_t q = threadIdx.x; q < c; q += kThreads)
-        s += chunk_kept[q];
+    // kept keys before and after this chunk, and the nearest earlier chunk that kept any
+    uint32_t s = 0, sa = 0, lastq = 0;
+    for (uint32_t q = threadIdx.x; q < nchunks; q += kThreads) {
+        const uint32_t v = chunk_kept[q];
+        if (q < c) {
+            s += v;
+            if (v)
+                lastq = max(lastq, q + 1);
+        } else if (q > c) {
+            sa += v;
+        }
+    }
+    if (threadIdx.x == 0)
+        n_gap = 0;
     const uint32_t pre = block_reduce_sum<kThreads>(s, tmp);
+    const uint32_t post = block_reduce_sum<kThreads>(sa, tmp);
+    lastq = block_reduce_max<kThreads>(lastq, tmp);     // 1 + that chunk, 0 when none
+    const uint32_t nk = pre + kc + post;
+    // position of the kept key just before this chunk's first (none: ~0)
+    uint64_t o_before = ~0ull;
+    if (lastq) {
+        const uint32_t q = lastq - 1;
+        o_before = Rc[(size_t)q * kRsTile + chunk_kept[q] - 1] + (pre - 1);
+    }
     for (uint32_t i = threadIdx.x; i < kc; i += kThreads) {
         if (i >= kThreads) {
             kb = Bc[base + i];
             kr = Rc[base + i];
+            kp = Rc[base + i - 1];
         }
         const uint32_t j = pre + i;
         const uint64_t o = kr + j;
         Bn[j] = kb;
         on[j] = o;
-        if ((j & 63u) == 0)
-            onS1[j >> kPosS1] = o;
-        if ((j & 4095u) == 0)
-            onS2[j >> kS2Shift] = o;
-    }
-    if (lastc) {
-        uint32_t hs = 0;
-        for (uint32_t q = threadIdx.x; q < nchunks; q += kThreads)
-            hs += chunk_heads[q];
-        hs = block_reduce_sum<kThreads>(hs, tmp);
-        if (threadIdx.x == 0) {
-            cnt->n_kept = pre + kc;
-            cnt->n_unique = hs;
-            cnt->n_new = hs ? hs - 1 : 0;
+        const uint64_t o_prev = i ? kp + (j - 1) : o_before;
+        const uint64_t t_lo = o_prev == ~0ull ? 0 : o_prev / kMgTile + 1, t_hi = o / kMgTile;
+        if (t_lo > t_hi)
+            continue;
+        if (t_hi - t_lo < kGapDirect) {
+            for (uint64_t t = t_lo; t <= t_hi; ++t)
+                if (t <= (uint64_t)Tbound) {
+                    part_b[t] = j;
+                    part_a[t] = t * kMgTile - j;
+                }
+        } else {
+            const uint32_t g = atomicAdd(&n_gap, 1u);
+            if (g < kGapList) {
+                gap_lo[g] = t_lo;
+                gap_hi[g] = t_hi;
+                gap_j[g] = j;
+            } else {
+                for (uint64_t t = t_lo; t <= t_hi && t <= (uint64_t)Tbound; ++t) {
+                    part_b[t] = j;
+                    part_a[t] = t * kMgTile - j;
+                }
+            }
         }
     }
-}
-
-// One thread per tile boundary t in [0, Tbound]: m = #(merged positions of new
-// voxels < min(kMgTile t, V1)).  Thread 0 also closes the frame's counts: the
-// new size, the capacity checks, the running totals and the host ring slot.
-__global__ __launch_bounds__(kThreads) void k_partition(const uint64_t *__restrict__ on,
-                                                        const uint64_t *__restrict__ onS1,
-                                                        const uint64_t *__restrict__ onS2,
-                                                        FrameCounters *__restrict__ cnt,
-                                                        uint64_t *__restrict__ part_a,
-                                                        uint64_t *__restrict__ part_b, int Tbound,
-                                                        uint64_t *__restrict__ d_size, uint64_t cap,
-                                                        FrameCounters *__restrict__ host_cnt)
-{
-    const uint64_t v0 = cnt->v_old, nk = cnt->n_kept, v1 = v0 + nk;
+    __syncthreads();
+    const uint32_t ng = min(n_gap, (uint32_t)kGapList);
+    for (uint32_t g = 0; g < ng; ++g) {
+        const uint64_t hi = min(gap_hi[g], (uint64_t)Tbound);
+        const uint32_t j = gap_j[g];
+        for (uint64_t t = gap_lo[g] + threadIdx.x; t <= hi; t += kThreads) {
+            part_b[t] = j;
+            part_a[t] = t * kMgTile - j;
+        }
+    }
+    // the last new voxel's chunk (chunk 0 when there is none)
+    if (!(post == 0 && (kc > 0 || nk == 0)))
+        return;
+    const uint64_t v0 = cnt->v_old, v1 = v0 + nk;
     const uint64_t T = (v1 + kMgTile - 1) / kMgTile;
-    const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (t == 0) {
+    const uint64_t o_last = nk ? Rc[base + kc - 1] + (nk - 1) : 0;
+    const uint64_t t0 = nk ? o_last / kMgTile + 1 : 0;
+    for (uint64_t t = t0 + threadIdx.x; t <= T && t <= (uint64_t)Tbound; t += kThreads) {
+        part_b[t] = nk;
+        part_a[t] = min(t * kMgTile, v1) - nk;
+    }
+    uint32_t hs = 0;
+    for (uint32_t q = threadIdx.x; q < nchunks; q += kThreads)
+        hs += chunk_heads[q];
+    hs = block_reduce_sum<kThreads>(hs, tmp);
+    if (threadIdx.x == 0) {
+        cnt->n_kept = nk;
+        cnt->n_unique = hs;
+        cnt->n_new = hs ? hs - 1 : 0;
         cnt->v_new = v1;
         cnt->n_tiles = (uint32_t)min(T, (uint64_t)0xffffffffu);
         if (v1 > cap || T > (uint64_t)Tbound) {
@@ -888,16 +939,6 @@ __global__ __launch_bounds__(kThreads) void k_partition(const uint64_t *__restri
         }
         *host_cnt = *cnt;   // host-mapped: read after the frame's event
     }
-    if (t > T || t > (uint64_t)Tbound)
-        return;
-    const uint64_t q = min(t * kMgTile, v1);
-    uint64_t m[1] = { 0 };
-    if (nk > 0 && q > 0) {
-        const uint64_t x[1] = { q - 1 };
-        rank_le<uint64_t, 1, kPosS1>(on, onS1, onS2, nk, x, m);
-    }
-    part_b[t] = m[0];
-    part_a[t] = q - m[0];
 }
 
 // The two sampled levels of the volume's keys, rebuilt after the host wrote
@@ -1413,7 +1454,7 @@ struct IntegrateScratch {
     uint32_t *chunk_kept;    // [Tb]
     uint32_t *chunk_heads;   // [Tb]
     uint64_t *rank;          // [8P] the kept candidates' volume ranks, per chunk
-    uint64_t *on, *onS1, *onS2;   // merged positions of the new voxels and their sampled levels
+    uint64_t *on;            // [8P] merged positions of the new voxels
     float    *depth_m;       // [P] a uint16 frame in metres
     size_t    bytes;
 };
@@ -1435,8 +1476,6 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     s.chunk_heads = (uint32_t *)(base + off); off += align256(Tb * 4);
     s.rank = (uint64_t *)(base + off); off += align256(ncand * 8);
     s.on = (uint64_t *)(base + off); off += align256(ncand * 8);
-    s.onS1 = (uint64_t *)(base + off); off += align256((s1_count<kPosS1>(ncand) + 1) * 8);
-    s.onS2 = (uint64_t *)(base + off); off += align256((s2_count(ncand) + 1) * 8);
     s.depth_m = (float *)(base + off); off += align256(P * 4);
     s.bytes = off;
     return s;
@@ -1541,7 +1580,7 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
         K *t = src; src = dst; dst = t;
     }
 
-    // new voxels -> dst, with their merged positions; tile partitions
+    // new voxels, their merged positions and the merge's tile partition
     const int cur = vol->cur, nxt = cur ^ 1;
     if (!vol->samples_valid) {
         const size_t n1 = s1_count<kVolS1>(bound);
@@ -1556,10 +1595,8 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
                        (const K *)vol->keys[cur], (const K *)vol->samp1[cur], (const K *)vol->samp2[cur],
                        vol->d_size, cnt, dst, s.rank, s.chunk_kept, s.chunk_heads);
     hipLaunchKernelGGL(k_compact<K>, dim3((unsigned)Tb), dim3(kThreads), 0, stream, dst, s.rank, &cnt->n_cand,
-                       s.chunk_kept, s.chunk_heads, cnt, src, s.on, s.onS1, s.onS2);
-    hipLaunchKernelGGL(k_partition, dim3((unsigned)((Tm + 1 + kThreads - 1) / kThreads)), dim3(kThreads), 0, stream,
-                       s.on, s.onS1, s.onS2, cnt, part_a, part_b, (int)Tm, vol->d_size, (uint64_t)vol->cap,
-                       vol->h_ring_dev + slot);
+                       s.chunk_kept, s.chunk_heads, cnt, src, s.on, part_a, part_b, (int)Tm, vol->d_size,
+                       (uint64_t)vol->cap, vol->h_ring_dev + slot);
     timer_end(ctx, KIFI_STAGE_SORT, t_sort);
     const K *Bnew = src;
     UpdateParams up;
@@ -1588,7 +1625,7 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;
 
-    // k_partition wrote the counters into the frame's host-mapped ring slot;
+    // k_compact wrote the counters into the frame's host-mapped ring slot;
     // the event marks when they may be read to refresh the host size bound
     KIFI_HIP(ctx, hipEventRecord(vol->ring_ev[slot], stream));
     vol->ring_frame[slot] = ++vol->frames_enq;

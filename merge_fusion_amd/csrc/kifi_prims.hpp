@@ -75,6 +75,26 @@ __device__ __forceinline__ T block_reduce_sum(T v, T *tmp)
     return acc;
 }
 
+template <int NT, typename T>
+__device__ __forceinline__ T block_reduce_max(T v, T *tmp)
+{
+    constexpr int NW = NT / kWave;
+    const int w = threadIdx.x / kWave, lane = __lane_id();
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const T u = __shfl_xor(v, o, kWave);
+        v = u > v ? u : v;
+    }
+    if (lane == 0)
+        tmp[w] = v;
+    __syncthreads();
+    T acc = tmp[0];
+    for (int i = 1; i < NW; ++i)
+        acc = tmp[i] > acc ? tmp[i] : acc;
+    __syncthreads();
+    return acc;
+}
+
 // Number of elements of sorted s[0, n) that are < x (lower bound).
 template <typename K>
 __device__ __forceinline__ uint32_t lds_lower_bound(const K *s, uint32_t n, K x)
