@@ -8,7 +8,8 @@ and 2 % invalid pixels; 4 mm voxels (res 1024, side 4.096 m, truncation 2 cm,
 binary32 sum + weight, 32-bit keys); per frame: 3-level point-to-plane ICP
 (10/5/4 iterations) against the model of the previous frame, integration at
 the tracked pose, model prediction; a mesh (Mesher::Mesh, vertices copied to
-page-locked host memory) at frames 49, 99, 149 and 199.
+page-locked host memory while the next frames run) at frames 49, 99, 149 and
+199, all on the host before the step ends.
 
 One step = the whole 200-frame trajectory from an empty volume (frame 0 at
 its true pose, frames 1..199 tracked, the four meshes), so every step is the
@@ -205,11 +206,15 @@ def headline(mf, ctx, cam, args, frames, dframes, poses):
             if record is not None:
                 record(f, pipe)
             if mesh and (f + 1) % MESH_EVERY == 0:
-                nv = pipe.volume.mesh_into(mesh_buf[0])
+                # the vertex copy overlaps the next frames; every mesh is on the
+                # host before the trajectory returns
+                nv = pipe.volume.mesh_into_async(mesh_buf[0])
                 if nv > len(mesh_buf[0]):
+                    ctx.mesh_wait()
                     mesh_buf[0] = ctx.host_empty((int(nv * 1.5), 3), np.float32)
-                    nv = pipe.volume.mesh_into(mesh_buf[0])
+                    nv = pipe.volume.mesh_into_async(mesh_buf[0])
                 meshes.append(int(nv))
+        ctx.mesh_wait()
         return meshes
 
     def run_pass(stages, mesh=True, host=None):

@@ -133,6 +133,18 @@ kifi_status kifi_mesh(kifi_volume *vol, float *verts, size_t vcap, size_t *nvert
 kifi_status kifi_mesh_indexed(kifi_volume *vol, float *verts, size_t vcap, uint32_t *idx,
                               size_t icap, size_t *nverts, size_t *nidx, int *stalled);
 
+/* kifi_mesh_async: kifi_mesh whose copy of the vertices to the host overlaps the
+ * work queued after it.  Returns once the count is known (*nverts, or
+ * KIFI_ERR_CAPACITY with nothing queued) and the mesh kernels and the copy are
+ * queued; verts must stay valid, and is complete, after kifi_mesh_wait (or
+ * kifi_ctx_synchronize, or kifi_host_free) returns.  A later mesh call waits for
+ * a pending copy on the device.  The copy overlaps only into page-locked memory
+ * (kifi_host_alloc).  No counterpart in the reference: Mesher::Mesh
+ * (Mesher.cpp:14-26) returns a filled vector; this splits its return for a
+ * caller that keeps integrating frames meanwhile. */
+kifi_status kifi_mesh_async(kifi_volume *vol, float *verts, size_t vcap, size_t *nverts);
+kifi_status kifi_mesh_wait(kifi_ctx *ctx);
+
 /* ---- model prediction: kifi::Renderer::Bin (Renderer.h:22-29, Renderer.cpp:39-80) ----
  * z-buffered splat of n float3 points (host) into a width x height float3 map
  * (host, 0 = empty) with the given worldToClip. Synchronous. */

@@ -33,7 +33,7 @@ HIP_EXPORTS = [
     "kifi_volume_create", "kifi_volume_destroy", "kifi_volume_clear", "kifi_volume_size",
     "kifi_volume_download", "kifi_volume_upload", "kifi_integrate", "kifi_integrate_device",
     "kifi_volume_save", "kifi_volume_load", "kifi_integrate_f32", "kifi_integrate_device_f32", "kifi_pipeline_integrate_f32", "kifi_icp_align_points_f32",
-    "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_bin",
+    "kifi_integrate_stats", "kifi_mesh", "kifi_mesh_indexed", "kifi_mesh_async", "kifi_mesh_wait", "kifi_bin",
     "kifi_icp_params_default", "kifi_model_create", "kifi_model_destroy", "kifi_model_predict",
     "kifi_model_download", "kifi_icp_align", "kifi_icp_normal_equations", "kifi_horn_align", "kifi_pipeline_create",
     "kifi_pipeline_destroy", "kifi_pipeline_volume", "kifi_pipeline_model", "kifi_pipeline_integrate",
@@ -150,6 +150,8 @@ def load_hip():
             "kifi_integrate_device_f32": ([vp, vp, ctypes.POINTER(Intrinsics), f32p], i32),
             "kifi_integrate_stats": ([vp, szp], i32),
             "kifi_mesh": ([vp, f32p, sz, szp], i32),
+            "kifi_mesh_async": ([vp, f32p, sz, szp], i32),
+            "kifi_mesh_wait": ([vp], i32),
             "kifi_mesh_indexed": ([vp, f32p, sz, ctypes.POINTER(ctypes.c_uint32), sz, szp, szp,
                                    ctypes.POINTER(ctypes.c_int)], i32),
             "kifi_bin": ([vp, f32p, sz, f32p, i32, i32, f32p], i32),
@@ -335,6 +337,10 @@ class Context:
 
     def synchronize(self):
         self.check(self._lib.kifi_ctx_synchronize(self._h), "synchronize")
+
+    def mesh_wait(self):
+        """Waits for the vertex copies queued by Volume.mesh_into_async."""
+        self.check(self._lib.kifi_mesh_wait(self._h), "kifi_mesh_wait")
 
     def close(self):
         if self._h:
@@ -576,6 +582,18 @@ class Volume:
         rc = self.ctx._lib.kifi_mesh(self._h, _f32p(buf), buf.shape[0], ctypes.byref(n))
         if rc not in (KIFI_OK, KIFI_ERR_CAPACITY):
             self.ctx.check(rc, "kifi_mesh")
+        return n.value
+
+    def mesh_into_async(self, buf):
+        """mesh_into whose copy to buf overlaps the work queued after it: returns the
+        vertex count once the kernels and the copy are queued (nothing is queued if
+        it exceeds len(buf)); buf holds the vertices after Context.mesh_wait (or
+        synchronize) and must be kept alive until then.  Overlaps only into
+        page-locked buffers (Context.host_empty)."""
+        n = ctypes.c_size_t()
+        rc = self.ctx._lib.kifi_mesh_async(self._h, _f32p(buf), buf.shape[0], ctypes.byref(n))
+        if rc not in (KIFI_OK, KIFI_ERR_CAPACITY):
+            self.ctx.check(rc, "kifi_mesh_async")
         return n.value
 
     def mesh_indexed(self):

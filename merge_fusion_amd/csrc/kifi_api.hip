@@ -13,7 +13,7 @@ namespace kifi {
 kifi_status integrate_dispatch(kifi_volume *vol, DepthIn d_depth, const kifi_intrinsics *cam,
                                const float *w2e, const Mat4 *w2e_dev, const SplatParams *splat);
 kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
-                          size_t *nverts, size_t *nidx, int *stalled);
+                          size_t *nverts, size_t *nidx, int *stalled, bool async_copy);
 kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,
                      float *out);
 
@@ -51,8 +51,22 @@ kifi_status ensure_scratch(kifi_ctx *ctx, size_t bytes)
     return ensure_buf(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes);
 }
 
+kifi_status out_settle(kifi_ctx *ctx)
+{
+    if (!ctx->out_pending)
+        return KIFI_OK;
+    ctx->out_pending = false;
+    KIFI_HIP(ctx, hipEventSynchronize(ctx->out_copied));
+    return KIFI_OK;
+}
+
 kifi_status ensure_out(kifi_ctx *ctx, size_t bytes)
 {
+    if (bytes > ctx->out_bytes) {      // the buffer is freed: no copy may still read it
+        kifi_status s = out_settle(ctx);
+        if (s != KIFI_OK)
+            return s;
+    }
     return ensure_buf(ctx, &ctx->out_scratch, &ctx->out_bytes, bytes);
 }
 
@@ -293,6 +307,9 @@ kifi_status kifi_host_free(kifi_ctx *ctx, void *ptr)
     if (!ctx)
         return KIFI_ERR_ARG;
     KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    kifi_status s = out_settle(ctx);       // a mesh copy may still write into ptr
+    if (s != KIFI_OK)
+        return s;
     if (ptr)
         KIFI_HIP(ctx, hipHostFree(ptr));
     return KIFI_OK;
@@ -384,6 +401,11 @@ void kifi_ctx_destroy(kifi_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->out_stream)
+        (void)hipStreamSynchronize(ctx->out_stream);
+    if (ctx->out_ready) (void)hipEventDestroy(ctx->out_ready);
+    if (ctx->out_copied) (void)hipEventDestroy(ctx->out_copied);
+    if (ctx->out_stream) (void)hipStreamDestroy(ctx->out_stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->out_scratch) (void)hipFree(ctx->out_scratch);
     if (ctx->copy_stream)
@@ -415,7 +437,7 @@ kifi_status kifi_ctx_synchronize(kifi_ctx *ctx)
     if (!ctx)
         return KIFI_ERR_ARG;
     KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return KIFI_OK;
+    return out_settle(ctx);
 }
 
 void *kifi_ctx_stream(kifi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
@@ -852,10 +874,27 @@ kifi_status kifi_mesh(kifi_volume *v, float *verts, size_t vcap, size_t *nverts)
 {
     if (!v || !nverts || (vcap && !verts))
         return KIFI_ERR_ARG;
-    kifi_status st = mesh_dispatch(v, false, verts, vcap, nullptr, 0, nverts, nullptr, nullptr);
+    kifi_status st = mesh_dispatch(v, false, verts, vcap, nullptr, 0, nverts, nullptr, nullptr, false);
     if (st == KIFI_ERR_CAPACITY && *nverts <= 0xffffffffull)   // else the 32-bit limit's message stands
         set_error(v->ctx, st, "kifi_mesh: vertex buffer too small");
     return st;
+}
+
+kifi_status kifi_mesh_async(kifi_volume *v, float *verts, size_t vcap, size_t *nverts)
+{
+    if (!v || !nverts || (vcap && !verts))
+        return KIFI_ERR_ARG;
+    kifi_status st = mesh_dispatch(v, false, verts, vcap, nullptr, 0, nverts, nullptr, nullptr, true);
+    if (st == KIFI_ERR_CAPACITY && *nverts <= 0xffffffffull)
+        set_error(v->ctx, st, "kifi_mesh_async: vertex buffer too small");
+    return st;
+}
+
+kifi_status kifi_mesh_wait(kifi_ctx *ctx)
+{
+    if (!ctx)
+        return KIFI_ERR_ARG;
+    return out_settle(ctx);
 }
 
 kifi_status kifi_mesh_indexed(kifi_volume *v, float *verts, size_t vcap, uint32_t *idx, size_t icap,
@@ -863,7 +902,7 @@ kifi_status kifi_mesh_indexed(kifi_volume *v, float *verts, size_t vcap, uint32_
 {
     if (!v || !nverts || !nidx || (vcap && !verts) || (icap && !idx))
         return KIFI_ERR_ARG;
-    kifi_status st = mesh_dispatch(v, true, verts, vcap, idx, icap, nverts, nidx, stalled);
+    kifi_status st = mesh_dispatch(v, true, verts, vcap, idx, icap, nverts, nidx, stalled, false);
     if (st == KIFI_ERR_CAPACITY && *nverts <= 0xffffffffull && *nidx <= 0xffffffffull)
         set_error(v->ctx, st, "kifi_mesh_indexed: buffer too small");
     return st;

@@ -226,6 +226,12 @@ struct kifi_ctx {
     size_t       scratch_bytes = 0;
     void        *out_scratch = nullptr;  // output staging (mesh, model maps)
     size_t       out_bytes = 0;
+    // asynchronous mesh delivery: the vertex copy out of out_scratch runs on
+    // out_stream after out_ready (recorded on stream); out_copied marks its end
+    hipStream_t  out_stream = nullptr;
+    hipEvent_t   out_ready = nullptr;
+    hipEvent_t   out_copied = nullptr;
+    bool         out_pending = false;   // a copy out of out_scratch may still run
     // host frames: uploaded on a copy stream into two staging buffers in turn,
     // so the upload of frame t+1 overlaps the kernels of frame t
     hipStream_t  copy_stream = nullptr;
@@ -312,5 +318,7 @@ void ring_settle(kifi_volume *v, size_t n);
 // scratch allocators (grow; contents not preserved across a growth)
 kifi_status ensure_scratch(kifi_ctx *ctx, size_t bytes);
 kifi_status ensure_out(kifi_ctx *ctx, size_t bytes);
+// Waits for a pending asynchronous copy out of out_scratch (no-op when none).
+kifi_status out_settle(kifi_ctx *ctx);
 
 } // namespace kifi

@@ -422,6 +422,269 @@ __global__ void k_mesh_stall_fix(const uint64_t *__restrict__ raw_ids, uint64_t 
             idx[q] = (uint32_t)raw_ids[q];
 }
 
+// ------------------------------------------------- vertices only (Mesh(volume, verts))
+// The vertex-only mesh needs each voxel's vertex flags and, for the output
+// positions, only how many vertices the voxels after it emit.  Pass 1 keeps
+// the flags (one byte per voxel) and one vertex count per tile; the tile
+// counts are scanned in generation order (descending tiles); pass 2 reads the
+// flags, places each voxel by a scan within its tile and revisits only the
+// voxels that emit.
+//
+// Tiles go to XCDs in contiguous runs (workgroup b runs on XCD b % 8): the
+// +y and +z key windows of consecutive tiles overlap, and a tile's keys were
+// read shortly before as part of an earlier tile's windows, so they meet in
+// the XCD's own L2.
+constexpr int kVQ = kMeshTile / kMT;          // voxels per thread
+constexpr int kWinRounds = kMeshWin / kMT;    // window loads per thread per offset
+constexpr int kXcds = 8;
+
+template <typename K>
+struct VWin {
+    K s[2][kMeshWin];                         // +y and +z windows
+};
+
+template <typename K>
+struct VRange {
+    uint64_t lo[2];
+    uint32_t len[2];
+    bool lds[2];
+};
+
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t T)
+{
+    const uint64_t per = (T + kXcds - 1) / kXcds;
+    return (uint64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+// The +y / +z windows of tile t ([w_t - 1, w_{t+1}] clipped to the volume,
+// as in mesh_load_windows) and their fill into LDS: every load of both windows
+// is issued before the first LDS store.
+template <typename K>
+__device__ __forceinline__ VRange<K> vwin_load(const K *keys, uint64_t n, const uint64_t *w, uint64_t T,
+                                               uint64_t t, VWin<K> &mw)
+{
+    VRange<K> r;
+    K x[2][kWinRounds];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        const uint64_t a = min(w[o * (T + 1) + t], n - 1), b = w[o * (T + 1) + t + 1];
+        const uint64_t lo = a >= 1 ? a - 1 : 0, hi = min(b + 1, n);
+        r.lo[o] = lo;
+        r.lds[o] = hi - lo <= (uint64_t)kMeshWin;
+        r.len[o] = r.lds[o] ? (uint32_t)(hi - lo) : 0u;
+        const uint32_t last = r.len[o] ? r.len[o] - 1 : 0u;
+#pragma unroll
+        for (int q = 0; q < kWinRounds; ++q)       // clamped: in bounds, the extra ones unused
+            x[o][q] = keys[lo + min((uint32_t)(q * kMT) + threadIdx.x, last)];
+    }
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int q = 0; q < kWinRounds; ++q)
+            if ((uint32_t)(q * kMT) + threadIdx.x < r.len[o])
+                mw.s[o][q * kMT + threadIdx.x] = x[o][q];
+    __syncthreads();
+    return r;
+}
+
+// upper_bound of NC targets per offset (chains interleaved step by step) ->
+// min(V-2, upper_bound - 1) in r and the key there in k_at.  Targets of an
+// offset whose window did not fit LDS are searched in global memory.
+template <typename K, int NC>
+__device__ __forceinline__ void vwin_search(const K *keys, uint64_t n, const VWin<K> &mw, const VRange<K> &wr,
+                                            const K (&tg)[2][NC], const bool (&need)[2][NC],
+                                            int64_t (&r)[2][NC], K (&k_at)[2][NC])
+{
+    uint32_t a[2][NC];
+    uint32_t L = 1;
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        L = max(L, wr.len[o]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            a[o][c] = 0;
+    }
+    for (uint32_t st = 1u << (31 - __clz(L)); st; st >>= 1) {
+#pragma unroll
+        for (int o = 0; o < 2; ++o)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const uint32_t p = a[o][c] + st;
+                const K x = mw.s[o][max(min(p, wr.len[o]), 1u) - 1];
+                a[o][c] = (p <= wr.len[o] && x <= tg[o][c]) ? p : a[o][c];
+            }
+    }
+    const int64_t cl = (int64_t)n - 2;
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            uint64_t u;
+            if (wr.lds[o])
+                u = wr.lo[o] + a[o][c];
+            else
+                u = need[o][c] ? upper_bound_g(keys, wr.lo[o], n, tg[o][c]) : wr.lo[o] + 1;
+            const int64_t rr = min((int64_t)u - 1, cl);
+            r[o][c] = rr;
+            k_at[o][c] = wr.lds[o] ? mw.s[o][rr - (int64_t)wr.lo[o]] : keys[rr];
+        }
+}
+
+// Pass 1: the vertex flags of voxels i0 + q * kMT + tid (Mesher.cpp:117-168)
+// and the tile's vertex count at tile_rev[T - 1 - t].
+template <typename K, typename V>
+__global__ __launch_bounds__(kMT) void k_mesh_vcount(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                     uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
+                                                     uint8_t *__restrict__ vflags, uint32_t *__restrict__ tile_rev)
+{
+    __shared__ VWin<K> mw;
+    __shared__ uint32_t red[kMT / kWave + 1];
+    constexpr K d1 = 1, d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
+    const uint64_t t = xcd_tile(T);
+    if (t >= T)
+        return;
+    const uint64_t m = n - 1;
+    const uint64_t i0 = t * kMeshTile, i1 = min(i0 + kMeshTile, m);
+    // own keys, successors and values in flight with the windows
+    K k[kVQ], kn[kVQ];
+    V v0[kVQ];
+#pragma unroll
+    for (int q = 0; q < kVQ; ++q) {
+        const uint64_t i = min(i0 + (uint64_t)(q * kMT) + threadIdx.x, m - 1);   // i + 1 <= V - 1
+        k[q] = keys[i];
+        kn[q] = keys[i + 1];
+        v0[q] = vals[i];
+    }
+    const VRange<K> wr = vwin_load<K>(keys, n, win, T, t, mw);
+    K tg[2][kVQ], k_at[2][kVQ];
+    bool need[2][kVQ];
+    int64_t r[2][kVQ];
+#pragma unroll
+    for (int q = 0; q < kVQ; ++q) {
+        tg[0][q] = k[q] + d2;                                        // :122-123
+        tg[1][q] = k[q] + d4;                                        // :125-126
+        need[0][q] = need[1][q] = i0 + (uint64_t)(q * kMT) + threadIdx.x < i1;
+    }
+    vwin_search<K, kVQ>(keys, n, mw, wr, tg, need, r, k_at);
+    // the corner values: absent corners read the voxel's own value
+    V vc[3][kVQ];
+    int64_t cn[3][kVQ];
+#pragma unroll
+    for (int q = 0; q < kVQ; ++q) {
+        const int64_t i = (int64_t)min(i0 + (uint64_t)(q * kMT) + threadIdx.x, m - 1);
+        cn[0][q] = kn[q] == k[q] + d1 ? i + 1 : -1;                  // :132
+        cn[1][q] = k_at[0][q] == k[q] + d2 ? r[0][q] : -1;           // :129
+        cn[2][q] = k_at[1][q] == k[q] + d4 ? r[1][q] : -1;           // :130
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            vc[a][q] = vals[cn[a][q] >= 0 ? cn[a][q] : i];
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kVQ; ++q) {
+        const uint64_t i = i0 + (uint64_t)(q * kMT) + threadIdx.x;
+        uint32_t f = 0;
+        if (vweight(v0[q]) != 0.0f) {                                // :119-120
+            const float dself = vdist(v0[q]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)                              // :140-168
+                if (cn[a][q] >= 0 && vweight(vc[a][q]) > 0.0f && dself * vdist(vc[a][q]) <= 0.0f)
+                    f |= 1u << a;
+        }
+        if (i < i1) {
+            vflags[i] = (uint8_t)f;
+            cnt += __popc(f);
+        }
+    }
+    cnt = block_reduce_sum<kMT>(cnt, red);
+    if (threadIdx.x == 0)
+        tile_rev[T - 1 - t] = cnt;
+}
+
+// Pass 2: the vertices of the flagged voxels of tile t.  Thread tid reads the
+// flags of voxels i0 + 4 tid .. + 3; one block scan gives both each voxel's
+// place in generation order (the vertices of the voxels after it, then its
+// own in axis order x, y, z) and its slot in an LDS list of flagged voxels,
+// which the workgroup then works through.
+template <typename K, typename V>
+__global__ __launch_bounds__(kMT) void k_mesh_vemit(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                    uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
+                                                    MeshParams mp, const uint8_t *__restrict__ vflags,
+                                                    const uint32_t *__restrict__ tile_off_rev,
+                                                    float *__restrict__ verts)
+{
+    static_assert(kVQ == 4, "four flag bytes per thread");
+    __shared__ VWin<K> mw;
+    __shared__ uint32_t tmp[kMT / kWave + 1];
+    __shared__ uint16_t sh_ent[kMeshTile];    // voxel within the tile | flags << 10
+    __shared__ uint32_t sh_off[kMeshTile];    // its first vertex
+    constexpr K d2 = K(1) << KeyTraits<K>::kb, d4 = K(1) << (2 * KeyTraits<K>::kb);
+    const uint64_t t = xcd_tile(T);
+    if (t >= T)
+        return;
+    const uint64_t m = n - 1;
+    const uint64_t i0 = t * kMeshTile, i1 = min(i0 + kMeshTile, m);
+    const uint64_t my = i0 + 4 * (uint64_t)threadIdx.x;
+    uint32_t f4 = my < i1 ? *(const uint32_t *)(vflags + my) : 0u;  // 4 bytes from a 4-aligned voxel
+    if (my + 4 > i1 && my < i1)
+        f4 &= 0xffffffffu >> (8 * (uint32_t)(my + 4 - i1));
+    const uint32_t base = tile_off_rev[T - 1 - t];
+    const VRange<K> wr = vwin_load<K>(keys, n, win, T, t, mw);
+    uint32_t nf = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        nf += (f4 >> (8 * q) & 0xffu) != 0u;
+    const uint32_t cnt = __popc(f4);
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<kMT>(cnt << 16 | nf, tmp, tot);
+    // vertices of the voxels after mine in this tile
+    uint32_t o = base + (tot >> 16) - (ex >> 16) - cnt;
+    uint32_t slot = ex & 0xffffu;
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+        const uint32_t f = f4 >> (8 * q) & 0xffu;
+        if (f) {
+            sh_ent[slot] = (uint16_t)((4 * threadIdx.x + q) | f << 10);
+            sh_off[slot] = o;
+            ++slot;
+            o += __popc(f);
+        }
+    }
+    __syncthreads();
+    const uint32_t nlist = tot & 0xffffu;
+    for (uint32_t e = threadIdx.x; e < nlist; e += kMT) {
+        const uint32_t ent = sh_ent[e];
+        const uint64_t i = i0 + (ent & 1023u);
+        const uint32_t f = ent >> 10;
+        const K k = keys[i];
+        const V v0 = vals[i];
+        K tg[2][1] = { { k + d2 }, { k + d4 } }, k_at[2][1];
+        const bool need[2][1] = { { (f & 2u) != 0u }, { (f & 4u) != 0u } };
+        int64_t r[2][1];
+        vwin_search<K, 1>(keys, n, mw, wr, tg, need, r, k_at);
+        // a set flag means the corner exists: i + 1, and the searched voxels
+        const V vn[3] = { vals[i + 1], vals[r[0][0]], vals[r[1][0]] };
+        uint32_t x0, y0, z0;
+        unpack_key<K>(k, x0, y0, z0);
+        // Volume::VoxelCenter(int, int, int), Volume.cpp:80-88
+        const float vx = (float)(int)x0 * mp.vl + mp.c0, vy = (float)(int)y0 * mp.vl + mp.c0,
+                    vz = (float)(int)z0 * mp.vl + mp.c0;
+        const float dself = vdist(v0);
+        uint32_t oo = sh_off[e];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (!(f >> a & 1u))
+                continue;
+            const float dn = vdist(vn[a]);
+            const float tt = fabsf(dself) / (fabsf(dself) + fabsf(dn)) * mp.vl;   // :143/153/163
+            verts[3 * (size_t)oo + 0] = a == 0 ? vx + tt : vx;
+            verts[3 * (size_t)oo + 1] = a == 1 ? vy + tt : vy;
+            verts[3 * (size_t)oo + 2] = a == 2 ? vz + tt : vz;
+            ++oo;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- Bin
 __device__ __forceinline__ bool to_int_checked(float f, int &out)
 {
@@ -492,7 +755,7 @@ static size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
 template <typename K, typename V>
 static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx,
-                             size_t icap, size_t *nverts, size_t *nidx, int *stalled)
+                             size_t icap, size_t *nverts, size_t *nidx, int *stalled, bool async_copy)
 {
     kifi_ctx *ctx = vol->ctx;
     hipStream_t st = ctx->stream;
@@ -511,38 +774,42 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     const uint64_t nt = (m + kScanTile - 1) / kScanTile + 1;
     const uint64_t T = (m + kMeshTile - 1) / kMeshTile;
     const int NO = tris ? 3 : 2;
-    size_t bytes = a256(m) + 3 * a256(m * 4) + a256(nt * 4) + 2 * a256(16) + a256((T + 1) * NO * 8);
+    // vertices only: flags (read 4 at a time) and tile counts; with indices:
+    // flags, per-voxel counts and their scans
+    size_t bytes = a256(m + 4) + a256(16) + a256(16) + a256((T + 1) * NO * 8);
+    bytes += tris ? 2 * a256(m * 4) + a256(nt * 4) : a256(T * 4);
     kifi_status s = ensure_scratch(ctx, bytes);
     if (s != KIFI_OK)
         return s;
     char *p = (char *)ctx->scratch;
-    uint8_t *vflags = (uint8_t *)p; p += a256(m);
-    uint32_t *cv_fwd = (uint32_t *)p; p += a256(m * 4);
-    uint32_t *cv_rev = (uint32_t *)p; p += a256(m * 4);
-    uint32_t *ci_rev = (uint32_t *)p; p += a256(m * 4);
-    uint32_t *tsum = (uint32_t *)p; p += a256(nt * 4);
+    uint8_t *vflags = (uint8_t *)p; p += a256(m + 4);
     uint64_t *totals = (uint64_t *)p; p += a256(16);
     unsigned long long *stall = (unsigned long long *)p; p += a256(16);
-    uint64_t *win = (uint64_t *)p;
+    uint64_t *win = (uint64_t *)p; p += a256((T + 1) * NO * 8);
+    uint32_t *cv_fwd = (uint32_t *)p, *ci_rev = cv_fwd, *tsum = cv_fwd, *tile_rev = cv_fwd;
+    if (tris) {
+        ci_rev = cv_fwd + a256(m * 4) / 4;
+        tsum = ci_rev + a256(m * 4) / 4;
+    }
 
     const K *keys = (const K *)vol->keys[vol->cur];
     const V *vals = (const V *)vol->vals[vol->cur];
     const unsigned grid = (unsigned)std::min<uint64_t>(T, (uint64_t)ctx->num_cus * 8);
+    const unsigned xgrid = (unsigned)((T + kXcds - 1) / kXcds * kXcds);   // a whole run per XCD
     const unsigned wgrid = (unsigned)(((T + 1) * NO + kMT - 1) / kMT);
     const int t_mesh = timer_begin(ctx, KIFI_STAGE_MESH);
     if (tris) {
         hipLaunchKernelGGL((k_mesh_windows<K, true>), dim3(wgrid), dim3(kMT), 0, st, keys, n, T, win);
         hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T,
                            vflags, cv_fwd, (uint32_t *)nullptr, ci_rev);
+        device_scan(st, cv_fwd, cv_fwd, m, tsum, totals);
+        device_scan(st, ci_rev, ci_rev, m, tsum, totals + 1);
     } else {
         hipLaunchKernelGGL((k_mesh_windows<K, false>), dim3(wgrid), dim3(kMT), 0, st, keys, n, T, win);
-        hipLaunchKernelGGL((k_mesh_count<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T,
-                           vflags, (uint32_t *)nullptr, cv_rev, (uint32_t *)nullptr);
+        hipLaunchKernelGGL((k_mesh_vcount<K, V>), dim3(xgrid), dim3(kMT), 0, st, keys, vals, n, win, T,
+                           vflags, tile_rev);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tile_rev, T, totals);
     }
-    uint32_t *voff = tris ? cv_fwd : cv_rev;
-    device_scan(st, voff, voff, m, tsum, totals);
-    if (tris)
-        device_scan(st, ci_rev, ci_rev, m, tsum, totals + 1);
     timer_end(ctx, KIFI_STAGE_MESH, t_mesh);
     uint64_t h_tot[2] = { 0, 0 };
     KIFI_HIP(ctx, hipMemcpyAsync(h_tot, totals, tris ? 16 : 8, hipMemcpyDeviceToHost, st));
@@ -563,6 +830,13 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     s = ensure_out(ctx, vb + ib + rb);
     if (s != KIFI_OK)
         return s;
+    if (async_copy && !ctx->out_stream) {
+        KIFI_HIP(ctx, hipStreamCreateWithFlags(&ctx->out_stream, hipStreamNonBlocking));
+        KIFI_HIP(ctx, hipEventCreateWithFlags(&ctx->out_ready, hipEventDisableTiming));
+        KIFI_HIP(ctx, hipEventCreateWithFlags(&ctx->out_copied, hipEventDisableTiming));
+    }
+    if (ctx->out_pending)          // the previous asynchronous copy still reads out_scratch
+        KIFI_HIP(ctx, hipStreamWaitEvent(st, ctx->out_copied, 0));
     char *q = (char *)ctx->out_scratch;
     float *d_verts = (float *)q; q += vb;
     uint32_t *d_idx = (uint32_t *)q; q += ib;
@@ -583,12 +857,24 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
                 *stalled = 1;
         }
     } else {
-        hipLaunchKernelGGL((k_mesh_emit<K, V, false>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T, mp,
-                           vflags, cv_rev, (uint32_t *)nullptr, d_verts, (uint32_t *)nullptr,
-                           (uint64_t *)nullptr, stall);
+        hipLaunchKernelGGL((k_mesh_vemit<K, V>), dim3(xgrid), dim3(kMT), 0, st, keys, vals, n, win, T, mp,
+                           vflags, tile_rev, d_verts);
     }
     timer_end(ctx, KIFI_STAGE_MESH, t_emit);
     KIFI_HIP(ctx, hipGetLastError());
+    if (async_copy) {
+        // the copy runs on its own stream behind the emit; the frames queued
+        // next on `stream` overlap it, and out_settle waits for its end
+        if (h_tot[0]) {
+            KIFI_HIP(ctx, hipEventRecord(ctx->out_ready, st));
+            KIFI_HIP(ctx, hipStreamWaitEvent(ctx->out_stream, ctx->out_ready, 0));
+            KIFI_HIP(ctx, hipMemcpyAsync(verts, d_verts, h_tot[0] * 12, hipMemcpyDeviceToHost,
+                                         ctx->out_stream));
+            KIFI_HIP(ctx, hipEventRecord(ctx->out_copied, ctx->out_stream));
+            ctx->out_pending = true;
+        }
+        return KIFI_OK;
+    }
     if (h_tot[0])
         KIFI_HIP(ctx, hipMemcpyAsync(verts, d_verts, h_tot[0] * 12, hipMemcpyDeviceToHost, st));
     if (tris && h_tot[1])
@@ -598,16 +884,17 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
 }
 
 kifi_status mesh_dispatch(kifi_volume *vol, bool tris, float *verts, size_t vcap, uint32_t *idx, size_t icap,
-                          size_t *nverts, size_t *nidx, int *stalled)
+                          size_t *nverts, size_t *nidx, int *stalled, bool async_copy)
 {
+    auto run = [&](auto impl) { return impl(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled, async_copy); };
     if (vol->desc.key_bits == 32) {
         if (vol->desc.storage == KIFI_STORAGE_F32)
-            return mesh_impl<uint32_t, VoxF32m>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
-        return mesh_impl<uint32_t, uint32_t>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+            return run(mesh_impl<uint32_t, VoxF32m>);
+        return run(mesh_impl<uint32_t, uint32_t>);
     }
     if (vol->desc.storage == KIFI_STORAGE_F32)
-        return mesh_impl<uint64_t, VoxF32m>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
-    return mesh_impl<uint64_t, uint32_t>(vol, tris, verts, vcap, idx, icap, nverts, nidx, stalled);
+        return run(mesh_impl<uint64_t, VoxF32m>);
+    return run(mesh_impl<uint64_t, uint32_t>);
 }
 
 kifi_status bin_host(kifi_ctx *ctx, const float *points, size_t n, const float w2c[16], int W, int H,

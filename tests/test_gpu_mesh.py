@@ -119,3 +119,39 @@ def test_mesh_into_page_locked_buffer(ctx):
     ok, ovals = ov.read()
     assert np.array_equal(gk, ok)
     assert np.array_equal(np.asarray(gvals).view(np.uint32).ravel(), np.asarray(ovals).view(np.uint32).ravel())
+
+
+def test_mesh_async_overlaps_later_frames(ctx):
+    """kifi_mesh_async / kifi_mesh_wait: each mesh is the volume as it was when
+    the call was made, although frames integrated after it (and a second
+    asynchronous mesh, and a synchronous one) are queued before the wait; a
+    too-small buffer reports the count and queues nothing; a pageable buffer
+    works too."""
+    gv, ov = _fused(ctx, mf.SCENE_ROOM, 2, mf.TRAJ_CIRCLE, [0, 3], 1024, 4.096, 0.02)
+    sc = mf.make_scene(mf.SCENE_ROOM, 2)
+    o1 = ov.mesh()
+    small = ctx.host_empty((16, 3), np.float32)
+    small[:] = 7.0
+    assert gv.mesh_into_async(small) == len(o1)
+    ctx.mesh_wait()
+    assert (small == 7.0).all()
+    buf1 = ctx.host_empty((len(o1) + 10, 3), np.float32)
+    assert gv.mesh_into_async(buf1) == len(o1)
+    for f in (6, 9, 12):
+        pose = mf.trajectory_pose(mf.TRAJ_CIRCLE, 2, f)
+        d = mf.render(sc, CAM, pose, noise_seed=500 + f)
+        w2e = mf.invert_rigid(pose).astype(np.float32)
+        gv.integrate(d, CAM, w2e)
+        ov.integrate(d, CAM, w2e)
+    o2 = ov.mesh()
+    buf2 = ctx.host_empty((len(o2) + 10, 3), np.float32)
+    assert gv.mesh_into_async(buf2) == len(o2)
+    g3 = gv.mesh()                                   # synchronous, behind the pending copies
+    assert np.array_equal(g3.view(np.uint32), o2.view(np.uint32))
+    pageable = np.zeros((len(o2) + 10, 3), np.float32)
+    assert gv.mesh_into_async(pageable) == len(o2)
+    ctx.mesh_wait()
+    assert np.array_equal(buf1[:len(o1)].view(np.uint32), o1.view(np.uint32))
+    assert np.array_equal(buf2[:len(o2)].view(np.uint32), o2.view(np.uint32))
+    assert np.array_equal(pageable[:len(o2)].view(np.uint32), o2.view(np.uint32))
+    assert len(o2) != len(o1)
