@@ -7,11 +7,11 @@
 // frame, 4x4-block mean depth / 4 m), five 3x3 stride-2 convolutions with
 // padding 1 (16-32-64-64-128 channels, ReLU), fully connected 2048 -> 256 (ReLU)
 // -> 6.  Weights and activations are bfloat16, accumulation is binary32
-// (v_mfma_f32_16x16x16_bf16), master weights and Adam moments binary32.
+// (v_mfma_f32_16x16x32_bf16), master weights and Adam moments binary32.
 //
 // Every layer is one implicit-GEMM kernel (forward, backward-data and
 // backward-weight), 64 x 64 output tile per workgroup, 32 x 32 per wave as 2 x 2
-// MFMA 16x16x16 tiles fed straight from the gathered operands; weight gradients
+// MFMA 16x16x32 tiles fed straight from the gathered operands; weight gradients
 // use a fixed split of the reduction dimension and a fixed-order sum, so a
 // training run is deterministic.
 #include <algorithm>
@@ -33,7 +33,8 @@ using namespace kifi;
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));      // bf16 bits
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // SplitMix64 (Steele, Lea, Flood 2014), the synthetic frames' generator
@@ -78,34 +79,38 @@ uint16_t f2bf_host(float f)
 // ------------------------------------------------------------------ GEMM core
 // C[M x N] (+)= A[M x K] B[K x N]; Op supplies a(m, k), b(k, n) as bf16 bits (0
 // outside the matrix) and store(m, n, value, split).  blockIdx.z splits K into
-// chunks of op.kchunk (a multiple of 16).
+// chunks of op.kchunk (a multiple of 32).  Operand layout of
+// v_mfma_f32_16x16x32_bf16: lane l holds row (A) / column (B) l % 16 at eight
+// consecutive k from 8 (l / 16); A and B share the k assignment.
 template <class Op>
 __global__ __launch_bounds__(256) void k_gemm(Op op)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m0 = blockIdx.x * 64 + (wave >> 1) * 32, n0 = blockIdx.y * 64 + (wave & 1) * 32;
     const int kb = blockIdx.z * op.kchunk, ke = min(op.K, kb + op.kchunk);
-    const int r = lane & 15, kq = 4 * (lane >> 4);
+    const int r = lane & 15, kq = 8 * (lane >> 4);
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
-    for (int k = kb; k < ke; k += 16) {
-        bf16x4 a[2], b[2];
+    for (int k = kb; k < ke; k += 32) {
+        bf16x8 a[2], b[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            op.a4(m0 + 16 * i + r, k + kq, ke, a[i]);
-            op.b4(k + kq, n0 + 16 * i + r, ke, b[i]);
+            op.a8(m0 + 16 * i + r, k + kq, ke, a[i]);
+            op.b8(k + kq, n0 + 16 * i + r, ke, b[i]);
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mfma_bf16x8, a[i]),
+                                                                    __builtin_bit_cast(mfma_bf16x8, b[j]),
+                                                                    acc[i][j], 0, 0, 0);
     }
-    // D layout of v_mfma_f32_16x16x16_bf16: lane holds rows 4*(lane/16)+e of column lane%16
+    // D layout of v_mfma_f32_16x16x32_bf16: lane holds rows 4*(lane/16)+e of column lane%16
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -125,29 +130,30 @@ void launch_gemm(const Op &op, int splits, hipStream_t st)
     hipLaunchKernelGGL(k_gemm<Op>, grid, dim3(256), 0, st, op);
 }
 
-// Four consecutive k (k0 .. k0+3) of an operand row or column, 0 at k >= ke.
+// Eight consecutive k (k0 .. k0+7) of an operand row or column, 0 at k >= ke.
 // The default takes them one element at a time; an operator whose element
 // index needs runtime divisions replaces it with one decomposition per call.
 template <class D>
 struct GemmElems {
-    __device__ void a4(int m, int k0, int ke, bf16x4 &o) const
+    __device__ void a8(int m, int k0, int ke, bf16x8 &o) const
     {
         const D &d = static_cast<const D &>(*this);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 8; ++j)
             o[j] = (short)(k0 + j < ke ? d.a(m, k0 + j) : 0);
     }
-    __device__ void b4(int k0, int n, int ke, bf16x4 &o) const
+    __device__ void b8(int k0, int n, int ke, bf16x8 &o) const
     {
         const D &d = static_cast<const D &>(*this);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 8; ++j)
             o[j] = (short)(k0 + j < ke ? d.b(k0 + j, n) : 0);
     }
 };
 
-// Split of a reduction dimension K into `splits` chunks of multiples of 16.
-inline int chunk_of(int K, int splits) { return ((K + splits - 1) / splits + 15) / 16 * 16; }
+// Split of a reduction dimension K into `splits` chunks of multiples of 32.
+inline int chunk_of(int K, int splits) { return ((K + splits - 1) / splits + 31) / 32 * 32; }
+inline int chunk_all(int K) { return (K + 31) / 32 * 32; }
 
 struct ConvGeom {
     int B, Ci, Hi, Wi, Co, Ho, Wo;
@@ -240,16 +246,16 @@ struct OpConvBwdW : GemmElems<OpConvBwdW> {
             return 0;
         return X[((size_t)(b * g.Ci + ci) * g.Hi + hi) * g.Wi + wi];
     }
-    // Four k of a row / column: k = (b, ho, wo) is decomposed once and stepped,
+    // Eight k of a row / column: k = (b, ho, wo) is decomposed once and stepped,
     // and every load is issued (indices clamped into the tensors) before the
     // masks apply.  Same values as a() / b().
-    __device__ void a4(int m, int k0, int ke, bf16x4 &o) const
+    __device__ void a8(int m, int k0, int ke, bf16x8 &o) const
     {
         const int hw = g.Ho * g.Wo, mm = min(m, M - 1);
         int b = k0 / hw, p = k0 - b * hw;
-        uint16_t v[4];
+        uint16_t v[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const size_t idx = k0 + j < K ? ((size_t)(b * g.Co + mm)) * hw + p : 0;
             v[j] = dY[idx];
             if (++p == hw) {
@@ -258,18 +264,18 @@ struct OpConvBwdW : GemmElems<OpConvBwdW> {
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 8; ++j)
             o[j] = (short)(m < M && k0 + j < ke ? v[j] : 0);
     }
-    __device__ void b4(int k0, int n, int ke, bf16x4 &o) const
+    __device__ void b8(int k0, int n, int ke, bf16x8 &o) const
     {
         const int hw = g.Ho * g.Wo, nn = min(n, N - 1);
         const int ci = nn / 9, t = nn - ci * 9, ky = t / 3, kx = t - ky * 3;
         int b = k0 / hw, p = k0 - b * hw, ho = p / g.Wo, wo = p - ho * g.Wo;
-        uint16_t v[4];
-        bool ok[4];
+        uint16_t v[8];
+        bool ok[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const int hi = 2 * ho - 1 + ky, wi = 2 * wo - 1 + kx;
             ok[j] = n < N && k0 + j < ke && hi >= 0 && hi < g.Hi && wi >= 0 && wi < g.Wi;
             const int bc = min(b, g.B - 1), hc = min(max(hi, 0), g.Hi - 1), wc = min(max(wi, 0), g.Wi - 1);
@@ -283,7 +289,7 @@ struct OpConvBwdW : GemmElems<OpConvBwdW> {
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 8; ++j)
             o[j] = (short)(ok[j] ? v[j] : 0);
     }
     __device__ void store(int m, int n, float v, int z) const { part[((size_t)z * M + m) * N + n] = v; }
@@ -989,7 +995,7 @@ void net_forward(kifi_twistnet *n, int B, hipStream_t st)
         op.M = B * op.g.Ho * op.g.Wo;
         op.N = op.g.Co;
         op.K = op.g.Ci * 9;
-        op.kchunk = (op.K + 15) / 16 * 16;
+        op.kchunk = chunk_all(op.K);
         if (B > kSmallBatch || !launch_conv_small(op, st))
             launch_gemm(op, 1, st);
     }
@@ -1063,7 +1069,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         op.M = kOut;
         op.N = kFc1;
         op.K = B;
-        op.kchunk = (B + 15) / 16 * 16;
+        op.kchunk = chunk_all(B);
         op.ldy = kOut;
         launch_gemm(op, 1, st);
         sum_splits(n, 1, kOut * kFc1, n->g + L.w[6], st);
@@ -1076,7 +1082,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         od.M = B;
         od.N = kFc1;
         od.K = kOut;
-        od.kchunk = 16;
+        od.kchunk = chunk_all(kOut);
         launch_gemm(od, 1, st);
     }
     // fc1
@@ -1089,7 +1095,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         op.M = kFc1;
         op.N = fin;
         op.K = B;
-        op.kchunk = (B + 15) / 16 * 16;
+        op.kchunk = chunk_all(B);
         op.ldy = kFc1;
         launch_gemm(op, 1, st);
         sum_splits(n, 1, kFc1 * fin, n->g + L.w[5], st);
@@ -1102,7 +1108,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         od.M = B;
         od.N = fin;
         od.K = kFc1;
-        od.kchunk = (kFc1 + 15) / 16 * 16;
+        od.kchunk = chunk_all(kFc1);
         launch_gemm(od, 1, st);
     }
     for (int l = kNumConv - 1; l >= 0; --l) {
@@ -1132,7 +1138,7 @@ void net_backward(kifi_twistnet *n, int B, hipStream_t st)
         od.M = B * g.Hi * g.Wi;
         od.N = g.Ci;
         od.K = g.Co * 9;
-        od.kchunk = (od.K + 15) / 16 * 16;
+        od.kchunk = chunk_all(od.K);
         launch_gemm(od, 1, st);
     }
 }
