@@ -132,17 +132,31 @@ __device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, int pi
     return valid;
 }
 
-constexpr int kKgItems = 4;   // pixels per thread: one counter atomic per 1024 pixels
+constexpr int kKgItems = 1;   // pixels per thread: a workgroup's 256 pixels give at most 2048 keys, one radix tile
+static_assert(kKgItems * kThreads * 8 == kRsTile, "a keygen workgroup fills at most one radix tile");
 
 // dm_out (uint16 frames): receives the frame in metres for the voxel update
-template <typename K, typename D>
+// Workgroup t writes its keys to radix tile t (out + t kRsTile; its key count
+// | valid pixels << 16 to tile_n[t]) and the tile's counts of the first radix
+// digit to counts[d Tb + t], so the sort's first pass needs no upsweep.  No
+// counter atomics: the first pass's scan sums tile_n into the frame's counts.  The four keys of a half-block
+// differ only in y and z, above the first digit's bits, so each thread adds to
+// at most two digit counters.
+template <typename K, typename D, int DB>
 __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth,
                                                      KeygenParams p, K *__restrict__ out,
-                                                     FrameCounters *__restrict__ cnt, float *__restrict__ dm_out)
+                                                     FrameCounters *__restrict__ cnt, float *__restrict__ dm_out,
+                                                     uint32_t *__restrict__ counts, int Tb,
+                                                     uint32_t *__restrict__ tile_n)
 {
+    constexpr int ND = 1 << DB;
+    constexpr uint32_t MASK = ND - 1;
+    static_assert(KeyTraits<K>::kb >= DB, "the y and z offsets leave the first digit unchanged");
     __shared__ uint32_t tmp[kThreads / kWave + 1];
-    __shared__ uint32_t base_s;
+    __shared__ uint32_t hist[ND];
     const int P = p.W * p.H;
+    for (int q = threadIdx.x; q < ND; q += kThreads)
+        hist[q] = 0;
     if (p.zbuf)
         for (int i = blockIdx.x * kThreads + threadIdx.x; i < p.nz; i += gridDim.x * kThreads)
             p.zbuf[i] = ~0ull;
@@ -190,17 +204,26 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
         halves |= h << (2 * j);
         nw += 4u * ((h & 1u) + (h >> 1));
     }
-    // one scan for both counts: written candidates (<= 16384 per block) and valid pixels
+    // one scan for both counts: written candidates (<= 2048 per block) and valid
+    // pixels; its barriers also order the histogram's clearing before the adds
     uint32_t tot;
     const uint32_t ex = block_exclusive_scan<kThreads>(nw | (c << 16), tmp, tot);
-    if (threadIdx.x == 0 && tot) {
-        base_s = atomicAdd(&cnt->n_cand, tot & 0xffffu);
-        atomicAdd(&cnt->n_keys, tot >> 16);
+#pragma unroll
+    for (int j = 0; j < kKgItems; ++j) {
+        const uint32_t h = halves >> (2 * j) & 3u;
+        if (h & 1u)
+            atomicAdd(&hist[(uint32_t)keys[j] & MASK], 4u);
+        if (h & 2u)
+            atomicAdd(&hist[(uint32_t)(keys[j] + dx) & MASK], 4u);
     }
+    if (threadIdx.x == 0)
+        tile_n[blockIdx.x] = tot;
     __syncthreads();
+    for (int q = threadIdx.x; q < ND; q += kThreads)
+        counts[(size_t)q * Tb + blockIdx.x] = hist[q];
     if (!nw)
         return;
-    K *o = out + base_s + (ex & 0xffffu);
+    K *o = out + (size_t)blockIdx.x * kRsTile + (ex & 0xffffu);
 #pragma unroll
     for (int j = 0; j < kKgItems; ++j) {
         const uint32_t h = halves >> (2 * j) & 3u;
@@ -265,11 +288,30 @@ __global__ __launch_bounds__(kThreads) void k_rs_upsweep(const K *__restrict__ k
 
 // Per digit (one workgroup each): exclusive scan of the digit's tile counts,
 // in place, and the digit's total (no atomics; the downsweep scans the totals).
+// First pass: one more workgroup sums keygen's tile counts into the frame's
+// candidate count and valid-pixel count.
 __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ counts, int Tb,
-                                                      uint32_t *__restrict__ totals)
+                                                      uint32_t *__restrict__ totals,
+                                                      const uint32_t *__restrict__ tile_n,
+                                                      FrameCounters *__restrict__ cnt)
 {
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     const int d = blockIdx.x;
+    if (tile_n && d == (int)gridDim.x - 1) {
+        uint32_t keys = 0, px = 0;
+        for (int t = threadIdx.x; t < Tb; t += kThreads) {
+            const uint32_t v = tile_n[t];
+            keys += v & 0xffffu;
+            px += v >> 16;
+        }
+        keys = block_reduce_sum<kThreads>(keys, tmp);
+        px = block_reduce_sum<kThreads>(px, tmp);
+        if (threadIdx.x == 0) {
+            cnt->n_cand = keys;
+            cnt->n_keys = px;
+        }
+        return;
+    }
     uint32_t carry = 0;
     uint32_t *row = counts + (size_t)d * Tb;
     constexpr int IT = 4;
@@ -295,12 +337,15 @@ __global__ __launch_bounds__(kThreads) void k_rs_scan(uint32_t *__restrict__ cou
         totals[d] = carry;
 }
 
+// tile_n (the first pass): tile t holds tile_n[t] & 0xffff keys at its start
+// (keygen's tiles); otherwise the n keys are dense.
 template <typename K, int DB>
 __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__ in,
                                                            K *__restrict__ out,
                                                            const uint32_t *__restrict__ d_n, int shift,
                                                            const uint32_t *__restrict__ offsets, int Tb,
-                                                           const uint32_t *__restrict__ totals)
+                                                           const uint32_t *__restrict__ totals,
+                                                           const uint32_t *__restrict__ tile_n)
 {
     constexpr int ND = 1 << DB;
     constexpr uint32_t MASK = ND - 1;
@@ -330,15 +375,21 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
         gtot[e] = totals[d0 + e];
         goff[e] = offsets[(size_t)(d0 + e) * Tb + blockIdx.x];
     }
-    const uint32_t n = *d_n;
-    if (tbase >= n)
+    uint32_t tn;          // keys of this tile, from its start
+    if (tile_n) {
+        tn = tile_n[blockIdx.x] & 0xffffu;
+    } else {
+        const uint32_t n = *d_n;
+        tn = tbase < n ? (uint32_t)min((size_t)kRsTile, (size_t)n - tbase) : 0u;
+    }
+    if (tn == 0)
         return;
+    const uint32_t wloc = (uint32_t)w * (kWave * kRsItems);   // the wave's first key in the tile
     for (int q = lane; q < ND; q += kWave)
         wcnt[w][q] = 0;
 #pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
-        const size_t i = wbase + (size_t)j * kWave + lane;
-        const bool ok = i < n;
+        const bool ok = wloc + (uint32_t)(j * kWave + lane) < tn;
         const K k = keys[j];
         const uint32_t d = (uint32_t)(k >> shift) & MASK;
         const uint64_t act = __ballot(ok);
@@ -384,14 +435,12 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(const K *__restrict__
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRsItems; ++j) {
-        const size_t i = wbase + (size_t)j * kWave + lane;
-        if (i < n) {
+        if (wloc + (uint32_t)(j * kWave + lane) < tn) {
             const uint32_t d = (uint32_t)(keys[j] >> shift) & MASK;
             stage[dstart[d] + wcnt[w][d] + rank[j]] = keys[j];
         }
     }
     __syncthreads();
-    const uint32_t tn = (uint32_t)min((size_t)kRsTile, (size_t)n - tbase);
     for (uint32_t i = threadIdx.x; i < tn; i += kThreads) {
         const K k = stage[i];
         const uint32_t d = (uint32_t)(k >> shift) & MASK;
@@ -1453,6 +1502,7 @@ struct IntegrateScratch {
     uint32_t *rs_totals;     // [passes][digits]
     uint32_t *chunk_kept;    // [Tb]
     uint32_t *chunk_heads;   // [Tb]
+    uint32_t *tile_n;        // [Tb] keys per keygen tile
     uint64_t *rank;          // [8P] the kept candidates' volume ranks, per chunk
     uint64_t *on;            // [8P] merged positions of the new voxels
     float    *depth_m;       // [P] a uint16 frame in metres
@@ -1474,6 +1524,7 @@ static IntegrateScratch layout_scratch(char *base, size_t P, size_t kbytes, int 
     s.rs_totals = (uint32_t *)(base + off); off += align256((size_t)passes * digits * 4);
     s.chunk_kept = (uint32_t *)(base + off); off += align256(Tb * 4);
     s.chunk_heads = (uint32_t *)(base + off); off += align256(Tb * 4);
+    s.tile_n = (uint32_t *)(base + off); off += align256(Tb * 4);
     s.rank = (uint64_t *)(base + off); off += align256(ncand * 8);
     s.on = (uint64_t *)(base + off); off += align256(ncand * 8);
     s.depth_m = (float *)(base + off); off += align256(P * 4);
@@ -1558,25 +1609,33 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     kp.zbuf = model_start ? splat->zbuf : nullptr;
     kp.nz = model_start ? splat->W * splat->H : 0;
     const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);
-    const dim3 kg_grid((unsigned)((P + kThreads * kKgItems - 1) / (kThreads * kKgItems)));
+    // keygen: one workgroup per radix tile, with the first pass's tile counts
+    const size_t kg_blocks = (P + kThreads * kKgItems - 1) / (kThreads * kKgItems);
+    if (kg_blocks != Tb)
+        return set_error(ctx, KIFI_ERR_STATE, "integrate: keygen tiles do not match the radix tiles");
     const float *frame_m = d_depth.metres ? (const float *)d_depth.p : s.depth_m;
     if (d_depth.metres)
-        hipLaunchKernelGGL((k_keygen<K, float>), kg_grid, dim3(kThreads), 0, stream, (const float *)d_depth.p, kp,
-                           (K *)s.candA, cnt, (float *)nullptr);
+        hipLaunchKernelGGL((k_keygen<K, float, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream,
+                           (const float *)d_depth.p, kp, (K *)s.candA, cnt, (float *)nullptr, s.rs_counts, (int)Tb,
+                           s.tile_n);
     else
-        hipLaunchKernelGGL((k_keygen<K, uint16_t>), kg_grid, dim3(kThreads), 0, stream,
-                           (const uint16_t *)d_depth.p, kp, (K *)s.candA, cnt, s.depth_m);
+        hipLaunchKernelGGL((k_keygen<K, uint16_t, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream,
+                           (const uint16_t *)d_depth.p, kp, (K *)s.candA, cnt, s.depth_m, s.rs_counts, (int)Tb,
+                           s.tile_n);
 
-    // radix sort of the candidates (the sorted keys end in src)
+    // radix sort of the candidates (the sorted keys end in src); the first
+    // pass sorts keygen's partly filled tiles into dense order
     K *src = (K *)s.candA, *dst = (K *)s.candB;
     for (int pass = 0; pass < passes; ++pass) {
         const int shift = DB * pass;
         uint32_t *tot = s.rs_totals + pass * ND;
-        hipLaunchKernelGGL((k_rs_upsweep<K, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
-                           &cnt->n_cand, shift, s.rs_counts, (int)Tb);
-        hipLaunchKernelGGL(k_rs_scan, dim3(ND), dim3(kThreads), 0, stream, s.rs_counts, (int)Tb, tot);
+        if (pass > 0)
+            hipLaunchKernelGGL((k_rs_upsweep<K, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream, src,
+                               &cnt->n_cand, shift, s.rs_counts, (int)Tb);
+        hipLaunchKernelGGL(k_rs_scan, dim3(pass == 0 ? ND + 1 : ND), dim3(kThreads), 0, stream, s.rs_counts,
+                           (int)Tb, tot, pass == 0 ? s.tile_n : nullptr, cnt);
         hipLaunchKernelGGL((k_rs_downsweep<K, DB>), dim3((unsigned)Tb), dim3(kThreads), 0, stream, src, dst,
-                           &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot);
+                           &cnt->n_cand, shift, s.rs_counts, (int)Tb, tot, pass == 0 ? s.tile_n : nullptr);
         K *t = src; src = dst; dst = t;
     }
 
