@@ -354,12 +354,16 @@ kifi_status kifi_integrate_totals(kifi_volume *v, uint64_t totals[6])
     KIFI_HIP(ctx, hipStreamSynchronize(ctx->stream));
     FrameCounters c;
     KIFI_HIP(ctx, hipMemcpy(&c, v->d_counters, sizeof c, hipMemcpyDeviceToHost));
+    uint64_t upd[kUpdSlots * kUpdStride];
+    KIFI_HIP(ctx, hipMemcpy(upd, v->d_upd, sizeof upd, hipMemcpyDeviceToHost));
     totals[0] = c.tot_frames;
     totals[1] = c.tot_v_old;
     totals[2] = c.tot_new;
     totals[3] = c.tot_v_new;
     totals[4] = c.tot_keys;
-    totals[5] = c.tot_upd;
+    totals[5] = 0;
+    for (int i = 0; i < kUpdSlots; ++i)
+        totals[5] += upd[i * kUpdStride];
     return KIFI_OK;
 }
 
@@ -473,6 +477,8 @@ kifi_status kifi_volume_create(kifi_ctx *ctx, const kifi_volume_desc *d, kifi_vo
     v->consts.trunc = d->trunc;
     const uint64_t zero = 0;
     hipError_t e = hipMalloc(&v->d_counters, sizeof(FrameCounters));
+    if (e == hipSuccess) e = hipMalloc(&v->d_upd, kUpdSlots * kUpdStride * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemset(v->d_upd, 0, kUpdSlots * kUpdStride * sizeof(uint64_t));
     if (e == hipSuccess) e = hipHostMalloc(&v->h_ring, kifi_volume::kSizeRing * sizeof(FrameCounters),
                                            hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&v->h_ring_dev, v->h_ring, 0);
@@ -509,6 +515,7 @@ void kifi_volume_destroy(kifi_volume *v)
         if (v->samp2[b]) (void)hipFree(v->samp2[b]);
     }
     if (v->d_counters) (void)hipFree(v->d_counters);
+    if (v->d_upd) (void)hipFree(v->d_upd);
     if (v->h_ring) (void)hipHostFree(v->h_ring);
     if (v->d_size) (void)hipFree(v->d_size);
     if (v->d_mats) (void)hipFree(v->d_mats);

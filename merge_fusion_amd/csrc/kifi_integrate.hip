@@ -66,7 +66,7 @@ __global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, 
     if (icp_end)
         icp_frame_end_dev(icp_end, w2e_dev);
     c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
-    c->v_new = 0; c->n_tiles = 0; c->overflow = 0; c->n_upd = 0;
+    c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
     FrameMats f;
     f.w2e = w2e_dev ? *w2e_dev : w2e_val;
     mat_mul(clip.m, f.w2e.m, f.w2c.m);
@@ -1011,6 +1011,7 @@ struct UpdateParams {
     float vl, c0, trunc, ndcU, ndcV, maxU, maxV;
     int   W, H;
     const float *depth;      // the frame in metres
+    unsigned long long *upd; // A's running counters (kUpdSlots x kUpdStride)
 };
 
 // Integrator::UpdateVoxels per voxel (Integrator.cpp:285-356), in two halves so
@@ -1280,7 +1281,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
     const K *__restrict__ A, const V *__restrict__ Av, const K *__restrict__ B, const uint64_t *__restrict__ on,
     const uint64_t *__restrict__ part_a, const uint64_t *__restrict__ part_b,
     const FrameCounters *__restrict__ cnt, K *__restrict__ O, V *__restrict__ Ov, K *__restrict__ OS1,
-    K *__restrict__ OS2, UpdateParams up, SplatParams sp, FrameCounters *__restrict__ upd_cnt)
+    K *__restrict__ OS2, UpdateParams up, SplatParams sp)
 {
     __shared__ K sA[kMgTile];
     __shared__ uint32_t h[kHistWords];   // per A position: new keys sorting just below it
@@ -1484,12 +1485,10 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(
             }
         }
     }
-    // A, the frame's updated voxels: one atomic per tile
+    // A, the updated voxels: one atomic per tile on the tile's running counter
     nupd = block_reduce_sum<kThreads>(nupd, tmp);
-    if (threadIdx.x == 0 && nupd) {
-        atomicAdd(&upd_cnt->n_upd, nupd);
-        atomicAdd((unsigned long long *)&upd_cnt->tot_upd, (unsigned long long)nupd);
-    }
+    if (threadIdx.x == 0 && nupd)
+        atomicAdd(&up.upd[(blockIdx.x % kUpdSlots) * kUpdStride], (unsigned long long)nupd);
 }
 
 // ====================================================================== host
@@ -1669,17 +1668,18 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     up.maxV = (float)H;
     up.W = W; up.H = H;
     up.depth = frame_m;
+    up.upd = (unsigned long long *)vol->d_upd;
     const int t_mu = timer_begin(ctx, KIFI_STAGE_MERGE_UPDATE);
     if (splat)
         hipLaunchKernelGGL((k_merge_update<K, V, true>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                            (const K *)vol->keys[cur], (const V *)vol->vals[cur], Bnew, s.on, part_a, part_b,
                            cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], (K *)vol->samp1[nxt],
-                           (K *)vol->samp2[nxt], up, *splat, cnt);
+                           (K *)vol->samp2[nxt], up, *splat);
     else
         hipLaunchKernelGGL((k_merge_update<K, V, false>), dim3((unsigned)Tm), dim3(kThreads), 0, stream,
                            (const K *)vol->keys[cur], (const V *)vol->vals[cur], Bnew, s.on, part_a, part_b,
                            cnt, (K *)vol->keys[nxt], (V *)vol->vals[nxt], (K *)vol->samp1[nxt],
-                           (K *)vol->samp2[nxt], up, SplatParams{}, cnt);
+                           (K *)vol->samp2[nxt], up, SplatParams{});
     timer_end(ctx, KIFI_STAGE_MERGE_UPDATE, t_mu);
     KIFI_HIP(ctx, hipGetLastError());
     vol->cur = nxt;

@@ -208,11 +208,17 @@ struct FrameCounters {
     uint64_t v_new;       // V1
     uint32_t n_tiles;     // merge tiles this frame
     uint32_t overflow;    // set when a device-side capacity check fails
-    uint32_t n_upd;       // A: voxels whose TSDF the update changed
     uint32_t n_kept;      // new voxels: V1 - V0
     // running totals over all integrations (never reset by a frame)
-    uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys, tot_upd;
+    uint64_t tot_frames, tot_v_old, tot_new, tot_v_new, tot_keys;
 };
+
+// Running count of the voxels whose TSDF an update changed (A), spread over
+// kUpdSlots counters kUpdStride words apart (one 128-byte line each): merge
+// tile t adds to counter t % kUpdSlots, so the tiles' atomics do not queue on
+// one address.  The total is the counters' sum.
+constexpr int kUpdSlots = 64;
+constexpr int kUpdStride = 16;
 
 } // namespace kifi
 
@@ -277,6 +283,7 @@ struct kifi_volume {
     int              cur = 0;         // which buffer pair holds the volume
     size_t           cap = 0;         // capacity of each buffer (voxels)
     kifi::FrameCounters *d_counters = nullptr;   // device
+    uint64_t        *d_upd = nullptr;            // device: kUpdSlots x kUpdStride, A's running counters
     uint64_t        *d_size = nullptr;           // device-resident volume size
     kifi::FrameMats *d_mats = nullptr;           // matrices of the last integration
     // Size read-back without stalling the stream: each integration's last
