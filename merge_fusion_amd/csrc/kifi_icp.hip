@@ -1185,6 +1185,66 @@ __global__ __launch_bounds__(kIT) void k_icp_level_tail(IcpLevel lv, ModelParams
     }
 }
 
+// The rest of level level_t (as k_icp_level_tail, on that level's own
+// workgroup count ntb: the first ntb workgroups of the grid take part, the
+// others wait) and then the first iteration of the next finer level (as
+// k_icp_assoc, on the whole grid).  A frame whose level level_t converged
+// within its launched iterations, the common case, reads the flag and goes
+// straight on: one launch for the tail and the next level's first iteration
+// instead of two.  The sums of both parts are those of their own launches bit
+// for bit.
+template <bool COMP>
+__global__ __launch_bounds__(kIT) void k_icp_tail_assoc(IcpLevel lt, float gate_t, IcpSolveParams sp_t, int level_t,
+                                                        int n_iters, int ntb, int slot, IcpLevel lv, float gate,
+                                                        IcpSolveParams sp, int level, ModelParams mp,
+                                                        const ModelPose *__restrict__ mpose,
+                                                        const float4 *__restrict__ mpts,
+                                                        const float4 *__restrict__ mnrm, IcpState *st,
+                                                        CSum *__restrict__ partials, unsigned spin_limit)
+{
+    constexpr int MODE = KIFI_ICP_POINT_TO_PLANE;
+    constexpr int NV = IcpNV<MODE>::nv;
+    __shared__ CsumLds<NV> lds;
+    __shared__ float s_pose[16];
+    __shared__ int s_done, s_next_done;
+    unsigned gen = 0;
+    for (int it = 0;; ++it) {
+        // the state after the last hand-off (coherent loads)
+        if (threadIdx.x < 16)
+            s_pose[threadIdx.x] = __hip_atomic_load(&st->pose_f.m[threadIdx.x], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        else if (threadIdx.x == 16)
+            s_done = __hip_atomic_load(&st->level_done[level_t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (threadIdx.x == 17)
+            s_next_done = __hip_atomic_load(&st->level_done[level], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const bool tail_done = s_done != 0 || it >= n_iters;
+        const bool next_done = s_next_done != 0;
+        Mat4 pose;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            pose.m[i] = s_pose[i];
+        __syncthreads();     // read by every wave before the next iteration rewrites them
+        if (tail_done) {
+            if (next_done)
+                return;
+            CSum acc[NV];
+            icp_accumulate<MODE, COMP>(lv, mp, mpose, mpts, mnrm, pose, gate, (int)gridDim.x, acc);
+            icp_publish<MODE, COMP>(acc, lds, partials, st, level, sp, mpose);
+            return;
+        }
+        ++gen;
+        if ((int)blockIdx.x < ntb) {
+            CSum acc[NV];
+            icp_accumulate<MODE, COMP>(lt, mp, mpose, mpts, mnrm, pose, gate_t, ntb, acc);
+            icp_publish<MODE, COMP, NV, true>(acc, lds, partials, st, level_t, sp_t, mpose, gen, ntb,
+                                              &st->tail_gen[slot]);
+        }
+        if (!icp_wait_gen(st, gen, spin_limit, &st->tail_gen[slot]))
+            return;
+    }
+}
+
 // Frame end; with w2e set, also the binary32 world->eye of the tracked pose
 // (as k_icp_w2e writes it).
 __global__ void k_icp_frame_end(IcpState *__restrict__ st, Mat4 *__restrict__ w2e)
@@ -1588,10 +1648,16 @@ static bool icp_tail_ok(kifi_ctx *ctx)
 {
     if (ctx->tail_ok < 0) {
         int a = 0, b = 0;
+        int c = 0, d = 0;
         const hipError_t e0 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_icp_level_tail<true>, kIT, 0);
         const hipError_t e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_icp_level_tail<false>, kIT, 0);
+        const hipError_t e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_icp_tail_assoc<true>, kIT, 0);
+        const hipError_t e3 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_icp_tail_assoc<false>, kIT, 0);
         (void)hipGetLastError();
-        ctx->tail_ok = e0 == hipSuccess && e1 == hipSuccess && a >= 1 && b >= 1 ? 1 : 0;
+        ctx->tail_ok = e0 == hipSuccess && e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess && a >= 1 &&
+                               b >= 1 && c >= 1 && d >= 1
+                           ? 1
+                           : 0;
     }
     return ctx->tail_ok != 0;
 }
@@ -1602,6 +1668,22 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
     hipStream_t st = m->ctx->stream;
     const bool tails = ip->mode != KIFI_ICP_HORN && icp_tail_ok(m->ctx);
     const unsigned spin_limit = ip->spin_limit > 0 ? (unsigned)ip->spin_limit : 1u << 18;
+    // the rest of a coarser level still to run: in the next level's first
+    // launch (k_icp_tail_assoc), or on its own before a level with no launches
+    struct Tail {
+        bool pending = false;
+        IcpLevel lv;
+        float gate;
+        IcpSolveParams sp;
+        int level, nb, iters;
+    } tail;
+    auto launch_tail = [&]() {
+        auto k = ip->naive_sums ? k_icp_level_tail<false> : k_icp_level_tail<true>;
+        hipLaunchKernelGGL(k, dim3(tail.nb), dim3(kIT), 0, st, tail.lv, m->mp, m->d_pose, m->d_pts, m->d_nrm,
+                           w->d_state, tail.gate, (CSum *)w->partials, tail.level, tail.sp, tail.iters, spin_limit,
+                           pass * 3 + tail.level);
+        tail.pending = false;
+    };
     // Horn mode is the reference's ICP::Align: full resolution only
     for (int l = ip->mode == KIFI_ICP_HORN ? 0 : levels - 1; l >= 0; --l) {
         const IcpSolveParams sp = solve_params(ip, l);
@@ -1609,11 +1691,19 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
         const IcpLevel lv = icp_level(w, cam, ip, l, &gate);
         const int nb = icp_blocks(m->ctx, lv);
         const int launched = tails ? std::min(ip->iters[l], kLaunchedIters[l]) : ip->iters[l];
+        if (tail.pending && (launched == 0 || tail.nb > nb))
+            launch_tail();
         for (int it = 0; it < launched; ++it) {
             if (ip->mode == KIFI_ICP_HORN) {
                 const int nbw = std::min((((lv.W + 3) / 4) * lv.H + kIT - 1) / kIT, std::max(1, m->ctx->num_cus));
                 hipLaunchKernelGGL(k_icp_assoc_window, dim3(nbw), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts,
                                    w->d_state, (CSum *)w->partials, l, sp);
+            } else if (tail.pending) {
+                auto k = ip->naive_sums ? k_icp_tail_assoc<false> : k_icp_tail_assoc<true>;
+                hipLaunchKernelGGL(k, dim3(nb), dim3(kIT), 0, st, tail.lv, tail.gate, tail.sp, tail.level, tail.iters,
+                                   tail.nb, pass * 3 + tail.level, lv, gate, sp, l, m->mp, m->d_pose, m->d_pts,
+                                   m->d_nrm, w->d_state, (CSum *)w->partials, spin_limit);
+                tail.pending = false;
             } else {
                 if (ip->naive_sums)
                     hipLaunchKernelGGL((k_icp_assoc<KIFI_ICP_POINT_TO_PLANE, false>), dim3(nb), dim3(kIT), 0, st,
@@ -1626,11 +1716,19 @@ static void icp_pass(kifi_model *m, IcpWork *w, const kifi_intrinsics *cam, cons
             }
         }
         if (launched < ip->iters[l]) {
-            auto k = ip->naive_sums ? k_icp_level_tail<false> : k_icp_level_tail<true>;
-            hipLaunchKernelGGL(k, dim3(nb), dim3(kIT), 0, st, lv, m->mp, m->d_pose, m->d_pts, m->d_nrm, w->d_state,
-                               gate, (CSum *)w->partials, l, sp, ip->iters[l] - launched, spin_limit, pass * 3 + l);
+            tail.pending = true;
+            tail.lv = lv;
+            tail.gate = gate;
+            tail.sp = sp;
+            tail.level = l;
+            tail.nb = nb;
+            tail.iters = ip->iters[l] - launched;
+            if (l == 0)
+                launch_tail();
         }
     }
+    if (tail.pending)
+        launch_tail();
 }
 
 // The point-to-plane fallback pass as one launch of k_icp_fallback_pass: a
