@@ -56,15 +56,12 @@ __device__ __forceinline__ void loaded_here(const T &v)
         asm volatile("" ::"v"(w[k]));
 }
 
+// The model sample of a voxel (key, val): false outside the band or behind
+// the near plane.
 template <typename K, typename V>
-__device__ __forceinline__ bool model_sample(const K *keys, const V *vals, uint64_t i, const ModelParams &mp,
-                                             const Mat4 &w2e, float &ex, float &ey, float &z, float &zs,
-                                             float c[3])
+__device__ __forceinline__ bool model_sample_of(K key, const V &val, const ModelParams &mp, const Mat4 &w2e,
+                                                float &ex, float &ey, float &z, float &zs, float c[3])
 {
-    const V val = vals[i];
-    const K key = keys[i];      // both loads in flight before the first test
-    loaded_here(val);
-    loaded_here(key);
     if (!(vw(val) > 0.0f))
         return false;
     const float D = vd(val);
@@ -86,6 +83,18 @@ __device__ __forceinline__ bool model_sample(const K *keys, const V *vals, uint6
     z = depth;
     zs = depth + D;
     return true;
+}
+
+template <typename K, typename V>
+__device__ __forceinline__ bool model_sample(const K *keys, const V *vals, uint64_t i, const ModelParams &mp,
+                                             const Mat4 &w2e, float &ex, float &ey, float &z, float &zs,
+                                             float c[3])
+{
+    const V val = vals[i];
+    const K key = keys[i];      // both loads in flight before the first test
+    loaded_here(val);
+    loaded_here(key);
+    return model_sample_of<K, V>(key, val, mp, w2e, ex, ey, z, zs, c);
 }
 
 template <typename K, typename V>
@@ -122,46 +131,14 @@ __global__ __launch_bounds__(kIT) void k_model_splat(const K *__restrict__ keys,
     }
 }
 
-template <typename K, typename V>
-__global__ __launch_bounds__(kIT) void k_model_resolve(const K *__restrict__ keys, const V *__restrict__ vals,
-                                                       ModelParams mp, const ModelPose *__restrict__ pose,
-                                                       const unsigned long long *__restrict__ zbuf,
-                                                       float4 *__restrict__ pts)
+// The model normal at pixel (x, y) from its point P and the points NB pixels
+// right, left, down and up (central differences), facing the camera; zero at
+// the border, where a neighbour is missing or further than disc from P.
+constexpr int kNrmBase = 4;   // central-difference baseline in pixels
+__device__ __forceinline__ float4 model_normal(const float4 &P, const float4 (&nb)[4], int x, int y, int W, int H,
+                                               const ModelPose *__restrict__ pose, float disc)
 {
-    const int q = blockIdx.x * kIT + threadIdx.x;
-    if (q >= mp.W * mp.H)
-        return;
-    const unsigned long long key = zbuf[q];
-    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (key != ~0ull) {
-        float ex, ey, z, zs, c[3];
-        model_sample<K, V>(keys, vals, (uint32_t)key, mp, pose->w2e, ex, ey, z, zs, c);
-        const float s = zs / z;
-        o.x = pose->cam[0] + (c[0] - pose->cam[0]) * s;
-        o.y = pose->cam[1] + (c[1] - pose->cam[1]) * s;
-        o.z = pose->cam[2] + (c[2] - pose->cam[2]) * s;
-        o.w = 1.0f;
-    }
-    pts[q] = o;
-}
-
-__global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict__ pts, int W, int H,
-                                                       const ModelPose *__restrict__ pose, float disc,
-                                                       float4 *__restrict__ nrm)
-{
-    const int q = blockIdx.x * kIT + threadIdx.x;
-    if (q >= W * H)
-        return;
-    const int y = q / W, x = q - y * W;
-    constexpr int NB = 4;   // central-difference baseline in pixels
-    const int dxs[4] = { NB, -NB, 0, 0 }, dys[4] = { 0, 0, NB, -NB };
-    // the point and its four neighbours in flight together (clamped to the image;
-    // the tests below decide whether they are used)
-    const float4 P = pts[q];
-    float4 nb[4];
-#pragma unroll
-    for (int d = 0; d < 4; ++d)
-        nb[d] = pts[(size_t)min(max(y + dys[d], 0), H - 1) * W + min(max(x + dxs[d], 0), W - 1)];
+    constexpr int NB = kNrmBase;
     float4 N = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (P.w != 0.0f && x >= NB && x < W - NB && y >= NB && y < H - NB) {
         const float disc2 = disc * disc;
@@ -188,7 +165,97 @@ __global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict_
             }
         }
     }
-    nrm[q] = N;
+    return N;
+}
+
+__global__ __launch_bounds__(kIT) void k_model_normals(const float4 *__restrict__ pts, int W, int H,
+                                                       const ModelPose *__restrict__ pose, float disc,
+                                                       float4 *__restrict__ nrm)
+{
+    const int q = blockIdx.x * kIT + threadIdx.x;
+    if (q >= W * H)
+        return;
+    const int y = q / W, x = q - y * W;
+    constexpr int NB = kNrmBase;
+    const int dxs[4] = { NB, -NB, 0, 0 }, dys[4] = { 0, 0, NB, -NB };
+    // the point and its four neighbours in flight together (clamped to the image;
+    // the tests decide whether they are used)
+    const float4 P = pts[q];
+    float4 nb[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+        nb[d] = pts[(size_t)min(max(y + dys[d], 0), H - 1) * W + min(max(x + dxs[d], 0), W - 1)];
+    nrm[q] = model_normal(P, nb, x, y, W, H, pose, disc);
+}
+
+// The model's points and normals in one launch.  A pixel's point is the
+// surface estimate of the voxel its z-buffer entry names: the voxel centre
+// moved along the viewing ray to depth z + D.  A workgroup resolves a
+// kMTile x kMTile tile of pixels and the kNrmBase-pixel ring around it into
+// LDS (every z-buffer read, then every voxel read, in flight together), then
+// writes the tile's points and takes their normals from LDS (k_model_normals'
+// rule).
+constexpr int kMTile = 16;
+constexpr int kMSpan = kMTile + 2 * kNrmBase;
+constexpr int kMRounds = (kMSpan * kMSpan + kIT - 1) / kIT;
+static_assert(kMTile * kMTile == kIT, "one tile pixel per thread");
+
+template <typename K, typename V>
+__global__ __launch_bounds__(kIT) void k_model_points(const K *__restrict__ keys, const V *__restrict__ vals,
+                                                      ModelParams mp, const ModelPose *__restrict__ pose,
+                                                      const unsigned long long *__restrict__ zbuf, float disc,
+                                                      float4 *__restrict__ pts, float4 *__restrict__ nrm)
+{
+    __shared__ float4 sp[kMSpan * kMSpan];
+    const int W = mp.W, H = mp.H;
+    const int x0 = (int)blockIdx.x * kMTile - kNrmBase, y0 = (int)blockIdx.y * kMTile - kNrmBase;
+    const Mat4 w2e = pose->w2e;
+    unsigned long long zk[kMRounds];
+#pragma unroll
+    for (int r = 0; r < kMRounds; ++r) {
+        const int i = r * kIT + (int)threadIdx.x;
+        const int x = x0 + i % kMSpan, y = y0 + i / kMSpan;
+        const bool in = i < kMSpan * kMSpan && x >= 0 && x < W && y >= 0 && y < H;
+        zk[r] = zbuf[in ? (size_t)y * W + x : 0];
+        if (!in)
+            zk[r] = ~0ull;
+    }
+    K kk[kMRounds];
+    V vv[kMRounds];
+#pragma unroll
+    for (int r = 0; r < kMRounds; ++r) {
+        const uint32_t idx = zk[r] != ~0ull ? (uint32_t)zk[r] : 0u;    // index 0: read, unused
+        kk[r] = keys[idx];
+        vv[r] = vals[idx];
+    }
+#pragma unroll
+    for (int r = 0; r < kMRounds; ++r) {
+        const int i = r * kIT + (int)threadIdx.x;
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (zk[r] != ~0ull) {
+            float ex, ey, z, zs, c[3];
+            model_sample_of<K, V>(kk[r], vv[r], mp, w2e, ex, ey, z, zs, c);
+            const float s = zs / z;
+            o.x = pose->cam[0] + (c[0] - pose->cam[0]) * s;
+            o.y = pose->cam[1] + (c[1] - pose->cam[1]) * s;
+            o.z = pose->cam[2] + (c[2] - pose->cam[2]) * s;
+            o.w = 1.0f;
+        }
+        if (i < kMSpan * kMSpan)
+            sp[i] = o;
+    }
+    __syncthreads();
+    const int lx = (int)threadIdx.x % kMTile + kNrmBase, ly = (int)threadIdx.x / kMTile + kNrmBase;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= W || y >= H)
+        return;
+    constexpr int NB = kNrmBase;
+    const float4 P = sp[ly * kMSpan + lx];
+    const float4 nb[4] = { sp[ly * kMSpan + lx + NB], sp[ly * kMSpan + lx - NB], sp[(ly + NB) * kMSpan + lx],
+                           sp[(ly - NB) * kMSpan + lx] };
+    const size_t q = (size_t)y * W + x;
+    pts[q] = P;
+    nrm[q] = model_normal(P, nb, x, y, W, H, pose, disc);
 }
 
 // model pose from a world->eye matrix (host value or device pointer)
@@ -1317,10 +1384,9 @@ static void launch_model(kifi_model *m, kifi_volume *vol, hipStream_t st, bool s
         hipLaunchKernelGGL((k_model_splat<K, V>), dim3(grid), dim3(kIT), 0, st, keys, vals, vol->d_size, mp,
                            m->d_pose, m->d_zbuf);
     }
-    hipLaunchKernelGGL((k_model_resolve<K, V>), dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, keys, vals, mp,
-                       m->d_pose, m->d_zbuf, m->d_pts);
-    hipLaunchKernelGGL(k_model_normals, dim3((P + kIT - 1) / kIT), dim3(kIT), 0, st, m->d_pts, mp.W, mp.H,
-                       m->d_pose, m->disc, m->d_nrm);
+    const dim3 tiles((unsigned)((mp.W + kMTile - 1) / kMTile), (unsigned)((mp.H + kMTile - 1) / kMTile));
+    hipLaunchKernelGGL((k_model_points<K, V>), tiles, dim3(kIT), 0, st, keys, vals, mp, m->d_pose, m->d_zbuf,
+                       m->disc, m->d_pts, m->d_nrm);
 }
 
 // Model pose of this prediction and a cleared z-buffer.
