@@ -118,7 +118,7 @@ size_t      icp_work_bytes(int W, int H);                        // work buffer 
 void        icp_work_carve(IcpWork *w, void *buf, int W, int H); // the levels and partials in buf
 void        icp_free(IcpWork *w);
 // defer_end: the frame's end (icp_frame_end_dev) is left to the caller's next
-// kernel (the integration's k_frame_begin), one launch fewer per tracked frame.
+// kernel (the integration's k_keygen, workgroup 0), one launch fewer per tracked frame.
 kifi_status icp_run(kifi_model *m, IcpWork *w, DepthIn depth, const kifi_intrinsics *cam,
                     const double *guess_host, const double *guess_dev, const kifi_icp_params *ip,
                     Mat4 *w2e_out = nullptr, bool defer_end = false);

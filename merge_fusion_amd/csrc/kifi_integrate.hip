@@ -50,38 +50,55 @@ __device__ void clip_pre(const Mat4 &w2c, const VolumeConsts &vc, ClipPre &p)
     p.eps = 64.0f * 0x1p-24f * smax;
 }
 
-// Frame start: per-frame counters and radix totals cleared (a fill kernel, never
-// a memset node), and the frame's matrices from worldToEye (Integrator.cpp:41-42).
-// With model_out, also the model pose of the prediction this frame splats
-// (k_model_begin's, from the same matrices).  With icp_end, first the end of
-// the frame's tracking, which writes the worldToEye read here.
-__global__ void k_frame_begin(FrameCounters *c, uint32_t *totals, int n_totals, Mat4 clip, Mat4 w2e_val,
-                              Mat4 *__restrict__ w2e_dev, FrameMats *__restrict__ out, VolumeConsts vc,
-                              ModelPose *__restrict__ model_out, IcpState *__restrict__ icp_end)
+// The frame's matrices (Integrator.cpp:41-42: worldToClip = EyeToClipRH() *
+// worldToEye, eyeToWorld = invert_transform(worldToEye)), in binary32 on the
+// device so that a device-resident pose (tracking) and a host pose give the
+// same bits.  worldToEye is the tracker's pose inverted when the frame ends a
+// tracking (icp_frame_end_dev writes the same matrix), else the device matrix,
+// else the host one.
+struct FrameStart {
+    Mat4 clip, w2e_val;
+    Mat4 *w2e_dev;           // device worldToEye (written here when icp_end is set)
+    FrameMats *mats;         // out: this frame's matrices
+    ModelPose *model_out;    // out when set: the model pose of the prediction this frame splats
+    IcpState *icp_end;       // when set: the tracking this frame ends
+    FrameCounters *cnt;
+};
+
+__device__ __forceinline__ Mat4 frame_w2e(const FrameStart &fs)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_totals; i += gridDim.x * blockDim.x)
-        totals[i] = 0;
-    if (blockIdx.x != 0 || threadIdx.x != 0)
-        return;
-    if (icp_end)
-        icp_frame_end_dev(icp_end, w2e_dev);
+    if (fs.icp_end) {
+        Mat4 m = fs.icp_end->pose_f;
+        invert_transform(m.m);
+        return m;
+    }
+    return fs.w2e_dev ? *fs.w2e_dev : fs.w2e_val;
+}
+
+// The frame's start, one thread: the end of the tracking (when icp_end), the
+// per-frame counters, the matrices and the model pose.
+__device__ void frame_start(const FrameStart &fs, const Mat4 &w2e, const VolumeConsts &vc)
+{
+    if (fs.icp_end)
+        icp_frame_end_dev(fs.icp_end, fs.w2e_dev);
+    FrameCounters *c = fs.cnt;
     c->n_cand = 0; c->n_keys = 0; c->n_unique = 0; c->n_new = 0;
     c->v_new = 0; c->n_tiles = 0; c->overflow = 0;
     FrameMats f;
-    f.w2e = w2e_dev ? *w2e_dev : w2e_val;
-    mat_mul(clip.m, f.w2e.m, f.w2c.m);
+    f.w2e = w2e;
+    mat_mul(fs.clip.m, f.w2e.m, f.w2c.m);
     f.e2w = f.w2e;
     invert_transform(f.e2w.m);
     clip_pre(f.w2c, vc, f.pre);
-    *out = f;
-    if (model_out) {
+    *fs.mats = f;
+    if (fs.model_out) {
         ModelPose mp;
         mp.w2e = f.w2e;
         mp.proj = f.w2c;
         mp.cam[0] = f.e2w.m[12];
         mp.cam[1] = f.e2w.m[13];
         mp.cam[2] = f.e2w.m[14];
-        *model_out = mp;
+        *fs.model_out = mp;
     }
 }
 
@@ -90,14 +107,11 @@ struct KeygenParams {
     int   W, H;
     float flInvX, flInvY, ppX, ppY, maxIndex;
     VolumeConsts vc;
-    const FrameMats *mats;   // device: e2w of this frame
+    FrameStart fs;           // the frame's pose and its start (workgroup 0)
     unsigned long long *zbuf;   // when set: the model z-buffer of nz pixels, cleared to ~0 (a fill, never a memset)
     int   nz;
 };
 
-// Integrator.cpp:41-42: worldToClip = EyeToClipRH() * worldToEye and
-// eyeToWorld = invert_transform(worldToEye), in binary32 on the device so that a
-// device-resident pose (tracking) and a host pose give the same bits.
 
 
 // Integrator::DepthMap2PointCloud (Integrator.cpp:76-182) per pixel, followed by
@@ -106,7 +120,7 @@ struct KeygenParams {
 // pixel_key: the key of pixel pix at depth d metres (Integrator.cpp:109-178);
 // false when it emits none.
 template <typename K>
-__device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, int pix, K &key)
+__device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, const Mat4 &e2w, int pix, K &key)
 {
     const int v = pix / p.W, u = pix - v * p.W;
     if (!(d > 0.0f))
@@ -120,7 +134,7 @@ __device__ __forceinline__ bool pixel_key(float d, const KeygenParams &p, int pi
     q[1] = q[1] * d + 0.0f;
     q[2] = q[2] * d + 0.0f;
     q[3] = q[3] * d + 1.0f;
-    mat_vec(p.mats->e2w, q, q);                                   // :141-144
+    mat_vec(e2w, q, q);                                           // :141-144
     const float ix = q[0] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;   // :146-149
     const float iy = q[1] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
     const float iz = q[2] * p.vc.vlInv + p.vc.minOverNegVl - 0.5f;
@@ -139,7 +153,10 @@ static_assert(kKgItems * kThreads * 8 == kRsTile, "a keygen workgroup fills at m
 // Workgroup t writes its keys to radix tile t (out + t kRsTile; its key count
 // | valid pixels << 16 to tile_n[t]) and the tile's counts of the first radix
 // digit to counts[d Tb + t], so the sort's first pass needs no upsweep.  No
-// counter atomics: the first pass's scan sums tile_n into the frame's counts.  The four keys of a half-block
+// counter atomics: the first pass's scan sums tile_n into the frame's counts.
+// Every workgroup derives the frame's eyeToWorld itself (frame_w2e, the same
+// operations as frame_start), and workgroup 0 also starts the frame, so the
+// frame needs no launch of its own before this one.  The four keys of a half-block
 // differ only in y and z, above the first digit's bits, so each thread adds to
 // at most two digit counters.
 template <typename K, typename D, int DB>
@@ -154,9 +171,18 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
     static_assert(KeyTraits<K>::kb >= DB, "the y and z offsets leave the first digit unchanged");
     __shared__ uint32_t tmp[kThreads / kWave + 1];
     __shared__ uint32_t hist[ND];
+    __shared__ Mat4 s_e2w;
     const int P = p.W * p.H;
     for (int q = threadIdx.x; q < ND; q += kThreads)
         hist[q] = 0;
+    if (threadIdx.x == 0) {
+        const Mat4 w2e = frame_w2e(p.fs);
+        Mat4 e2w = w2e;
+        invert_transform(e2w.m);
+        s_e2w = e2w;
+        if (blockIdx.x == 0)
+            frame_start(p.fs, w2e, p.vc);
+    }
     if (p.zbuf)
         for (int i = blockIdx.x * kThreads + threadIdx.x; i < p.nz; i += gridDim.x * kThreads)
             p.zbuf[i] = ~0ull;
@@ -168,13 +194,15 @@ __global__ __launch_bounds__(kThreads) void k_keygen(const D *__restrict__ depth
 #pragma unroll
     for (int j = 0; j < kKgItems; ++j)
         dm[j] = depth_m(depth, (size_t)min((blockIdx.x * kKgItems + j) * kThreads + threadIdx.x, P - 1));
+    __syncthreads();     // s_e2w
+    const Mat4 e2w = s_e2w;
 #pragma unroll
     for (int j = 0; j < kKgItems; ++j) {
         const int pix = (blockIdx.x * kKgItems + j) * kThreads + threadIdx.x;
         if (dm_out && pix < P)
             dm_out[pix] = dm[j];                                  // the metre frame of the update
         K k = 0;
-        const bool ok = pix < P && pixel_key<K>(dm[j], p, pix, k);   // DepthStream.cpp:89-90
+        const bool ok = pix < P && pixel_key<K>(dm[j], p, e2w, pix, k);   // DepthStream.cpp:89-90
         keys[j] = k;
         flags |= (uint32_t)ok << j;
         c += ok;
@@ -1592,9 +1620,6 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     for (int i = 0; i < 16; ++i)
         wval.m[i] = w2e ? w2e[i] : 0.0f;
     const bool model_start = splat && splat->pose_out;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(256), 0, stream, cnt, s.rs_totals, passes * ND, clip, wval,
-                       const_cast<Mat4 *>(w2e_dev), vol->d_mats, vol->consts,
-                       model_start ? splat->pose_out : nullptr, splat ? splat->icp_end : nullptr);
 
     KeygenParams kp;
     kp.W = W; kp.H = H;
@@ -1604,7 +1629,13 @@ static kifi_status integrate_impl(kifi_volume *vol, DepthIn d_depth, const kifi_
     kp.ppY = (cam->cy - 0.5f) * kp.flInvY;
     kp.maxIndex = (float)(vol->desc.res - 1);
     kp.vc = vol->consts;
-    kp.mats = vol->d_mats;
+    kp.fs.clip = clip;
+    kp.fs.w2e_val = wval;
+    kp.fs.w2e_dev = const_cast<Mat4 *>(w2e_dev);
+    kp.fs.mats = vol->d_mats;
+    kp.fs.model_out = model_start ? splat->pose_out : nullptr;
+    kp.fs.icp_end = splat ? splat->icp_end : nullptr;
+    kp.fs.cnt = cnt;
     kp.zbuf = model_start ? splat->zbuf : nullptr;
     kp.nz = model_start ? splat->W * splat->H : 0;
     const int t_sort = timer_begin(ctx, KIFI_STAGE_SORT);

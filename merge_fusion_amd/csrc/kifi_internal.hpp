@@ -188,12 +188,13 @@ struct SplatParams {
     unsigned long long *zbuf;        // W*H, cleared to ~0
     int   W, H;
     float fx, fy, cx, cy, vl, c0, dmax, znear;
-    // When set, the integration also starts the prediction: k_frame_begin
-    // writes the model pose of this frame (the camera is the integration's)
-    // and k_keygen clears zbuf.  Null when the caller has done both.
+    // When set, the integration also starts the prediction: k_keygen writes
+    // the model pose of this frame (the camera is the integration's) and
+    // clears zbuf.  Null when the caller has done both.
     ModelPose *pose_out;
-    // When set, k_frame_begin first ends the frame's tracking (the tracker's
-    // state, its worldToEye into w2e) before it reads w2e.
+    // When set, the integration's k_keygen ends the frame's tracking (the
+    // tracker's state, its worldToEye into w2e) and takes the frame's
+    // worldToEye from the tracker's pose.
     IcpState *icp_end;
 };
 

@@ -607,7 +607,7 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
         twistnet_reset_frames(p->net);
     kifi_status s = KIFI_OK;
     bool tracked = false;     // icp_run wrote this frame's worldToEye
-    bool defer_end = false;   // ... or k_frame_begin writes it, ending the tracking
+    bool defer_end = false;   // ... or the integration's keygen writes it, ending the tracking
     if (w2e) {
         // Pipeline.cpp:32-35: integrate at the given pose
         icp_set_pose(&p->work, nullptr, w2e, st);
@@ -633,7 +633,7 @@ static kifi_status pipeline_integrate(kifi_pipeline *p, const void *depth, bool 
         if (p->have_guess && net)
             twistnet_observe(net, d_depth, p->cam.width, p->cam.height, st);
         // a frame fused at its tracked pose ends its tracking in the
-        // integration's first kernel (k_frame_begin)
+        // integration's first kernel (k_keygen's frame start)
         defer_end = !fuse_w2e;
         s = icp_run(p->model, &p->work, din, &p->cam, p->have_guess ? p->guess : nullptr,
                     p->have_guess ? nullptr : guess_dev, &ip, p->d_w2e, defer_end);
