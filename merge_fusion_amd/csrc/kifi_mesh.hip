@@ -291,13 +291,15 @@ __device__ __forceinline__ uint32_t voxel_topology(const K *keys, const V *vals,
     return flags;
 }
 
-// Pass 1: per-voxel vertex flags and counts, a tile of kMeshTile voxels at a time.
+// The indexed mesh (Mesh(volume, verts, idx)), pass 1: per-voxel vertex flags,
+// vertex counts (forward: the vertex IDs' order) and index counts (reverse:
+// generation order), a tile of kMeshTile voxels at a time.  TRIS selects the
+// cube corners the topology reads (the vertex-only mesh has kernels of its own).
 template <typename K, typename V, bool TRIS>
 __global__ __launch_bounds__(kMT) void k_mesh_count(const K *__restrict__ keys, const V *__restrict__ vals,
                                                     uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
                                                     uint8_t *__restrict__ vflags,
                                                     uint32_t *__restrict__ cnt_v_fwd,
-                                                    uint32_t *__restrict__ cnt_v_rev,
                                                     uint32_t *__restrict__ cnt_i_rev)
 {
     __shared__ MeshWin<K, TRIS> mw;
@@ -311,10 +313,7 @@ __global__ __launch_bounds__(kMT) void k_mesh_count(const K *__restrict__ keys, 
             const uint32_t f = voxel_topology<K, V, TRIS>(keys, vals, n, i, mw, wr, cn, lut);
             const uint32_t nv = __popc(f);
             vflags[i] = (uint8_t)f;
-            if (cnt_v_fwd)
-                cnt_v_fwd[i] = nv;
-            if (cnt_v_rev)
-                cnt_v_rev[m - 1 - i] = nv;
+            cnt_v_fwd[i] = nv;
             if (TRIS)
                 cnt_i_rev[m - 1 - i] = lut >= 0 ? (uint32_t)(kTriOffsets[lut + 1] - kTriOffsets[lut]) : 0u;
         }
@@ -329,13 +328,14 @@ __constant__ int kEdgeAxis[12] = { 0, 2, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1 };
 __constant__ int kCornerOff[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 1, 1, 0 },
                                       { 0, 0, 1 }, { 1, 0, 1 }, { 0, 1, 1 }, { 1, 1, 1 } };
 
-// Pass 2: emit vertices and (optionally) indices at their scanned positions.
+// The indexed mesh, pass 2: vertices (in vertex-ID order) and indices at their
+// scanned positions.
 template <typename K, typename V, bool TRIS>
 __global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, const V *__restrict__ vals,
                                                    uint64_t n, const uint64_t *__restrict__ win, uint64_t T,
                                                    MeshParams mp,
                                                    const uint8_t *__restrict__ vflags,
-                                                   const uint32_t *__restrict__ voff,   // fwd (TRIS) or rev
+                                                   const uint32_t *__restrict__ voff,   // first vertex per voxel (ID order)
                                                    const uint32_t *__restrict__ ioff_rev,
                                                    float *__restrict__ verts, uint32_t *__restrict__ idx,
                                                    uint64_t *__restrict__ raw_ids,
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kMT) void k_mesh_emit(const K *__restrict__ keys, c
                         vz = (float)(int)z0 * mp.vl + mp.c0;
             const V v0 = vals[i];
             const float dself = vdist(v0);
-            uint32_t o = TRIS ? voff[i] : voff[m - 1 - i];
+            uint32_t o = voff[i];
             const int axis_corner[3] = { 1, 2, 4 };
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -801,7 +801,7 @@ static kifi_status mesh_impl(kifi_volume *vol, bool tris, float *verts, size_t v
     if (tris) {
         hipLaunchKernelGGL((k_mesh_windows<K, true>), dim3(wgrid), dim3(kMT), 0, st, keys, n, T, win);
         hipLaunchKernelGGL((k_mesh_count<K, V, true>), dim3(grid), dim3(kMT), 0, st, keys, vals, n, win, T,
-                           vflags, cv_fwd, (uint32_t *)nullptr, ci_rev);
+                           vflags, cv_fwd, ci_rev);
         device_scan(st, cv_fwd, cv_fwd, m, tsum, totals);
         device_scan(st, ci_rev, ci_rev, m, tsum, totals + 1);
     } else {
