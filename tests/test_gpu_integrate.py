@@ -224,3 +224,34 @@ def test_invalid_arguments_are_refused(ctx):
     v.integrate(d, CAM, np.eye(4, dtype=np.float32).reshape(16))
     assert v.size() > 100000
     v.close()
+
+
+def test_running_totals_match_the_oracle(ctx):
+    """kifi_integrate_totals over four room frames against the oracle: the number
+    of integrations, the sums of V0, V1, E (new voxels) and K (valid pixels),
+    and A, the voxels the update changed.  With binary32 storage an update
+    always raises a voxel's weight, so the oracle's A is the count of voxels
+    whose (sum, weight) differ from before the frame (a new voxel from the
+    default Voxel()).  A is summed on the device over many counters."""
+    gv = mf.Volume(ctx, 1024, 4.096, 0.02, 32, mf.STORAGE_F32)
+    ov = orc.Volume(1024, 4.096, 0.02, 10, mf.STORAGE_F32)
+    t0 = gv.totals()
+    exp = dict(frames=0, v_old=0, new=0, v_new=0, keys=0, updated=0)
+    for f in (0, 3, 6, 9):
+        d, w2e = frame(mf.SCENE_ROOM, 2, mf.TRAJ_CIRCLE, f)
+        k0, v0 = ov.read()
+        gv.integrate(d, CAM, w2e)
+        st = ov.integrate(d, CAM, w2e)
+        k1, v1 = ov.read()
+        prev = np.zeros((len(k1), 2), np.uint32)                # Voxel(): sum 0, weight 0
+        prev[np.searchsorted(k1, k0)] = _vals_bits(v0, mf.STORAGE_F32).reshape(len(k0), 2)
+        cur = _vals_bits(v1, mf.STORAGE_F32).reshape(len(k1), 2)
+        exp["frames"] += 1
+        exp["v_old"] += len(k0)
+        exp["v_new"] += len(k1)
+        exp["new"] += st["new"]
+        exp["keys"] += st["keys"]
+        exp["updated"] += int(np.any(cur != prev, axis=1).sum())
+    got = {k: v - t0[k] for k, v in gv.totals().items()}
+    assert got == exp
+    assert exp["updated"] > 0
