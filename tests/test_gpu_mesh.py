@@ -155,3 +155,55 @@ def test_mesh_async_overlaps_later_frames(ctx):
     assert np.array_equal(buf2[:len(o2)].view(np.uint32), o2.view(np.uint32))
     assert np.array_equal(pageable[:len(o2)].view(np.uint32), o2.view(np.uint32))
     assert len(o2) != len(o1)
+
+
+@pytest.mark.parametrize("key_bits,storage", [(32, mf.STORAGE_F32), (32, mf.STORAGE_F16U8),
+                                              (64, mf.STORAGE_F32), (64, mf.STORAGE_F16U8)])
+def test_mesh_wide_neighbour_windows(ctx, key_bits, storage):
+    """A volume whose +z key windows exceed the LDS window (2048 keys): a sparse
+    z-slice under a dense 64 x 64 block one slice up, so every tile of the
+    sparse slice searches its +z neighbours in global memory.  Vertices (both
+    Mesh overloads, and the asynchronous copy) equal the oracle's bit for bit."""
+    rng = np.random.default_rng(7 + key_bits + storage)
+    kb = 10 if key_bits == 32 else 21
+    res = 1024 if key_bits == 32 else 2048
+    z0, x0, y0 = 300, 400, 500
+    gx, gy = np.meshgrid(np.arange(64), np.arange(64))
+    dense = np.stack([gx.ravel() + x0, gy.ravel() + y0, np.full(4096, z0 + 1)], 1)
+    pick = rng.choice(4096, size=1500, replace=False)
+    sparse = dense[pick].copy()
+    sparse[:, 2] = z0
+    # a few x- and y-neighbours in the sparse slice too
+    extra = sparse[:300].copy()
+    extra[:150, 0] += 1
+    extra[150:, 1] += 1
+    xyz = np.concatenate([sparse, extra, dense]).astype(np.int64)
+    keys = np.unique((xyz[:, 2] << (2 * kb)) | (xyz[:, 1] << kb) | xyz[:, 0]).astype(np.uint64)
+    n = keys.size
+    w = rng.integers(1, 20, size=n)
+    dist = rng.uniform(-0.02, 0.02, size=n).astype(np.float32)
+    if storage == mf.STORAGE_F32:
+        ovals = np.stack([dist * w.astype(np.float32), w.astype(np.float32)], 1).astype(np.float32)
+    else:
+        ovals = (dist.astype(np.float16).view(np.uint16).astype(np.uint32) | (w.astype(np.uint32) << 16))
+    gv = mf.Volume(ctx, res, 8.192, 0.04, key_bits, storage)
+    ov = orc.Volume(res, 8.192, 0.04, kb, storage)
+    ov.write(keys, ovals)
+    if storage == mf.STORAGE_F32:
+        gvals = np.zeros(n, gv.val_dtype)
+        gvals["sum"], gvals["weight"] = ovals[:, 0], ovals[:, 1]
+    else:
+        gvals = ovals.astype(np.uint32)
+    gv.upload(keys.astype(gv.key_dtype), gvals)
+    o = ov.mesh()
+    assert len(o) > 1000
+    g = gv.mesh()
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    buf = ctx.host_empty((len(o) + 1, 3), np.float32)
+    assert gv.mesh_into_async(buf) == len(o)
+    ctx.mesh_wait()
+    assert np.array_equal(buf[:len(o)].view(np.uint32), o.view(np.uint32))
+    gvtx, gidx, gst = gv.mesh_indexed()
+    ovtx, oidx, ost = ov.mesh_indexed()
+    assert gst == ost and np.array_equal(gvtx.view(np.uint32), ovtx.view(np.uint32))
+    assert np.array_equal(gidx, oidx)
